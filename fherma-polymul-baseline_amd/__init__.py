@@ -1,0 +1,205 @@
+"""Python mirror of the MI355X exact negacyclic product (C-ABI: include/polymul.h).
+
+The reference exposes one C++ entry-point triple (fherma.h:46-48):
+fherma_init(Point) -> state, fherma_run(state, Inputs) -> Outputs,
+fherma_free(state).  `Solver` mirrors it one to one (init / run / free with
+the same argument meaning and the same failure behaviour: a RuntimeError
+naming the operation), on top of `Context`, which binds the C-ABI.
+
+The product path is the HIP library `libpolymul.so` built next to this file;
+nothing here falls back to a CPU implementation -- if the library is absent,
+loading fails loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import workload  # noqa: F401  (seeded synthetic inputs)
+
+LIMBS = 28
+WIDTH = 868
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+PM_OK, PM_EINVAL, PM_EUNSUPPORTED, PM_EDEVICE, PM_ENOMEM, PM_EBUDGET, PM_ETIMEOUT = range(7)
+
+
+class PolymulError(RuntimeError):
+    def __init__(self, operation, status, reason):
+        super().__init__(f"{operation} failed (status {status}): {reason}")
+        self.status = status
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("max_n", ctypes.c_uint32), ("k", ctypes.c_uint32), ("product_bits", ctypes.c_uint32),
+                ("device", ctypes.c_uint32), ("table_bytes", ctypes.c_uint64),
+                ("workspace_bytes", ctypes.c_uint64)]
+
+
+def library_path():
+    return os.path.join(_HERE, "libpolymul.so")
+
+
+def library():
+    """Load libpolymul.so (raises OSError when it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with __graft_entry__.build() or make -C {_HERE}")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    u32 = ctypes.c_uint32
+    lib.pm_create.argtypes = [u32, u32, u32, P, ctypes.c_int, ctypes.POINTER(P)]
+    lib.pm_destroy.argtypes = [P]
+    lib.pm_destroy.restype = None
+    lib.pm_last_error.argtypes = [P]
+    lib.pm_last_error.restype = ctypes.c_char_p
+    lib.pm_plan_rns.argtypes = [u32, P, ctypes.POINTER(u32), P, P, ctypes.POINTER(u32)]
+    lib.pm_get_info.argtypes = [P, ctypes.POINTER(_Info)]
+    lib.pm_get_prime.argtypes = [P, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+    lib.pm_multiply.argtypes = [P, u32, P, P, P]
+    lib.pm_multiply_device.argtypes = [P, u32, P, P, P, P]
+    lib.pm_synchronize.argtypes = [P]
+    lib.pm_stage_convert.argtypes = [P, u32, P, P, P, P, P]
+    lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
+    lib.pm_stage_crt.argtypes = [P, u32, P, P, P]
+    for name in EXPORTED:
+        getattr(lib, name)  # every declared symbol must resolve
+    _LIB = lib
+    return lib
+
+
+# every extern "C" entry point declared in include/polymul.h
+EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
+            "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_ntt", "pm_stage_crt"]
+
+
+def _ptr(x):
+    """Address of a numpy array or a torch tensor (device or host)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    return x.ctypes.data
+
+
+def plan_rns(q, max_n):
+    """Host-only RNS plan: (k, primes, psi_max, product_bits); needs no GPU."""
+    lib = library()
+    q = np.ascontiguousarray(np.asarray(q, dtype=np.uint32).reshape(LIMBS))
+    k, bits = ctypes.c_uint32(), ctypes.c_uint32()
+    primes = np.zeros(64, dtype=np.uint32)
+    psi = np.zeros(64, dtype=np.uint32)
+    status = lib.pm_plan_rns(max_n, q.ctypes.data, ctypes.byref(k), primes.ctypes.data, psi.ctypes.data,
+                             ctypes.byref(bits))
+    if status != PM_OK:
+        raise PolymulError("pm_plan_rns", status, lib.pm_last_error(None).decode())
+    return k.value, primes[:k.value].copy(), psi[:k.value].copy(), bits.value
+
+
+class Context:
+    """An RNS product context for one modulus q (pm_create / pm_destroy)."""
+
+    def __init__(self, q, max_n, device=0):
+        lib = library()
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.uint32).reshape(LIMBS))
+        handle = ctypes.c_void_p()
+        status = lib.pm_create(max_n, WIDTH, LIMBS, q.ctypes.data, device, ctypes.byref(handle))
+        if status != PM_OK:
+            raise PolymulError("pm_create", status, lib.pm_last_error(None).decode())
+        self._lib, self._h, self.q = lib, handle, q.copy()
+
+    def _check(self, operation, status):
+        if status != PM_OK:
+            raise PolymulError(operation, status, self._lib.pm_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        out = _Info()
+        self._check("pm_get_info", self._lib.pm_get_info(self._h, ctypes.byref(out)))
+        return {f: getattr(out, f) for f, _ in _Info._fields_}
+
+    def primes(self):
+        """[(p_i, psi_max_i)] of the RNS."""
+        out = []
+        for i in range(self.info()["k"]):
+            p, psi = ctypes.c_uint32(), ctypes.c_uint32()
+            self._check("pm_get_prime", self._lib.pm_get_prime(self._h, i, ctypes.byref(p), ctypes.byref(psi)))
+            out.append((p.value, psi.value))
+        return out
+
+    def multiply(self, a, b):
+        """Host arrays [n, 28] uint32 -> c [n, 28] uint32 (pm_multiply)."""
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        if a.shape != b.shape or a.ndim != 2 or a.shape[1] != LIMBS:
+            raise ValueError("a and b must both be [n, 28] uint32")
+        c = np.empty_like(a)
+        self._check("pm_multiply", self._lib.pm_multiply(self._h, a.shape[0], a.ctypes.data, b.ctypes.data,
+                                                          c.ctypes.data))
+        return c
+
+    def multiply_device(self, n, a, b, c, stream=None):
+        self._check("pm_multiply_device",
+                    self._lib.pm_multiply_device(self._h, n, _ptr(a), _ptr(b), _ptr(c), stream))
+
+    def synchronize(self):
+        self._check("pm_synchronize", self._lib.pm_synchronize(self._h))
+
+    def stage_convert(self, n, a, b, ra, rb, stream=None):
+        self._check("pm_stage_convert",
+                    self._lib.pm_stage_convert(self._h, n, _ptr(a), _ptr(b), _ptr(ra), _ptr(rb), stream))
+
+    def stage_ntt(self, n, ra, rb, outer=0, stream=None):
+        self._check("pm_stage_ntt", self._lib.pm_stage_ntt(self._h, n, outer, _ptr(ra), _ptr(rb), stream))
+
+    def stage_crt(self, n, t, c, stream=None):
+        self._check("pm_stage_crt", self._lib.pm_stage_crt(self._h, n, _ptr(t), _ptr(c), stream))
+
+
+class Solver:
+    """fherma_init / fherma_run / fherma_free (fherma.h:46-48) in Python."""
+
+    def __init__(self, N, W, L, q):
+        if N < 2 or N & (N - 1) or N > 131072 or W != WIDTH or L != LIMBS or len(q) != LIMBS:
+            raise RuntimeError("coverage: power-of-two 2<=N<=131072, W=868, L=28")
+        self.N = N
+        self._ctx = Context(q, N)
+
+    def run(self, a, b):
+        a = np.asarray(a, dtype=np.uint32).reshape(-1, LIMBS)
+        b = np.asarray(b, dtype=np.uint32).reshape(-1, LIMBS)
+        if a.shape != b.shape:
+            raise RuntimeError("fherma_run: a and b must both hold N x 28 words")
+        try:
+            return self._ctx.multiply(a, b)
+        except PolymulError as e:
+            raise RuntimeError(f"fherma_run: {e}") from e
+
+    def free(self):
+        self._ctx.close()

@@ -1,0 +1,53 @@
+// fherma.h entry points (fherma.h:46-48) on top of the polymul C-ABI.
+//
+// fherma_init  -- coverage checks and every input-independent step: RNS
+//                 parameters, resident tables, device workspace
+//                 (role of rns/solve.cu:581-718);
+// fherma_run   -- the timed product: inputs are read, not retained; the
+//                 returned Outputs owns a fresh vector (rns/solve.cu:721-853);
+// fherma_free  -- releases the state (rns/solve.cu:854).
+// Failures throw std::runtime_error naming the operation, as the reference
+// does (rns/solve.cu:421-423); the harness records them per case.
+#include <stdexcept>
+#include <string>
+
+#include "fherma.h"
+#include "polymul.h"
+
+namespace {
+
+[[noreturn]] void raise(const char* operation, const pm_context* ctx, int status) {
+    throw std::runtime_error(std::string(operation) + " failed (status " + std::to_string(status) +
+                             "): " + pm_last_error(ctx));
+}
+
+}  // namespace
+
+void* fherma_init(const fherma::Point& p) {
+    if (p.N < 2 || (p.N & (p.N - 1)) || p.N > 131072 || p.W != PM_WIDTH || p.L != PM_LIMBS ||
+        p.q.data.size() != PM_LIMBS)
+        throw std::runtime_error("coverage: power-of-two 2<=N<=131072, W=868, L=28");
+    pm_context* ctx = nullptr;
+    const int status = pm_create(p.N, p.W, p.L, p.q.data.data(), 0, &ctx);
+    if (status != PM_OK) raise("fherma_init", nullptr, status);
+    return ctx;
+}
+
+fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
+    auto* ctx = static_cast<pm_context*>(state);
+    if (!ctx) throw std::runtime_error("fherma_run: null state");
+    pm_info_t info{};
+    pm_get_info(ctx, &info);
+    const size_t words = in.a.data.size();
+    if (words == 0 || words % PM_LIMBS != 0 || in.b.data.size() != words)
+        throw std::runtime_error("fherma_run: a and b must both hold N x 28 words");
+    const uint32_t n = uint32_t(words / PM_LIMBS);
+    fherma::Outputs out;
+    out.c.shape = {int64_t(n), int64_t(PM_LIMBS)};
+    out.c.data.resize(words);
+    const int status = pm_multiply(ctx, n, in.a.data.data(), in.b.data.data(), out.c.data.data());
+    if (status != PM_OK) raise("fherma_run", ctx, status);
+    return out;
+}
+
+void fherma_free(void* state) { pm_destroy(static_cast<pm_context*>(state)); }

@@ -1,0 +1,51 @@
+// Device-side data shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace pmd {
+
+constexpr uint32_t kLimbs = 28;
+constexpr uint32_t kCrtLimbs = 29;
+constexpr uint32_t kWaveSize = 64;
+constexpr uint32_t kBlock = 1024;          // threads of NTT / persistent blocks
+constexpr uint32_t kTileElems = 32768;     // residues held by one NTT block (32 per thread)
+constexpr uint32_t kLdsWords = kTileElems + kTileElems / 32;  // padded exchange buffer
+constexpr uint32_t kConvPerWave = 64;      // coefficients per wave in the conversion
+constexpr uint32_t kConvPerBlock = kConvPerWave * (kBlock / kWaveSize);  // 1024
+constexpr uint32_t kCrtPerBlock = 256;
+
+struct Tw {
+    uint32_t value, shoup;
+};
+
+// Resident per-modulus tables (device pointers), passed by value to kernels.
+struct Tables {
+    const uint32_t* primes;       // [k]
+    const uint32_t* mont;         // [k] -p^-1 mod 2^32
+    const Tw* fwd;                // [k][max_n] psi^brv(i)
+    const Tw* inv;                // [k][max_n] psi^-brv(i)
+    const Tw* scale;              // [max_logn+1][k]
+    const uint4* conv_w;          // [quads][2][64] int8x16 weight operands
+    const int32_t* conv_corr;     // [quads*4][4]
+    const uint32_t* conv_sh16;    // [k] floor(2^48/p)
+    const uint32_t* crt_m30;      // [k][29]
+    const uint32_t* crt_round;    // [k] floor(2^42/p)
+    const uint32_t* crt_negkp;    // [k+1][28]
+    const uint32_t* q;            // [28]
+    uint32_t mu0, mu1;            // floor(2^908/q)
+    uint32_t k, quads, max_logn, pad;
+};
+
+// One product of the batch / the plain path.
+struct Job {
+    uint32_t logn;
+    uint32_t pad;
+    const uint32_t* a;   // [n][28] coefficients (device)
+    const uint32_t* b;
+    uint32_t* c;         // [n][28] result (device)
+    uint32_t* ra;        // [k][n] residues of a, reused for CRT inputs
+    uint32_t* rb;        // [k][n] residues of b
+};
+
+}  // namespace pmd

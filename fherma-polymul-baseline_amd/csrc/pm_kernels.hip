@@ -1,0 +1,608 @@
+// gfx950 kernels for the exact 868-bit negacyclic product over a 30-bit RNS.
+//
+//  * residue conversion: exact int8 x int8 -> int32 MFMA contraction of the
+//    coefficient bytes against balanced base-256 digits of 256^j mod p_i
+//    (role of rns/tensor_prepare.h:20-71 and rns/solve.cu:138-157);
+//  * fused per-prime convolution tile: Harvey lazy odd-root Cooley-Tukey
+//    forward NTT of a and b, Montgomery pointwise product, Gentleman-Sande
+//    inverse, scale -- spectra stay in registers/LDS (role of
+//    quartic/solve.cu:641-770 and rns/solve.cu:225-333);
+//  * outer radix-2 stages for N > 2^15 (role of quartic/solve.cu:771-817);
+//  * exact CRT with 30-bit limb accumulation, nearest-integer quotient and a
+//    Barrett reduction modulo any odd q (role of rns/solve.cu:336-420).
+// All arithmetic is exact 32/64-bit integer arithmetic.
+#include <hip/hip_runtime.h>
+
+#include "pm_device.h"
+#include "pm_kernels.h"
+
+namespace pmd {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// small-modulus primitives (p < 2^30)
+// ---------------------------------------------------------------------------
+// a*w mod p, result in [0,2p), for any 32-bit a (Shoup).
+__device__ __forceinline__ uint32_t shoup(uint32_t a, Tw w, uint32_t p) {
+    const uint32_t quotient = __umulhi(a, w.shoup);
+    return a * w.value - quotient * p;
+}
+// x < 2m -> x mod m   (x - m wraps above x when x < m)
+__device__ __forceinline__ uint32_t csub(uint32_t x, uint32_t m) {
+    const uint32_t t = x - m;
+    return t < x ? t : x;
+}
+// Cooley-Tukey: X in [0,4p), Y any -> both in [0,4p)
+__device__ __forceinline__ void ct_bfly(uint32_t& X, uint32_t& Y, Tw w, uint32_t p, uint32_t p2) {
+    const uint32_t x = csub(X, p2);
+    const uint32_t t = shoup(Y, w, p);
+    X = x + t;
+    Y = x - t + p2;
+}
+// Gentleman-Sande: X, Y in [0,2p) -> both in [0,2p)
+__device__ __forceinline__ void gs_bfly(uint32_t& X, uint32_t& Y, Tw w, uint32_t p, uint32_t p2) {
+    const uint32_t s = X + Y;
+    const uint32_t d = X - Y + p2;
+    X = csub(s, p2);
+    Y = shoup(d, w, p);
+}
+// a*b*2^-32 mod p in [0,2p) for a, b in [0,2p) (Montgomery REDC).
+__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t neg_inv) {
+    const uint32_t lo = a * b, hi = __umulhi(a, b);
+    const uint32_t m = lo * neg_inv;
+    return hi + __umulhi(m, p) + (lo != 0u ? 1u : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// NTT tile geometry: a tile of T = 2^TLOG residues is held by 2^(TLOG-ELOG)
+// threads with E = 2^ELOG registers each.  The TLOG index bits are split into
+// NPH phases of (nearly) equal width; in phase P a thread's registers span
+// that phase's bits (plus filler bits when the width is below ELOG), so every
+// butterfly of the phase is register-local.  Phases are joined by an exchange
+// through padded LDS (address x + x/32).
+// ---------------------------------------------------------------------------
+template <int TLOG>
+struct Geo {
+    static constexpr int ELOG = TLOG < 5 ? TLOG : 5;
+    static constexpr int E = 1 << ELOG;
+    static constexpr int TPTLOG = TLOG - ELOG;
+    static constexpr int TPT = 1 << TPTLOG;
+    static constexpr int NPH = (TLOG + ELOG - 1) / ELOG;
+    static constexpr int T = 1 << TLOG;
+    static constexpr int STRIDE = T + (T >> 5);
+    static constexpr int width(int p) { return TLOG / NPH + (p < TLOG % NPH ? 1 : 0); }
+    static constexpr int lo(int p) {
+        int h = TLOG;
+        for (int i = 0; i <= p; ++i) h -= width(i);
+        return h;
+    }
+    static constexpr bool in_range(int p, int pos) { return pos >= lo(p) && pos < lo(p) + width(p); }
+    static constexpr int reg_pos(int p, int j) {
+        if (j < width(p)) return lo(p) + j;
+        int need = j - width(p);
+        for (int pos = 0; pos < TLOG; ++pos) {
+            if (in_range(p, pos)) continue;
+            if (need == 0) return pos;
+            --need;
+        }
+        return -1;
+    }
+    static constexpr bool is_reg_pos(int p, int pos) {
+        for (int j = 0; j < ELOG; ++j)
+            if (reg_pos(p, j) == pos) return true;
+        return false;
+    }
+    static constexpr int thr_pos(int p, int j) {
+        int need = j;
+        for (int pos = 0; pos < TLOG; ++pos) {
+            if (is_reg_pos(p, pos)) continue;
+            if (need == 0) return pos;
+            --need;
+        }
+        return -1;
+    }
+    static constexpr uint32_t reg_scatter(int p, int r) {
+        uint32_t x = 0;
+        for (int j = 0; j < ELOG; ++j)
+            if ((r >> j) & 1) x |= 1u << reg_pos(p, j);
+        return x;
+    }
+};
+
+template <int TLOG, int P>
+__device__ __forceinline__ uint32_t thr_scatter(uint32_t tau) {
+    using G = Geo<TLOG>;
+    uint32_t x = 0;
+#pragma unroll
+    for (int j = 0; j < G::TPTLOG; ++j) x |= ((tau >> j) & 1u) << G::thr_pos(P, j);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t padded(uint32_t x) { return x + (x >> 5); }
+
+template <int TLOG, int PFROM, int PTO>
+__device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* lds, uint32_t tau) {
+    using G = Geo<TLOG>;
+    // thread and register bits are disjoint, so padded(x|c) = padded(x)+padded(c):
+    // one base address per thread plus immediate offsets
+    uint32_t* wbase = lds + padded(thr_scatter<TLOG, PFROM>(tau));
+    uint32_t* rbase = lds + padded(thr_scatter<TLOG, PTO>(tau));
+#pragma unroll
+    for (int r = 0; r < G::E; ++r) wbase[padded(G::reg_scatter(PFROM, r))] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < G::E; ++r) v[r] = rbase[padded(G::reg_scatter(PTO, r))];
+    __syncthreads();
+}
+
+// Group index (index bits above beta) of register r in phase P.  When every
+// thread bit of the phase lies below beta+1 the group -- hence the twiddle --
+// is the same for all threads, which lets the compiler use scalar loads.
+template <int TLOG, int P>
+__device__ __forceinline__ uint32_t group_of(uint32_t tau_x, int r, int beta) {
+    using G = Geo<TLOG>;
+    bool thread_bits_above = false;
+    for (int j = 0; j < G::TPTLOG; ++j)
+        if (G::thr_pos(P, j) > beta) thread_bits_above = true;
+    const uint32_t reg_part = G::reg_scatter(P, r) >> (beta + 1);
+    // thread and register bits are disjoint: + is |, and folds into offsets
+    return thread_bits_above ? ((tau_x >> (beta + 1)) + reg_part) : reg_part;
+}
+
+// Forward CT stages of phase P on two polynomials that share twiddles.
+// Twiddle of local stage (m groups, group g) is table[bm*m + g] with
+// bm = 2^u + s for sub-block s of a transform with u outer stages.
+template <int TLOG, int P>
+__device__ __forceinline__ void fwd_phase2(uint32_t (&a)[Geo<TLOG>::E], uint32_t (&b)[Geo<TLOG>::E],
+                                           const Tw* __restrict__ tw, uint32_t bm, uint32_t tau_x,
+                                           uint32_t p, uint32_t p2) {
+    using G = Geo<TLOG>;
+#pragma unroll
+    for (int bit = G::width(P) - 1; bit >= 0; --bit) {
+        const int beta = G::lo(P) + bit;
+        const uint32_t m = 1u << (TLOG - 1 - beta);
+#pragma unroll
+        for (int r = 0; r < G::E; ++r) {
+            if (r & (1 << bit)) continue;
+            const Tw w = tw[bm * m + group_of<TLOG, P>(tau_x, r, beta)];
+            ct_bfly(a[r], a[r | (1 << bit)], w, p, p2);
+            ct_bfly(b[r], b[r | (1 << bit)], w, p, p2);
+        }
+        // keep the twiddles of one stage live at a time (VGPR budget)
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int TLOG, int P>
+__device__ __forceinline__ void inv_phase(uint32_t (&a)[Geo<TLOG>::E], const Tw* __restrict__ tw,
+                                          uint32_t bm, uint32_t tau_x, uint32_t p, uint32_t p2) {
+    using G = Geo<TLOG>;
+#pragma unroll
+    for (int bit = 0; bit < G::width(P); ++bit) {
+        const int beta = G::lo(P) + bit;
+        const uint32_t m = 1u << (TLOG - 1 - beta);
+#pragma unroll
+        for (int r = 0; r < G::E; ++r) {
+            if (r & (1 << bit)) continue;
+            const Tw w = tw[bm * m + group_of<TLOG, P>(tau_x, r, beta)];
+            gs_bfly(a[r], a[r | (1 << bit)], w, p, p2);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int TLOG, int P>
+__device__ __forceinline__ void forward_from(uint32_t (&a)[Geo<TLOG>::E], uint32_t (&b)[Geo<TLOG>::E],
+                                             uint32_t* lds, uint32_t tau, const Tw* tw, uint32_t bm,
+                                             uint32_t p, uint32_t p2) {
+    using G = Geo<TLOG>;
+    if constexpr (P > 0) {
+        exchange<TLOG, P - 1, P>(a, lds, tau);
+        exchange<TLOG, P - 1, P>(b, lds, tau);
+    }
+    fwd_phase2<TLOG, P>(a, b, tw, bm, thr_scatter<TLOG, P>(tau), p, p2);
+    if constexpr (P + 1 < G::NPH) forward_from<TLOG, P + 1>(a, b, lds, tau, tw, bm, p, p2);
+}
+
+template <int TLOG, int P>
+__device__ __forceinline__ void inverse_from(uint32_t (&a)[Geo<TLOG>::E], uint32_t* lds, uint32_t tau,
+                                             const Tw* tw, uint32_t bm, uint32_t p, uint32_t p2) {
+    using G = Geo<TLOG>;
+    if constexpr (P + 1 < G::NPH) exchange<TLOG, P + 1, P>(a, lds, tau);
+    inv_phase<TLOG, P>(a, tw, bm, thr_scatter<TLOG, P>(tau), p, p2);
+    if constexpr (P > 0) inverse_from<TLOG, P - 1>(a, lds, tau, tw, bm, p, p2);
+}
+
+// One tile: forward(a), forward(b), pointwise, inverse, scale to [0,p).
+// `valid` false means this thread's tile is padding: it joins every barrier
+// but touches no global memory.
+template <int TLOG>
+__device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __restrict__ a_src,
+                                          const uint32_t* __restrict__ b_src, uint32_t* __restrict__ dst,
+                                          uint32_t prime, uint32_t bm, uint32_t logn_full,
+                                          uint32_t* lds, uint32_t tau, bool valid) {
+    using G = Geo<TLOG>;
+    const uint32_t pi = valid ? prime : 0u;
+    const uint32_t p = tb.primes[pi], p2 = 2u * p, neg_inv = tb.mont[pi];
+    const size_t table = size_t(pi) << tb.max_logn;
+    uint32_t a[G::E], b[G::E];
+    const uint32_t x0 = thr_scatter<TLOG, 0>(tau);
+    const uint32_t* a_base = a_src + x0;
+    const uint32_t* b_base = b_src + x0;
+#pragma unroll
+    for (int r = 0; r < G::E; ++r) {
+        a[r] = valid ? a_base[G::reg_scatter(0, r)] : 0u;
+        b[r] = valid ? b_base[G::reg_scatter(0, r)] : 0u;
+    }
+    forward_from<TLOG, 0>(a, b, lds, tau, tb.fwd + table, bm, p, p2);
+#pragma unroll
+    for (int r = 0; r < G::E; ++r) a[r] = mont_mul(csub(a[r], p2), csub(b[r], p2), p, neg_inv);
+    inverse_from<TLOG, G::NPH - 1>(a, lds, tau, tb.inv + table, bm, p, p2);
+    const Tw s = tb.scale[logn_full * tb.k + pi];
+#pragma unroll
+    for (int r = 0; r < G::E; ++r) {
+        const uint32_t t = csub(shoup(a[r], s, p), p);  // [0,2p) -> [0,p)
+        if (valid) dst[x0 + G::reg_scatter(0, r)] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// residue conversion on MFMA
+// ---------------------------------------------------------------------------
+// Wave-level: 64 coefficients starting at c0 (4 column tiles of 16), all
+// primes.  Lane l supplies bytes [16*(l/16), +16) and [64+16*(l/16), +16) of
+// coefficient c0 + 16*tile + l%16; the weight operand of quad Q holds rows
+// (prime 4Q + i/4, digit i%4).  Output lane l gets the 4 digit sums of
+// (prime 4Q + l/16, coefficient c0 + 16*tile + l%16).
+__device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* __restrict__ coef, uint32_t n,
+                                             uint32_t c0, uint32_t* __restrict__ res,
+                                             const uint4* __restrict__ lds_w, uint32_t lane) {
+    const uint32_t col = lane & 15u, kg = lane >> 4;
+    v4i b0[4], b1[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const uint32_t c = c0 + 16u * ct + col;
+        uint4 v0 = make_uint4(0, 0, 0, 0), v1 = make_uint4(0, 0, 0, 0);
+        if (c < n) {
+            const uint4* src = reinterpret_cast<const uint4*>(coef + size_t(c) * kLimbs);
+            v0 = src[kg];
+            if (kg < 3) v1 = src[4 + kg];
+        }
+        // bytes a in [0,255] -> a - 128 in [-128,127]
+        b0[ct] = v4i{int(v0.x ^ 0x80808080u), int(v0.y ^ 0x80808080u), int(v0.z ^ 0x80808080u),
+                     int(v0.w ^ 0x80808080u)};
+        b1[ct] = kg < 3 ? v4i{int(v1.x ^ 0x80808080u), int(v1.y ^ 0x80808080u), int(v1.z ^ 0x80808080u),
+                              int(v1.w ^ 0x80808080u)}
+                        : v4i{0, 0, 0, 0};
+    }
+    for (uint32_t quad = 0; quad < tb.quads; ++quad) {
+        const uint4 w0u = lds_w[(quad * 2 + 0) * 64 + lane];
+        const uint4 w1u = lds_w[(quad * 2 + 1) * 64 + lane];
+        const v4i w0 = v4i{int(w0u.x), int(w0u.y), int(w0u.z), int(w0u.w)};
+        const v4i w1 = v4i{int(w1u.x), int(w1u.y), int(w1u.z), int(w1u.w)};
+        const uint32_t prime = quad * 4u + kg;
+        const bool live = prime < tb.k;
+        const uint32_t pi = live ? prime : 0u;
+        const uint32_t p = tb.primes[pi], p2 = 2u * p;
+        const int4 corr = reinterpret_cast<const int4*>(tb.conv_corr)[quad * 4u + kg];
+        const Tw c16{65536u, tb.conv_sh16[pi]};
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            v4i acc = v4i{0, 0, 0, 0};
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w0, b0[ct], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w1, b1[ct], acc, 0, 0, 0);
+            const uint32_t c = c0 + 16u * ct + col;
+            // exact digit sums T_d = sum_j a_j * digit_d(256^j mod p)
+            const int32_t t0 = acc[0] + corr.x, t1 = acc[1] + corr.y;
+            const int32_t t2 = acc[2] + corr.z, t3 = acc[3] + corr.w;
+            const int32_t u = t0 + t1 * 256;  // |u| < 2^30
+            const int32_t w = t2 + t3 * 256;  // |w| < 2^30
+            uint32_t x = csub(shoup(uint32_t(w) + p2, c16, p), p);   // w * 2^16 mod p, [0,p)
+            uint32_t r = uint32_t(u) + p2 + x;                          // < 2^32, > 0
+            r = csub(csub(r, p2), p2);                                  // [0,2p)
+            if (live && c < n) res[size_t(prime) * n + c] = r;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CRT: one output coefficient per thread
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void crt_one(const Tables& tb, const uint32_t* __restrict__ t_res, uint32_t n,
+                                        uint32_t c, uint32_t* __restrict__ out) {
+    uint64_t acc[31];
+#pragma unroll
+    for (int l = 0; l < 31; ++l) acc[l] = 0;
+    uint64_t est = 0;
+    const uint32_t k = tb.k;
+    for (uint32_t i0 = 0; i0 < k; i0 += 15) {
+        const uint32_t i1 = i0 + 15 < k ? i0 + 15 : k;
+        for (uint32_t i = i0; i < i1; ++i) {
+            const uint32_t t = t_res[size_t(i) * n + c];  // canonical, < p < 2^30
+            est += uint64_t(t) * tb.crt_round[i];
+            const uint32_t* m = tb.crt_m30 + size_t(i) * kCrtLimbs;
+#pragma unroll
+            for (int l = 0; l < 29; ++l) acc[l] += uint64_t(t) * m[l];
+        }
+        // keep every limb below 2^35 so 15 more products (< 2^60 each) fit
+#pragma unroll
+        for (int l = 0; l < 30; ++l) {
+            acc[l + 1] += (acc[l] >> 32) << 2;
+            acc[l] &= 0xFFFFFFFFull;
+        }
+    }
+    // exact 30-bit normalisation
+    uint32_t limb[31];
+#pragma unroll
+    for (int l = 0; l < 30; ++l) {
+        acc[l + 1] += acc[l] >> 30;
+        limb[l] = uint32_t(acc[l]) & 0x3FFFFFFFu;
+    }
+    limb[30] = uint32_t(acc[30]);
+    // 29 words of 32 bits
+    uint32_t x[29];
+#pragma unroll
+    for (int w = 0; w < 29; ++w) {
+        const int L = (32 * w) / 30, s = (32 * w) % 30;
+        x[w] = (limb[L] >> s) | (L + 1 < 31 ? (limb[L + 1] << (30 - s)) : 0u);
+    }
+    // nearest integer quotient of the CRT sum by P; add (-K*P) mod q
+    uint32_t K = uint32_t((est + (uint64_t(1) << 41)) >> 42);
+    K = K <= k ? K : k;  // canonical t_i give K <= k; clamp keeps the table read in bounds
+    const uint32_t* nk = tb.crt_negkp + size_t(K) * kLimbs;
+    uint64_t carry = 0;
+#pragma unroll
+    for (int w = 0; w < 28; ++w) {
+        const uint64_t s = uint64_t(x[w]) + nk[w] + carry;
+        x[w] = uint32_t(s);
+        carry = s >> 32;
+    }
+    x[28] += uint32_t(carry);
+    // Barrett: X < 2^904, Qh = floor(floor(X/2^864) * floor(2^908/q) / 2^44)
+    const uint64_t xh = uint64_t(x[27]) | (uint64_t(x[28]) << 32);
+    const uint64_t mu = uint64_t(tb.mu0) | (uint64_t(tb.mu1) << 32);
+    const uint64_t plo = xh * mu, phi = __umul64hi(xh, mu);
+    const uint64_t qh = (phi << 20) | (plo >> 44);
+    const uint32_t q0 = uint32_t(qh), q1 = uint32_t(qh >> 32);
+    // X -= qh * q
+    const uint32_t* qw = tb.q;
+    uint64_t mc = 0;          // carry of qh*q
+    int64_t borrow = 0;
+    uint32_t prev_hi1 = 0;    // high part of q1*q[w-1]
+#pragma unroll
+    for (int w = 0; w < 29; ++w) {
+        const uint32_t qwd = w < 28 ? qw[w] : 0u;
+        const uint32_t qwp = (w >= 1 && w - 1 < 28) ? qw[w - 1] : 0u;
+        const uint64_t p0 = uint64_t(q0) * qwd;
+        const uint64_t p1 = uint64_t(q1) * qwp;
+        const uint64_t sum = (p0 & 0xFFFFFFFFull) + (p1 & 0xFFFFFFFFull) + mc + prev_hi1;
+        const uint32_t prod_w = uint32_t(sum);
+        mc = (sum >> 32) + (p0 >> 32);
+        prev_hi1 = uint32_t(p1 >> 32);
+        const int64_t d = int64_t(x[w]) - int64_t(prod_w) - borrow;
+        x[w] = uint32_t(d);
+        borrow = d < 0 ? 1 : 0;
+    }
+    // result in [0, 2q): one conditional subtraction
+    uint32_t y[28];
+    borrow = 0;
+#pragma unroll
+    for (int w = 0; w < 28; ++w) {
+        const int64_t d = int64_t(x[w]) - int64_t(qw[w]) - borrow;
+        y[w] = uint32_t(d);
+        borrow = d < 0 ? 1 : 0;
+    }
+    const int64_t top = int64_t(x[28]) - borrow;
+    const bool ge = top >= 0;
+    uint4* dst = reinterpret_cast<uint4*>(out + size_t(c) * kLimbs);
+#pragma unroll
+    for (int w = 0; w < 28; w += 4)
+        dst[w / 4] = ge ? make_uint4(y[w], y[w + 1], y[w + 2], y[w + 3])
+                        : make_uint4(x[w], x[w + 1], x[w + 2], x[w + 3]);
+}
+
+// ---------------------------------------------------------------------------
+// standalone kernels (plain path)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_convert(Tables tb, Job job) {
+    __shared__ uint4 lds_w[16 * 2 * 64];
+    for (uint32_t i = threadIdx.x; i < tb.quads * 128u; i += blockDim.x) lds_w[i] = tb.conv_w[i];
+    __syncthreads();
+    const uint32_t n = 1u << job.logn;
+    const uint32_t poly = blockIdx.y;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t c0 = blockIdx.x * kConvPerBlock + wave * kConvPerWave;
+    if (c0 >= n) return;
+    convert_wave(tb, poly ? job.b : job.a, n, c0, poly ? job.rb : job.ra, lds_w, lane);
+}
+
+// units = k << u sub-blocks; unit -> (prime = unit >> u, s = unit & (2^u-1))
+template <int TLOG>
+__global__ void __launch_bounds__(kBlock) k_conv_tile(Tables tb, Job job, uint32_t u, uint32_t units) {
+    extern __shared__ uint32_t lds[];
+    using G = Geo<TLOG>;
+    const uint32_t tiles_per_block = blockDim.x >> G::TPTLOG;
+    const uint32_t tile = threadIdx.x >> G::TPTLOG, tau = threadIdx.x & (G::TPT - 1);
+    const uint32_t unit = blockIdx.x * tiles_per_block + tile;
+    const bool valid = unit < units;
+    const uint32_t prime = valid ? (unit >> u) : 0u, s = valid ? (unit & ((1u << u) - 1u)) : 0u;
+    const size_t n = size_t(1) << job.logn;
+    const size_t off = size_t(prime) * n + (size_t(s) << TLOG);
+    conv_tile<TLOG>(tb, job.ra + off, job.rb + off, job.ra + off, prime, (1u << u) + s, job.logn,
+                    lds + tile * G::STRIDE, tau, valid);
+}
+
+// Outer forward CT stages (bits logn-1 .. logn-u) for a and b: one thread per
+// column x of the (2^u x T) view; grid.y = 2*k (poly, prime).
+template <int U>
+__global__ void __launch_bounds__(256) k_outer_forward(Tables tb, Job job) {
+    const uint32_t logn = job.logn, tlog = logn - U;
+    const uint32_t T = 1u << tlog;
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= T) return;
+    const uint32_t poly = blockIdx.y / tb.k, prime = blockIdx.y % tb.k;
+    const uint32_t p = tb.primes[prime], p2 = 2u * p;
+    const Tw* tw = tb.fwd + (size_t(prime) << tb.max_logn);
+    uint32_t* v = (poly ? job.rb : job.ra) + (size_t(prime) << logn) + x;
+    constexpr int R = 1 << U;
+    uint32_t r[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) r[j] = v[size_t(j) << tlog];
+#pragma unroll
+    for (int bit = U - 1; bit >= 0; --bit) {
+        const uint32_t m = 1u << (U - 1 - bit);  // groups at this stage
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (j & (1 << bit)) continue;
+            ct_bfly(r[j], r[j | (1 << bit)], tw[m + (uint32_t(j) >> (bit + 1))], p, p2);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[size_t(j) << tlog] = r[j];
+}
+
+template <int U>
+__global__ void __launch_bounds__(256) k_outer_inverse(Tables tb, Job job) {
+    const uint32_t logn = job.logn, tlog = logn - U;
+    const uint32_t T = 1u << tlog;
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= T) return;
+    const uint32_t prime = blockIdx.y;
+    const uint32_t p = tb.primes[prime], p2 = 2u * p;
+    const Tw* tw = tb.inv + (size_t(prime) << tb.max_logn);
+    uint32_t* v = job.ra + (size_t(prime) << logn) + x;
+    constexpr int R = 1 << U;
+    uint32_t r[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) r[j] = v[size_t(j) << tlog];
+#pragma unroll
+    for (int bit = 0; bit < U; ++bit) {
+        const uint32_t m = 1u << (U - 1 - bit);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (j & (1 << bit)) continue;
+            gs_bfly(r[j], r[j | (1 << bit)], tw[m + (uint32_t(j) >> (bit + 1))], p, p2);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[size_t(j) << tlog] = csub(r[j], p);
+}
+
+__global__ void __launch_bounds__(kCrtPerBlock) k_crt(Tables tb, Job job) {
+    const uint32_t n = 1u << job.logn;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    crt_one(tb, job.ra, n, c, job.c);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <int TLOG>
+static hipError_t launch_tile(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream) {
+    using G = Geo<TLOG>;
+    const uint32_t units = tb.k << u;
+    uint32_t tiles = kBlock >> G::TPTLOG;
+    if (tiles > units) tiles = units;
+    uint32_t threads = tiles << G::TPTLOG;
+    if (threads < 64) {  // whole waves
+        tiles = (64u >> G::TPTLOG) ? (64u >> G::TPTLOG) : 1u;
+        threads = tiles << G::TPTLOG;
+    }
+    const uint32_t blocks = (units + tiles - 1) / tiles;
+    const size_t lds = size_t(tiles) * G::STRIDE * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_conv_tile<TLOG>, dim3(blocks), dim3(threads), lds, stream, tb, job, u, units);
+    return hipGetLastError();
+}
+
+static hipError_t launch_tile_dispatch(uint32_t tlog, const Tables& tb, const Job& job, uint32_t u,
+                                       hipStream_t stream) {
+    switch (tlog) {
+        case 1: return launch_tile<1>(tb, job, u, stream);
+        case 2: return launch_tile<2>(tb, job, u, stream);
+        case 3: return launch_tile<3>(tb, job, u, stream);
+        case 4: return launch_tile<4>(tb, job, u, stream);
+        case 5: return launch_tile<5>(tb, job, u, stream);
+        case 6: return launch_tile<6>(tb, job, u, stream);
+        case 7: return launch_tile<7>(tb, job, u, stream);
+        case 8: return launch_tile<8>(tb, job, u, stream);
+        case 9: return launch_tile<9>(tb, job, u, stream);
+        case 10: return launch_tile<10>(tb, job, u, stream);
+        case 11: return launch_tile<11>(tb, job, u, stream);
+        case 12: return launch_tile<12>(tb, job, u, stream);
+        case 13: return launch_tile<13>(tb, job, u, stream);
+        case 14: return launch_tile<14>(tb, job, u, stream);
+        case 15: return launch_tile<15>(tb, job, u, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream) {
+    const uint32_t n = 1u << job.logn;
+    const uint32_t blocks = (n + kConvPerBlock - 1) / kConvPerBlock;
+    uint32_t threads = kBlock;
+    if (n < kConvPerBlock) threads = ((n + kConvPerWave - 1) / kConvPerWave) * kWaveSize;
+    hipLaunchKernelGGL(k_convert, dim3(blocks, 2), dim3(threads), 0, stream, tb, job);
+    return hipGetLastError();
+}
+
+hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream) {
+    if (u > job.logn || job.logn - u > 15 || u > 2) return hipErrorInvalidValue;
+    const uint32_t tlog = job.logn - u;
+    if (u > 0) {
+        const uint32_t T = 1u << tlog;
+        const dim3 grid((T + 255) / 256, 2 * tb.k);
+        if (u == 1) hipLaunchKernelGGL(k_outer_forward<1>, grid, dim3(256), 0, stream, tb, job);
+        else hipLaunchKernelGGL(k_outer_forward<2>, grid, dim3(256), 0, stream, tb, job);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = launch_tile_dispatch(tlog, tb, job, u, stream);
+    if (e != hipSuccess) return e;
+    if (u > 0) {
+        const uint32_t T = 1u << tlog;
+        const dim3 grid((T + 255) / 256, tb.k);
+        if (u == 1) hipLaunchKernelGGL(k_outer_inverse<1>, grid, dim3(256), 0, stream, tb, job);
+        else hipLaunchKernelGGL(k_outer_inverse<2>, grid, dim3(256), 0, stream, tb, job);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
+    const uint32_t n = 1u << job.logn;
+    const uint32_t threads = n < kCrtPerBlock ? ((n + 63) / 64) * 64 : kCrtPerBlock;
+    hipLaunchKernelGGL(k_crt, dim3((n + threads - 1) / threads), dim3(threads), 0, stream, tb, job);
+    return hipGetLastError();
+}
+
+}  // namespace pmd
+
+namespace pmd {
+template <int TLOG>
+static hipError_t allow_lds() {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv_tile<TLOG>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(kLdsWords * sizeof(uint32_t)));
+}
+hipError_t configure_kernels() {
+    hipError_t e = hipSuccess;
+    if ((e = allow_lds<1>()) != hipSuccess) return e;
+    if ((e = allow_lds<2>()) != hipSuccess) return e;
+    if ((e = allow_lds<3>()) != hipSuccess) return e;
+    if ((e = allow_lds<4>()) != hipSuccess) return e;
+    if ((e = allow_lds<5>()) != hipSuccess) return e;
+    if ((e = allow_lds<6>()) != hipSuccess) return e;
+    if ((e = allow_lds<7>()) != hipSuccess) return e;
+    if ((e = allow_lds<8>()) != hipSuccess) return e;
+    if ((e = allow_lds<9>()) != hipSuccess) return e;
+    if ((e = allow_lds<10>()) != hipSuccess) return e;
+    if ((e = allow_lds<11>()) != hipSuccess) return e;
+    if ((e = allow_lds<12>()) != hipSuccess) return e;
+    if ((e = allow_lds<13>()) != hipSuccess) return e;
+    if ((e = allow_lds<14>()) != hipSuccess) return e;
+    if ((e = allow_lds<15>()) != hipSuccess) return e;
+    return e;
+}
+}  // namespace pmd

@@ -1,0 +1,66 @@
+// Host-side parameter math for the RNS negacyclic product (input-independent,
+// computed once per modulus q).  Plain C++17, no HIP.
+//
+// Follows the role of rns/host_setup.h:61-126 (prime search, CRT bases,
+// P mod q, per-prime roots and Shoup twiddles) with three deliberate changes:
+//   * primes are p = c*2^18 + 1 < 2^30 (30-bit, so Harvey lazy butterflies keep
+//     values in [0,4p) inside 32 bits, and 2N | p-1 for every N <= 2^17);
+//   * one primitive 2^18-th root per prime: the bit-reversed twiddle table of
+//     N = 2^17 has every smaller N as a prefix, so tables are shared by all N;
+//   * the CRT accumulates M_i mod q in 30-bit limbs and reduces modulo any odd
+//     868-bit q (the reference folds only q = 2^868 - c, rns/solve.cu:336-357).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pm {
+
+constexpr uint32_t kLimbs = 28;          // u32 limbs per ABI coefficient (fherma.h:35)
+constexpr uint32_t kWidth = 868;         // W
+constexpr uint32_t kMaxLogN = 17;        // largest supported N = 2^17
+constexpr uint32_t kMaxPrimes = 64;
+constexpr uint32_t kCrtLimbs = 29;       // 30-bit limbs of M_i mod q (870 bits)
+constexpr uint32_t kConvBytes = 112;     // bytes of an ABI coefficient
+constexpr uint32_t kRoundShift = 42;     // fixed point of the CRT rounding estimate
+
+struct Twiddle {
+    uint32_t value;
+    uint32_t shoup;  // floor(value * 2^32 / p)
+};
+
+struct Params {
+    uint32_t max_logn = 0;
+    uint32_t k = 0;                              // number of primes
+    std::vector<uint32_t> q;                     // 28 words
+    std::vector<uint32_t> primes;                // k
+    std::vector<uint32_t> psi_max;               // k, primitive 2^(max_logn+1)-th roots
+    std::vector<uint32_t> mont_neg_inv;          // k, -p^-1 mod 2^32
+    std::vector<Twiddle> forward;                // k * 2^max_logn, psi^brv(i)
+    std::vector<Twiddle> inverse;                // k * 2^max_logn, psi^-brv(i)
+    // post-inverse scale per (logn, prime): N^-1 * (M_i mod p_i)^-1 * 2^32 mod p
+    // (the 2^32 undoes the Montgomery factor of the pointwise product)
+    std::vector<Twiddle> scale;                  // (max_logn+1) * k
+    // residue conversion: balanced base-256 digits of 256^j mod p_i
+    std::vector<int8_t> conv_digits;             // k * 112 * 4  [prime][byte j][digit b]
+    std::vector<int32_t> conv_correction;        // k * 4: 128 * sum_j digit
+    std::vector<uint32_t> conv_shoup16;          // k: floor(2^16 * 2^32 / p)
+    // CRT
+    std::vector<uint32_t> crt_bases30;           // k * 29: M_i mod q in 30-bit limbs
+    std::vector<uint32_t> crt_round;             // k: floor(2^42 / p_i)
+    std::vector<uint32_t> crt_neg_kp;            // (k+1) * 28: (-K * P) mod q
+    std::vector<uint32_t> barrett_mu;            // 2 words: floor(2^906 / q)
+    uint32_t product_bits = 0;
+};
+
+// Returns an empty string on success, else the reason the modulus/size is
+// unsupported (mirrors the coverage checks of rns/solve.cu:581-590).
+std::string build_params(uint32_t max_logn, const uint32_t* q, uint32_t limbs, Params& out);
+
+// Shared helpers (also used by tests through the C-ABI).
+uint32_t pow_mod(uint32_t a, uint64_t e, uint32_t p);
+bool is_prime(uint32_t p);
+uint32_t bit_reverse(uint32_t x, uint32_t bits);
+
+}  // namespace pm

@@ -1,0 +1,303 @@
+// Context, resident tables and the plain product path behind include/polymul.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pm_device.h"
+#include "pm_kernels.h"
+#include "pm_params.h"
+#include "pm_runtime.h"
+#include "polymul.h"
+
+using pmd::kLimbs;
+
+namespace {
+
+std::once_flag g_configure_once;
+hipError_t g_configure_status = hipSuccess;
+thread_local std::string g_create_error;
+
+int create_failed(int status, const std::string& why) {
+    g_create_error = why;
+    return status;
+}
+
+}  // namespace
+
+int pm_fail(pm_context* ctx, int status, const std::string& why) {
+    if (ctx) ctx->last_error = why;
+    return status;
+}
+
+int pm_hip(pm_context* ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess) return PM_OK;
+    return pm_fail(ctx, e == hipErrorOutOfMemory ? PM_ENOMEM : PM_EDEVICE,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* pm_context::device_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    allocations.push_back(p);
+    return p;
+}
+
+template <class T>
+static const T* upload(pm_context* ctx, const std::vector<T>& v, int* status) {
+    if (*status != PM_OK) return nullptr;
+    void* p = ctx->device_alloc(v.size() * sizeof(T));
+    if (!p) {
+        *status = pm_fail(ctx, PM_ENOMEM, "device allocation of a parameter table failed");
+        return nullptr;
+    }
+    ctx->table_bytes += v.size() * sizeof(T);
+    hipError_t e = hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) *status = pm_hip(ctx, e, "upload parameter table");
+    return static_cast<const T*>(p);
+}
+
+uint32_t pm_default_outer(uint32_t logn) { return logn > 15 ? logn - 15 : 0; }
+
+extern "C" {
+
+int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q, int device,
+              pm_context** out) {
+    g_create_error.clear();
+    if (!out) return create_failed(PM_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!q) return create_failed(PM_EINVAL, "null modulus");
+    if (max_n < 2 || (max_n & (max_n - 1)) || max_n > (1u << pm::kMaxLogN))
+        return create_failed(PM_EUNSUPPORTED, "coverage: power-of-two 2 <= N <= 131072");
+    if (w_bits != pm::kWidth || limbs != pm::kLimbs)
+        return create_failed(PM_EUNSUPPORTED, "coverage: W=868, L=28");
+    pm_context* ctx = new (std::nothrow) pm_context();
+    if (!ctx) return create_failed(PM_ENOMEM, "host allocation failed");
+    ctx->device = device;
+    ctx->max_logn = uint32_t(__builtin_ctz(max_n));
+    const std::string why = pm::build_params(ctx->max_logn, q, limbs, ctx->params);
+    if (!why.empty()) {
+        delete ctx;
+        return create_failed(PM_EUNSUPPORTED, "coverage: " + why);
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return create_failed(PM_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    }
+    std::call_once(g_configure_once, [] { g_configure_status = pmd::configure_kernels(); });
+    if (g_configure_status != hipSuccess) {
+        delete ctx;
+        return create_failed(PM_EDEVICE, std::string("kernel configuration: ") + hipGetErrorString(g_configure_status));
+    }
+    int status = PM_OK;
+    const pm::Params& s = ctx->params;
+    const uint32_t k = s.k, quads = (k + 3) / 4;
+
+    // conversion operands: [quad][step][lane] x 16 bytes
+    std::vector<uint8_t> conv_w(size_t(quads) * 2 * 64 * 16, 0);
+    for (uint32_t quad = 0; quad < quads; ++quad)
+        for (uint32_t step = 0; step < 2; ++step)
+            for (uint32_t lane = 0; lane < 64; ++lane)
+                for (uint32_t e8 = 0; e8 < 16; ++e8) {
+                    const uint32_t row = lane % 16, prime = quad * 4 + row / 4, digit = row % 4;
+                    const uint32_t byte = 64 * step + 16 * (lane / 16) + e8;
+                    int8_t v = 0;
+                    if (prime < k && byte < pm::kConvBytes)
+                        v = s.conv_digits[(size_t(prime) * pm::kConvBytes + byte) * 4 + digit];
+                    conv_w[((size_t(quad) * 2 + step) * 64 + lane) * 16 + e8] = uint8_t(v);
+                }
+    std::vector<int32_t> conv_corr(size_t(quads) * 4 * 4, 0);
+    std::copy(s.conv_correction.begin(), s.conv_correction.end(), conv_corr.begin());
+    std::vector<pmd::Tw> fwd(s.forward.size()), inv(s.inverse.size()), scale(s.scale.size());
+    for (size_t i = 0; i < fwd.size(); ++i) fwd[i] = {s.forward[i].value, s.forward[i].shoup};
+    for (size_t i = 0; i < inv.size(); ++i) inv[i] = {s.inverse[i].value, s.inverse[i].shoup};
+    for (size_t i = 0; i < scale.size(); ++i) scale[i] = {s.scale[i].value, s.scale[i].shoup};
+
+    pmd::Tables& tb = ctx->tables;
+    tb.primes = upload(ctx, s.primes, &status);
+    tb.mont = upload(ctx, s.mont_neg_inv, &status);
+    tb.fwd = upload(ctx, fwd, &status);
+    tb.inv = upload(ctx, inv, &status);
+    tb.scale = upload(ctx, scale, &status);
+    tb.conv_w = reinterpret_cast<const uint4*>(upload(ctx, conv_w, &status));
+    tb.conv_corr = upload(ctx, conv_corr, &status);
+    tb.conv_sh16 = upload(ctx, s.conv_shoup16, &status);
+    tb.crt_m30 = upload(ctx, s.crt_bases30, &status);
+    tb.crt_round = upload(ctx, s.crt_round, &status);
+    tb.crt_negkp = upload(ctx, s.crt_neg_kp, &status);
+    tb.q = upload(ctx, s.q, &status);
+    tb.mu0 = s.barrett_mu[0];
+    tb.mu1 = s.barrett_mu[1];
+    tb.k = k;
+    tb.quads = quads;
+    tb.max_logn = ctx->max_logn;
+    if (status != PM_OK) {
+        const std::string reason = ctx->last_error;
+        pm_destroy(ctx);
+        return create_failed(status, reason);
+    }
+    // plain-path workspace for N up to max_n
+    const size_t coef_bytes = size_t(max_n) * kLimbs * sizeof(uint32_t);
+    const size_t res_bytes = size_t(k) * max_n * sizeof(uint32_t);
+    ctx->d_a = static_cast<uint32_t*>(ctx->device_alloc(coef_bytes));
+    ctx->d_b = static_cast<uint32_t*>(ctx->device_alloc(coef_bytes));
+    ctx->d_c = static_cast<uint32_t*>(ctx->device_alloc(coef_bytes));
+    ctx->d_ra = static_cast<uint32_t*>(ctx->device_alloc(res_bytes));
+    ctx->d_rb = static_cast<uint32_t*>(ctx->device_alloc(res_bytes));
+    ctx->workspace_bytes = 3 * coef_bytes + 2 * res_bytes;
+    if (!ctx->d_a || !ctx->d_b || !ctx->d_c || !ctx->d_ra || !ctx->d_rb) {
+        pm_destroy(ctx);
+        return create_failed(PM_ENOMEM, "device workspace allocation failed");
+    }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        pm_destroy(ctx);
+        return create_failed(PM_EDEVICE, "stream creation failed");
+    }
+    for (uint32_t logn = 0; logn <= pm::kMaxLogN; ++logn) ctx->outer[logn] = pm_default_outer(logn);
+    *out = ctx;
+    return PM_OK;
+}
+
+int pm_plan_rns(uint32_t max_n, const uint32_t* q, uint32_t* k, uint32_t* primes, uint32_t* psi_max,
+                uint32_t* product_bits) {
+    g_create_error.clear();
+    if (!q || !k) return create_failed(PM_EINVAL, "null argument");
+    if (max_n < 2 || (max_n & (max_n - 1)) || max_n > (1u << pm::kMaxLogN))
+        return create_failed(PM_EUNSUPPORTED, "coverage: power-of-two 2 <= N <= 131072");
+    pm::Params s;
+    const std::string why = pm::build_params(uint32_t(__builtin_ctz(max_n)), q, pm::kLimbs, s);
+    if (!why.empty()) return create_failed(PM_EUNSUPPORTED, "coverage: " + why);
+    *k = s.k;
+    for (uint32_t i = 0; i < s.k; ++i) {
+        if (primes) primes[i] = s.primes[i];
+        if (psi_max) psi_max[i] = s.psi_max[i];
+    }
+    if (product_bits) *product_bits = s.product_bits;
+    return PM_OK;
+}
+
+void pm_destroy(pm_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    for (void* p : ctx->allocations) (void)hipFree(p);
+    delete ctx;
+}
+
+const char* pm_last_error(const pm_context* ctx) { return ctx ? ctx->last_error.c_str() : g_create_error.c_str(); }
+
+int pm_get_info(const pm_context* ctx, pm_info_t* info) {
+    if (!ctx || !info) return PM_EINVAL;
+    info->max_n = 1u << ctx->max_logn;
+    info->k = ctx->params.k;
+    info->product_bits = ctx->params.product_bits;
+    info->device = uint32_t(ctx->device);
+    info->table_bytes = ctx->table_bytes;
+    info->workspace_bytes = ctx->workspace_bytes;
+    return PM_OK;
+}
+
+int pm_get_prime(const pm_context* ctx, uint32_t i, uint32_t* p, uint32_t* psi_max) {
+    if (!ctx || i >= ctx->params.k) return PM_EINVAL;
+    if (p) *p = ctx->params.primes[i];
+    if (psi_max) *psi_max = ctx->params.psi_max[i];
+    return PM_OK;
+}
+
+int pm_synchronize(pm_context* ctx) {
+    if (!ctx) return PM_EINVAL;
+    return pm_hip(ctx, hipStreamSynchronize(ctx->stream), "synchronize");
+}
+
+}  // extern "C"
+
+int pm_check_n(pm_context* ctx, uint32_t n, uint32_t* logn) {
+    if (n < 2 || (n & (n - 1))) return pm_fail(ctx, PM_EINVAL, "N must be a power of two >= 2");
+    *logn = uint32_t(__builtin_ctz(n));
+    if (*logn > ctx->max_logn) return pm_fail(ctx, PM_EUNSUPPORTED, "N exceeds the context's max_n");
+    return PM_OK;
+}
+
+int pm_run_plain(pm_context* ctx, const pmd::Job& job, uint32_t outer, hipStream_t stream) {
+    hipError_t e = pmd::launch_convert(ctx->tables, job, stream);
+    if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, outer, stream);
+    if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, stream);
+    return pm_hip(ctx, e, "launch plain product");
+}
+
+extern "C" {
+
+int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                       void* hip_stream) {
+    if (!ctx || !d_a || !d_b || !d_c) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
+    pmd::Job job{logn, 0, d_a, d_b, d_c, ctx->d_ra, ctx->d_rb};
+    return pm_run_plain(ctx, job, ctx->outer[logn], stream);
+}
+
+int pm_multiply(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t* c) {
+    if (!ctx || !a || !b || !c) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return pm_hip(ctx, e, "set device");
+    const size_t bytes = size_t(n) * kLimbs * sizeof(uint32_t);
+    e = hipMemcpyAsync(ctx->d_a, a, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_b, b, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return pm_hip(ctx, e, "copy inputs to device");
+    pmd::Job job{logn, 0, ctx->d_a, ctx->d_b, ctx->d_c, ctx->d_ra, ctx->d_rb};
+    st = pm_run_plain(ctx, job, ctx->outer[logn], ctx->stream);
+    if (st != PM_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return st;
+    }
+    e = hipMemcpyAsync(c, ctx->d_c, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return pm_hip(ctx, e, "product");
+}
+
+int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
+                     uint32_t* d_rb, void* hip_stream) {
+    if (!ctx || !d_a || !d_b || !d_ra || !d_rb) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
+    pmd::Job job{logn, 0, d_a, d_b, nullptr, d_ra, d_rb};
+    return pm_hip(ctx, pmd::launch_convert(ctx->tables, job, stream), "launch conversion");
+}
+
+int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, uint32_t* d_rb, void* hip_stream) {
+    if (!ctx || !d_ra || !d_rb) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    if (outer > 2 || outer > logn || logn - outer > 15)
+        return pm_fail(ctx, PM_EUNSUPPORTED, "outer stages must leave a tile of at most 2^15 (outer <= 2)");
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
+    pmd::Job job{logn, 0, nullptr, nullptr, nullptr, d_ra, d_rb};
+    return pm_hip(ctx, pmd::launch_ntt(ctx->tables, job, outer, stream), "launch NTT convolution");
+}
+
+int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream) {
+    if (!ctx || !d_t || !d_c) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
+    pmd::Job job{logn, 0, nullptr, nullptr, d_c, const_cast<uint32_t*>(d_t), nullptr};
+    return pm_hip(ctx, pmd::launch_crt(ctx->tables, job, stream), "launch CRT");
+}
+
+}  // extern "C"

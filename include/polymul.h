@@ -1,0 +1,90 @@
+/*
+ * polymul.h -- C-ABI of the MI355X (gfx950) exact negacyclic product
+ *     c = a * b  mod (X^N + 1, q),   q odd, 64 < bits(q) <= 868,
+ * with coefficients as 28 little-endian uint32 limbs (fherma.h:31-40).
+ *
+ * Plain pointers and sizes only; every call returns a pm_status and the
+ * context keeps a text reason (pm_last_error).  This is the layer the C++
+ * entry points of fherma.h (fherma_init / fherma_run / fherma_free,
+ * fherma.h:46-48) are implemented on, and what an FFI binds.
+ */
+#ifndef POLYMUL_H
+#define POLYMUL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pm_context pm_context;
+
+typedef enum {
+    PM_OK = 0,
+    PM_EINVAL = 1,        /* bad argument (null pointer, N not a power of two, ...) */
+    PM_EUNSUPPORTED = 2,  /* outside coverage: N > max_n, q too large/small/even     */
+    PM_EDEVICE = 3,       /* HIP runtime error (text in pm_last_error)              */
+    PM_ENOMEM = 4,        /* device or host allocation failed                       */
+    PM_EBUDGET = 5,       /* request exceeds the admission budget                   */
+    PM_ETIMEOUT = 6       /* a bounded wait expired                                 */
+} pm_status;
+
+#define PM_LIMBS 28
+#define PM_WIDTH 868
+
+typedef struct {
+    uint32_t max_n;        /* largest supported N (power of two, <= 2^17) */
+    uint32_t k;            /* RNS primes */
+    uint32_t product_bits; /* bits of the prime product P */
+    uint32_t device;       /* HIP device ordinal */
+    uint64_t table_bytes;  /* device bytes held by resident tables */
+    uint64_t workspace_bytes; /* device bytes held by the plain-path workspace */
+} pm_info_t;
+
+/* Host-only parameter plan (no device needed): the RNS of pm_create for this
+ * q and max_n.  primes/psi_max may be NULL or hold 64 entries.  Same
+ * coverage checks and prime rule as pm_create (rns/host_setup.h:61-78). */
+int pm_plan_rns(uint32_t max_n, const uint32_t* q, uint32_t* k, uint32_t* primes, uint32_t* psi_max,
+                uint32_t* product_bits);
+
+/* Replaces the input-independent part of fherma_init (rns/solve.cu:581-718):
+ * coverage checks, RNS parameters, resident device tables, workspace for N up
+ * to max_n.  w_bits must be 868 and limbs 28 (fherma.h:28-31). */
+int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q, int device,
+              pm_context** out);
+/* Replaces fherma_free (fherma.h:48). */
+void pm_destroy(pm_context* ctx);
+/* reason of the last failure on ctx; with ctx == NULL, of this thread's last
+ * failed pm_create */
+const char* pm_last_error(const pm_context* ctx);
+int pm_get_info(const pm_context* ctx, pm_info_t* info);
+/* prime i of the RNS and its primitive 2^(log2(max_n)+1)-th root of unity */
+int pm_get_prime(const pm_context* ctx, uint32_t i, uint32_t* p, uint32_t* psi_max);
+
+/* Replaces the timed body of fherma_run (rns/solve.cu:721-853): host buffers
+ * a, b, c of n*28 uint32 each (c may not alias a or b). */
+int pm_multiply(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t* c);
+/* Same product with device-resident a, b, c on a HIP stream (0 = the
+ * context's stream); returns once the work is enqueued. */
+int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
+                       void* hip_stream);
+/* Waits for the context's stream. */
+int pm_synchronize(pm_context* ctx);
+
+/* ---- stage entry points (device buffers, context stream) ------------------
+ * The three kernels of the plain path, exposed so each can be checked alone.
+ * Residue buffers are [k][n] uint32. */
+/* residues of a and b: ra[i*n+j] = a_j mod p_i (value below 2*p_i) -- rns/solve.cu:138-157 */
+int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
+                     uint32_t* d_rb, void* hip_stream);
+/* forward NTTs, pointwise, inverse, scale: ra <- N^-1 (M_i mod p_i)^-1 * (a*b mod p_i) --
+ * rns/solve.cu:225-333; outer = number of radix-2 stages run as separate passes */
+int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, uint32_t* d_rb, void* hip_stream);
+/* CRT + mod q from scaled residues -- rns/solve.cu:336-420 */
+int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,217 @@
+"""GPU parity: each HIP kernel against the oracle, and the whole product
+against the GMP Kronecker fixtures, bit for bit (integer work: exact)."""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import PKG_DIR, ROOT
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+
+
+def sha(x):
+    return hashlib.sha256(np.ascontiguousarray(x, dtype="<u4").tobytes()).hexdigest()
+
+
+def modulus_of(spec, workload):
+    kind, value = spec.split(":")
+    return workload.modulus(int(value)) if kind == "seed" else workload.special_modulus(int(value))
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def to_dev(torch, x):
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.uint32).view(np.int32)).cuda()
+
+
+def to_host(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+_CTX = {}
+
+
+def context(pm, q, max_n):
+    key = (bytes(np.asarray(q, dtype=np.uint32)), max_n)
+    if key not in _CTX:
+        _CTX[key] = pm.Context(q, max_n)
+    return _CTX[key]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def expected_t(orc, ctx, q, a, b):
+    """Scaled inverse-NTT residues t_i = N^-1 (M_i mod p_i)^-1 (a*b mod X^N+1) mod p_i."""
+    n = a.shape[0]
+    max_n = ctx.info()["max_n"]
+    primes = [p for p, _ in ctx.primes()]
+    psis = [g for _, g in ctx.primes()]
+    _, basis_inv, _ = orc.crt_constants(q, np.array(primes, dtype=np.uint32))
+    ra = orc.residues(a, np.array(primes, dtype=np.uint32))
+    rb = orc.residues(b, np.array(primes, dtype=np.uint32))
+    out = np.zeros((len(primes), n), dtype=np.uint32)
+    for i, (p, g) in enumerate(zip(primes, psis)):
+        psi_n = pow(int(g), max_n // n, p)
+        fa, fb = orc.ntt_forward(p, psi_n, ra[i]), orc.ntt_forward(p, psi_n, rb[i])
+        prod = ((fa.astype(np.uint64) * fb.astype(np.uint64)) % p).astype(np.uint32)
+        c = orc.ntt_inverse(p, psi_n, prod).astype(np.uint64)
+        scale = pow(n, p - 2, p) * int(basis_inv[i]) % p
+        out[i] = (c * np.uint64(scale) % np.uint64(p)).astype(np.uint32)
+    return out, ra, rb
+
+
+@pytest.mark.parametrize("n", [32, 1024, 4096])
+def test_convert_stage(pm, orc, workload, torch_cuda, n):
+    say("convert", n)
+    q = workload.modulus(11)
+    a, b = workload.family("dense-random-1", n, q, 2)
+    a[0] = workload.from_int(workload.to_int(q) - 1)  # largest valid coefficient
+    ctx = context(pm, q, 1 << 15)
+    primes = np.array([p for p, _ in ctx.primes()], dtype=np.uint32)
+    k = len(primes)
+    da, db = to_dev(torch_cuda, a), to_dev(torch_cuda, b)
+    ra = torch_cuda.zeros(k * n, dtype=torch_cuda.int32, device="cuda")
+    rb = torch_cuda.zeros(k * n, dtype=torch_cuda.int32, device="cuda")
+    ctx.stage_convert(n, da, db, ra, rb)
+    ctx.synchronize()
+    ga, gb = to_host(ra).reshape(k, n), to_host(rb).reshape(k, n)
+    assert (ga < 2 * primes[:, None]).all() and (gb < 2 * primes[:, None]).all()
+    assert ((ga % primes[:, None]) == orc.residues(a, primes)).all()
+    assert ((gb % primes[:, None]) == orc.residues(b, primes)).all()
+
+
+@pytest.mark.parametrize("n,outer", [(32, 0), (1024, 0), (8192, 0), (32768, 0), (32768, 1), (32768, 2)])
+def test_ntt_stage(pm, orc, workload, torch_cuda, n, outer):
+    say("ntt", n, outer)
+    q = workload.modulus(12)
+    a, b = workload.family("dense-random-1", n, q, 3)
+    ctx = context(pm, q, 1 << 17)
+    want, ra, rb = expected_t(orc, ctx, q, a, b)
+    dra, drb = to_dev(torch_cuda, ra), to_dev(torch_cuda, rb)
+    ctx.stage_ntt(n, dra, drb, outer=outer)
+    ctx.synchronize()
+    got = to_host(dra).reshape(want.shape)
+    assert (got == want).all()
+
+
+@pytest.mark.parametrize("n", [32, 4096])
+def test_crt_stage(pm, orc, workload, torch_cuda, n):
+    say("crt", n)
+    for family in ("dense-random-1", "all-q-minus-one", "zero", "limb-boundaries"):
+        q = workload.modulus(13)
+        a, b = workload.family(family, n, q, 4)
+        ctx = context(pm, q, 1 << 15)
+        t, _, _ = expected_t(orc, ctx, q, a, b)
+        c = torch_cuda.zeros(n * 28, dtype=torch_cuda.int32, device="cuda")
+        ctx.stage_crt(n, to_dev(torch_cuda, t), c)
+        ctx.synchronize()
+        want = orc.rns_multiply(q, a, b, orc.reference_primes_for(n, q))
+        assert (to_host(c).reshape(n, 28) == want).all(), family
+
+
+@pytest.mark.parametrize("n", [32, 64, 1024, 4096, 8192, 16384, 32768, 65536, 131072])
+def test_product_vs_golden(pm, workload, n):
+    for case in GOLDEN["cases"]:
+        if case["n"] != n:
+            continue
+        say("golden", n, case["modulus"], case["family"])
+        q = modulus_of(case["modulus"], workload)
+        a, b = workload.family(case["family"], n, q, case["seed"])
+        ctx = context(pm, q, max(n, 1 << 15))
+        c = ctx.multiply(a, b)
+        assert sha(c) == case["sha256"]["c"], (n, case["modulus"], case["family"])
+
+
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 128, 2048])
+def test_product_small_sizes_vs_schoolbook(pm, orc, workload, n):
+    say("small", n)
+    q = workload.modulus(21)
+    ctx = context(pm, q, 1 << 15)
+    for family in ("dense-random-1", "all-q-minus-one", "wrap-sign"):
+        a, b = workload.family(family, n, q, 9)
+        want = orc.schoolbook(q, a, b) if n <= 256 else orc.rns_multiply(q, a, b, orc.primes_reference())
+        assert (ctx.multiply(a, b) == want).all(), (n, family)
+
+
+def test_outer_schedules_agree(pm, workload, torch_cuda):
+    say("outer schedules")
+    case = [c for c in GOLDEN["cases"] if c["n"] == 32768 and c["family"] == "dense-random-1"][0]
+    q = modulus_of(case["modulus"], workload)
+    a, b = workload.family("dense-random-1", 32768, q, case["seed"])
+    ctx = context(pm, q, 1 << 17)
+    k = ctx.info()["k"]
+    for outer in (0, 1, 2):
+        ra = torch_cuda.zeros(k * 32768, dtype=torch_cuda.int32, device="cuda")
+        rb = torch_cuda.zeros_like(ra)
+        c = torch_cuda.zeros(32768 * 28, dtype=torch_cuda.int32, device="cuda")
+        ctx.stage_convert(32768, to_dev(torch_cuda, a), to_dev(torch_cuda, b), ra, rb)
+        ctx.stage_ntt(32768, ra, rb, outer=outer)
+        ctx.stage_crt(32768, ra, c)
+        ctx.synchronize()
+        assert sha(to_host(c)) == case["sha256"]["c"], outer
+
+
+def test_device_multiply_and_reuse(pm, orc, workload, torch_cuda):
+    say("device multiply")
+    q = workload.modulus(5)
+    ctx = context(pm, q, 1 << 15)
+    for seed, n in ((1, 4096), (2, 1024), (3, 4096)):  # state reuse across sizes
+        a, b = workload.family("dense-random-1", n, q, seed)
+        c = torch_cuda.zeros(n * 28, dtype=torch_cuda.int32, device="cuda")
+        ctx.multiply_device(n, to_dev(torch_cuda, a), to_dev(torch_cuda, b), c)
+        ctx.synchronize()
+        assert (to_host(c).reshape(n, 28) == orc.rns_multiply(q, a, b, orc.primes_reference())).all()
+
+
+def test_solver_mirror(pm, workload):
+    say("solver")
+    case = [c for c in GOLDEN["cases"] if c["n"] == 1024 and c["family"] == "limb-boundaries"][0]
+    q = modulus_of(case["modulus"], workload)
+    a, b = workload.family(case["family"], 1024, q, case["seed"])
+    solver = pm.Solver(1024, 868, 28, q)
+    assert sha(solver.run(a, b)) == case["sha256"]["c"]
+    with pytest.raises(RuntimeError):
+        solver.run(a, b[:512])
+    solver.free()
+
+
+def test_solution_harness(workload, tmp_path):
+    say("solution harness")
+    n = 1024
+    cases = [c for c in GOLDEN["cases"] if c["n"] == n][:3]
+    q = modulus_of(cases[0]["modulus"], workload)
+    (tmp_path / "point").mkdir()
+    (tmp_path / "point" / "q.bin").write_bytes(q.astype("<u4").tobytes())
+    for i, case in enumerate(cases):
+        d = tmp_path / "cases" / f"{i:06d}"
+        d.mkdir(parents=True)
+        a, b = workload.family(case["family"], n, q, case["seed"])
+        (d / "a.bin").write_bytes(a.astype("<u4").tobytes())
+        (d / "b.bin").write_bytes(b.astype("<u4").tobytes())
+    # a truncated case is reported as crashed, the others still run
+    bad = tmp_path / "cases" / f"{len(cases):06d}"
+    bad.mkdir(parents=True)
+    (bad / "a.bin").write_bytes(b"\0" * 100)
+    (bad / "b.bin").write_bytes(b"\0" * 100)
+    (tmp_path / "manifest.json").write_text(json.dumps({"point": {"N": n, "W": 868, "L": 28},
+                                                        "cases": len(cases) + 1}))
+    subprocess.run([os.path.join(PKG_DIR, "solution"), str(tmp_path)], check=True, timeout=120)
+    report = json.loads((tmp_path / "out" / "results.json").read_text())
+    assert [c["status"] for c in report["cases"]] == ["ok"] * len(cases) + ["crashed"]
+    assert report["init_s"] >= 0
+    for i, case in enumerate(cases):
+        got = (tmp_path / "out" / f"{i:06d}" / "c.bin").read_bytes()
+        assert hashlib.sha256(got).hexdigest() == case["sha256"]["c"]
