@@ -66,6 +66,15 @@ def library():
     lib.pm_stage_convert.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
     lib.pm_stage_crt.argtypes = [P, u32, P, P, P]
+    lib.pm_batch_create.argtypes = [P, u32, P, P, P, P, ctypes.POINTER(P)]
+    lib.pm_batch_destroy.argtypes = [P]
+    lib.pm_batch_destroy.restype = None
+    lib.pm_batch_io.argtypes = [P, u32, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)]
+    lib.pm_batch_upload.argtypes = [P, u32, P, P]
+    lib.pm_batch_download.argtypes = [P, u32, P]
+    lib.pm_batch_run.argtypes = [P, P]
+    lib.pm_batch_wait.argtypes = [P, P]
+    lib.pm_batch_info.argtypes = [P, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_uint64)]
     for name in EXPORTED:
         getattr(lib, name)  # every declared symbol must resolve
     _LIB = lib
@@ -74,7 +83,9 @@ def library():
 
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
-            "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_ntt", "pm_stage_crt"]
+            "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_ntt", "pm_stage_crt",
+            "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
+            "pm_batch_run", "pm_batch_wait", "pm_batch_info"]
 
 
 def _ptr(x):
@@ -180,6 +191,57 @@ class Context:
 
     def stage_crt(self, n, t, c, stream=None):
         self._check("pm_stage_crt", self._lib.pm_stage_crt(self._h, n, _ptr(t), _ptr(c), stream))
+
+
+class Batch:
+    """Products of uneven N in one persistent, graph-captured launch."""
+
+    def __init__(self, ctx, ns, d_a=None, d_b=None, d_c=None):
+        self._ctx, self._lib = ctx, ctx._lib
+        self.ns = [int(n) for n in ns]
+        count = len(self.ns)
+        arr = (ctypes.c_uint32 * count)(*self.ns)
+        ptrs = [None, None, None]
+        if d_a is not None:
+            ptrs = [(ctypes.c_void_p * count)(*[_ptr(x) for x in v]) for v in (d_a, d_b, d_c)]
+        handle = ctypes.c_void_p()
+        ctx._check("pm_batch_create", self._lib.pm_batch_create(ctx.handle, count, arr, ptrs[0], ptrs[1], ptrs[2],
+                                                                 ctypes.byref(handle)))
+        self._h = handle
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, i, a, b):
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        assert a.shape == (self.ns[i], LIMBS) and b.shape == a.shape
+        self._ctx._check("pm_batch_upload", self._lib.pm_batch_upload(self._h, i, a.ctypes.data, b.ctypes.data))
+
+    def download(self, i):
+        c = np.empty((self.ns[i], LIMBS), dtype=np.uint32)
+        self._ctx._check("pm_batch_download", self._lib.pm_batch_download(self._h, i, c.ctypes.data))
+        return c
+
+    def run(self, stream=None):
+        self._ctx._check("pm_batch_run", self._lib.pm_batch_run(self._h, stream))
+
+    def wait(self, stream=None):
+        self._ctx._check("pm_batch_wait", self._lib.pm_batch_wait(self._h, stream))
+
+    def info(self):
+        items, blocks, nbytes = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        self._ctx._check("pm_batch_info", self._lib.pm_batch_info(self._h, ctypes.byref(items), ctypes.byref(blocks),
+                                                                   ctypes.byref(nbytes)))
+        return {"items": items.value, "blocks": blocks.value, "device_bytes": nbytes.value}
 
 
 class Solver:

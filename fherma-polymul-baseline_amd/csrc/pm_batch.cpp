@@ -1,0 +1,244 @@
+// Batched products of uneven N in one persistent launch captured in a hipGraph.
+//
+// The reference has no batch path (one product per fherma_run call,
+// rns/solve.cu:721-853); this builds a work list over many products -- residue
+// conversion blocks, optional outer NTT stages, fused convolution tiles, CRT
+// blocks -- that resident blocks claim in order and run as soon as the items
+// they depend on have finished (per-product counters), so conversion, NTT and
+// CRT of different products overlap inside a single kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pm_batch.h"
+#include "pm_device.h"
+#include "pm_kernels.h"
+#include "pm_runtime.h"
+#include "polymul.h"
+
+using pmd::kLimbs;
+
+namespace {
+
+uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+pm_batch::~pm_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void* p : allocations) (void)hipFree(p);
+}
+
+void* pm_batch::alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    allocations.push_back(p);
+    device_bytes += bytes;
+    return p;
+}
+
+int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint32_t* const* d_a,
+                   const uint32_t* const* d_b, uint32_t* const* d_c) {
+    const uint32_t k = ctx->params.k, count = b->count;
+    b->logn.resize(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t n = ns[i];
+        if (n < 32 || (n & (n - 1))) return pm_fail(ctx, PM_EUNSUPPORTED, "batch sizes must be powers of two >= 32");
+        b->logn[i] = uint32_t(__builtin_ctz(n));
+        if (b->logn[i] > ctx->max_logn) return pm_fail(ctx, PM_EUNSUPPORTED, "batch N exceeds the context's max_n");
+    }
+    // device memory: inputs/outputs (unless the caller owns them) and residues
+    std::vector<pmd::Job> jobs(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        const size_t n = size_t(1) << b->logn[i];
+        pmd::Job& j = jobs[i];
+        j.logn = b->logn[i];
+        j.outer = b->logn[i] > 15 ? b->logn[i] - 15 : 0;
+        const bool caller_io = d_a && d_b && d_c;
+        if (caller_io) {
+            j.a = d_a[i];
+            j.b = d_b[i];
+            j.c = d_c[i];
+        } else {
+            auto* io = static_cast<uint32_t*>(b->alloc(3 * n * kLimbs * sizeof(uint32_t)));
+            if (!io) return pm_fail(ctx, PM_ENOMEM, "batch input/output allocation failed");
+            j.a = io;
+            j.b = io + n * kLimbs;
+            j.c = io + 2 * n * kLimbs;
+        }
+        auto* res = static_cast<uint32_t*>(b->alloc(2 * size_t(k) * n * sizeof(uint32_t)));
+        if (!res) return pm_fail(ctx, PM_ENOMEM, "batch residue allocation failed");
+        j.ra = res;
+        j.rb = res + size_t(k) * n;
+    }
+    // work list: conversions (largest N first), outer forward stages, tiles by
+    // size class, outer inverse stages, CRT blocks -- every dependency points
+    // to an earlier item, so in-order claiming cannot deadlock
+    std::vector<uint32_t> order(count);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return b->logn[x] > b->logn[y]; });
+    std::vector<pmd::Item> items;
+    std::vector<pmd::Unit> units;
+    for (uint32_t j : order) {
+        const uint32_t blocks = ceil_div(1u << b->logn[j], pmd::kConvPerBlock);
+        for (uint32_t blk = 0; blk < blocks; ++blk)
+            for (uint32_t poly = 0; poly < 2; ++poly) items.push_back({pmd::kItemConv, j, poly, blk});
+    }
+    for (uint32_t j : order)
+        if (jobs[j].outer)
+            for (uint32_t x = 0; x < 2 * k; ++x) items.push_back({pmd::kItemOuterFwd, j, x, 0});
+    for (uint32_t tlog = 15; tlog >= 5; --tlog) {
+        const uint32_t per_item = 1u << (15 - tlog);  // 1024 threads x 32 residues / tile size
+        std::vector<pmd::Unit> cls;
+        for (uint32_t j : order) {
+            if (jobs[j].logn - jobs[j].outer != tlog) continue;
+            for (uint32_t prime = 0; prime < k; ++prime)
+                for (uint32_t s = 0; s < (1u << jobs[j].outer); ++s) cls.push_back({j, (prime << 2) | s});
+        }
+        for (size_t first = 0; first < cls.size(); first += per_item) {
+            const uint32_t n_units = uint32_t(std::min<size_t>(per_item, cls.size() - first));
+            items.push_back({pmd::kItemTile, 0, tlog | (n_units << 8), uint32_t(units.size())});
+            units.insert(units.end(), cls.begin() + first, cls.begin() + first + n_units);
+        }
+    }
+    for (uint32_t j : order)
+        if (jobs[j].outer)
+            for (uint32_t prime = 0; prime < k; ++prime) items.push_back({pmd::kItemOuterInv, j, prime, 0});
+    for (uint32_t j : order) {
+        const uint32_t blocks = ceil_div(1u << b->logn[j], pmd::kBlock);
+        for (uint32_t blk = 0; blk < blocks; ++blk) items.push_back({pmd::kItemCrt, j, blk, 0});
+    }
+    if (units.empty()) units.push_back({0, 0});
+    b->n_items = uint32_t(items.size());
+
+    auto* d_jobs = static_cast<pmd::Job*>(b->alloc(jobs.size() * sizeof(pmd::Job)));
+    auto* d_items = static_cast<pmd::Item*>(b->alloc(items.size() * sizeof(pmd::Item)));
+    auto* d_units = static_cast<pmd::Unit*>(b->alloc(units.size() * sizeof(pmd::Unit)));
+    b->reset_count = 2 + 4 * count;
+    b->d_words = static_cast<uint32_t*>(b->alloc(size_t(b->reset_count) * sizeof(uint32_t)));
+    if (!d_jobs || !d_items || !d_units || !b->d_words) return pm_fail(ctx, PM_ENOMEM, "batch table allocation failed");
+    hipError_t e = hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(pmd::Job), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_items, items.data(), items.size() * sizeof(pmd::Item), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_units, units.data(), units.size() * sizeof(pmd::Unit), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->d_words, 0, size_t(b->reset_count) * sizeof(uint32_t));
+    if (e != hipSuccess) return pm_hip(ctx, e, "upload batch tables");
+    b->jobs = jobs;
+
+    pmd::BatchArgs& ba = b->args;
+    ba.jobs = d_jobs;
+    ba.items = d_items;
+    ba.units = d_units;
+    ba.next = b->d_words;
+    ba.error = b->d_words + 1;
+    ba.counters = reinterpret_cast<pmd::JobCounters*>(b->d_words + 2);
+    ba.n_items = b->n_items;
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    ba.wait_ticks = uint64_t(rate_khz) * 1000u * 10u;  // 10 s per dependency wait
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, ctx->device);
+    if (e != hipSuccess) return pm_hip(ctx, e, "device properties");
+    b->blocks = uint32_t(prop.multiProcessorCount);  // one 1024-thread block per CU (LDS bound)
+    if (b->blocks > b->n_items) b->blocks = b->n_items;
+    if (b->blocks == 0) b->blocks = 1;
+
+    // capture reset + persistent kernel once; runs replay the graph
+    hipStream_t cap = nullptr;
+    e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return pm_hip(ctx, e, "capture stream");
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t le = pmd::launch_persistent(ctx->tables, ba, b->d_words, b->reset_count, b->blocks, cap);
+        hipError_t ce = hipStreamEndCapture(cap, &b->graph);
+        e = le != hipSuccess ? le : ce;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cap);
+    if (e != hipSuccess) return pm_hip(ctx, e, "capture batch graph");
+    return PM_OK;
+}
+
+extern "C" {
+
+int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
+                    const uint32_t* const* d_b, uint32_t* const* d_c, pm_batch** out) {
+    if (!ctx || !ns || !out || count == 0) return PM_EINVAL;
+    *out = nullptr;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return pm_hip(ctx, e, "set device");
+    pm_batch* b = new (std::nothrow) pm_batch();
+    if (!b) return pm_fail(ctx, PM_ENOMEM, "host allocation failed");
+    b->ctx = ctx;
+    b->count = count;
+    const int st = pm_batch_build(ctx, b, ns, d_a, d_b, d_c);
+    if (st != PM_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return PM_OK;
+}
+
+void pm_batch_destroy(pm_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int pm_batch_io(pm_batch* b, uint32_t i, uint32_t** d_a, uint32_t** d_b, uint32_t** d_c) {
+    if (!b || i >= b->count) return PM_EINVAL;
+    if (d_a) *d_a = const_cast<uint32_t*>(b->jobs[i].a);
+    if (d_b) *d_b = const_cast<uint32_t*>(b->jobs[i].b);
+    if (d_c) *d_c = b->jobs[i].c;
+    return PM_OK;
+}
+
+int pm_batch_upload(pm_batch* b, uint32_t i, const uint32_t* a, const uint32_t* bb) {
+    if (!b || i >= b->count || !a || !bb) return PM_EINVAL;
+    const size_t bytes = (size_t(1) << b->logn[i]) * kLimbs * sizeof(uint32_t);
+    hipError_t e = hipMemcpy(const_cast<uint32_t*>(b->jobs[i].a), a, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(const_cast<uint32_t*>(b->jobs[i].b), bb, bytes, hipMemcpyHostToDevice);
+    return pm_hip(b->ctx, e, "batch upload");
+}
+
+int pm_batch_download(pm_batch* b, uint32_t i, uint32_t* c) {
+    if (!b || i >= b->count || !c) return PM_EINVAL;
+    const size_t bytes = (size_t(1) << b->logn[i]) * kLimbs * sizeof(uint32_t);
+    return pm_hip(b->ctx, hipMemcpy(c, b->jobs[i].c, bytes, hipMemcpyDeviceToHost), "batch download");
+}
+
+int pm_batch_run(pm_batch* b, void* hip_stream) {
+    if (!b) return PM_EINVAL;
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->ctx->stream;
+    return pm_hip(b->ctx, hipGraphLaunch(b->exec, stream), "launch batch graph");
+}
+
+int pm_batch_wait(pm_batch* b, void* hip_stream) {
+    if (!b) return PM_EINVAL;
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->ctx->stream;
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return pm_hip(b->ctx, e, "batch wait");
+    uint32_t err = 0;
+    e = hipMemcpy(&err, b->d_words + 1, sizeof(err), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return pm_hip(b->ctx, e, "read batch status");
+    if (err == pmd::kErrTimeout) return pm_fail(b->ctx, PM_ETIMEOUT, "a batch dependency wait timed out");
+    return PM_OK;
+}
+
+int pm_batch_info(const pm_batch* b, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes) {
+    if (!b) return PM_EINVAL;
+    if (items) *items = b->n_items;
+    if (blocks) *blocks = b->blocks;
+    if (device_bytes) *device_bytes = b->device_bytes;
+    return PM_OK;
+}
+
+}  // extern "C"

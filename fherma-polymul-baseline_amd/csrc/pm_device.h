@@ -40,12 +40,46 @@ struct Tables {
 // One product of the batch / the plain path.
 struct Job {
     uint32_t logn;
-    uint32_t pad;
+    uint32_t outer;      // radix-2 stages run as separate passes (tile = 2^(logn-outer))
     const uint32_t* a;   // [n][28] coefficients (device)
     const uint32_t* b;
     uint32_t* c;         // [n][28] result (device)
     uint32_t* ra;        // [k][n] residues of a, reused for CRT inputs
     uint32_t* rb;        // [k][n] residues of b
 };
+
+// Persistent batch: a work list of items claimed in order by resident blocks.
+enum ItemType : uint32_t { kItemConv = 0, kItemOuterFwd = 1, kItemTile = 2, kItemOuterInv = 3, kItemCrt = 4 };
+
+struct Item {
+    uint32_t type;
+    uint32_t job;   // product (tile items: unused)
+    uint32_t a0;    // conv: poly; outer fwd: poly*k+prime; tile: tlog | count<<8; outer inv: prime; crt: block
+    uint32_t a1;    // conv: coefficient block; tile: first unit
+};
+
+struct Unit {
+    uint32_t job;
+    uint32_t prime_s;  // prime << 2 | sub-block
+};
+
+// per-job completion counters, zeroed before every run
+struct JobCounters {
+    uint32_t conv, fwd, tile, inv;
+};
+
+struct BatchArgs {
+    const Job* jobs;
+    const Item* items;
+    const Unit* units;
+    uint32_t* next;             // item claim counter
+    uint32_t* error;            // 0 ok, 2 wait timed out
+    JobCounters* counters;      // [jobs]
+    uint32_t n_items;
+    uint32_t pad;
+    uint64_t wait_ticks;        // bound on one dependency wait (s_memrealtime ticks)
+};
+
+constexpr uint32_t kErrTimeout = 2;
 
 }  // namespace pmd

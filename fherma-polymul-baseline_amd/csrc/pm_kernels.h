@@ -14,6 +14,9 @@ hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream);
 hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream);
 // CRT and reduction modulo q into 28-limb coefficients
 hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream);
+// reset counters, then one persistent launch over the batch work list
+hipError_t launch_persistent(const Tables& tb, const BatchArgs& ba, uint32_t* reset_words, uint32_t reset_count,
+                             uint32_t blocks, hipStream_t stream);
 // raise the dynamic LDS limit of the tile kernels (call once per device)
 hipError_t configure_kernels();
 

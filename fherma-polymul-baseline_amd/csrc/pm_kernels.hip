@@ -121,6 +121,11 @@ __device__ __forceinline__ uint32_t thr_scatter(uint32_t tau) {
 
 __device__ __forceinline__ uint32_t padded(uint32_t x) { return x + (x >> 5); }
 
+#ifndef PM_TWIDDLE_CHUNK
+#define PM_TWIDDLE_CHUNK 8
+#endif
+constexpr int kTwiddleChunk = PM_TWIDDLE_CHUNK;
+
 template <int TLOG, int PFROM, int PTO>
 __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* lds, uint32_t tau) {
     using G = Geo<TLOG>;
@@ -162,15 +167,16 @@ __device__ __forceinline__ void fwd_phase2(uint32_t (&a)[Geo<TLOG>::E], uint32_t
     for (int bit = G::width(P) - 1; bit >= 0; --bit) {
         const int beta = G::lo(P) + bit;
         const uint32_t m = 1u << (TLOG - 1 - beta);
+        int done = 0;
 #pragma unroll
         for (int r = 0; r < G::E; ++r) {
             if (r & (1 << bit)) continue;
             const Tw w = tw[bm * m + group_of<TLOG, P>(tau_x, r, beta)];
             ct_bfly(a[r], a[r | (1 << bit)], w, p, p2);
             ct_bfly(b[r], b[r | (1 << bit)], w, p, p2);
+            // bound the twiddles in flight (VGPR budget of 1024-thread blocks)
+            if (++done % kTwiddleChunk == 0) __builtin_amdgcn_sched_barrier(0);
         }
-        // keep the twiddles of one stage live at a time (VGPR budget)
-        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -182,13 +188,14 @@ __device__ __forceinline__ void inv_phase(uint32_t (&a)[Geo<TLOG>::E], const Tw*
     for (int bit = 0; bit < G::width(P); ++bit) {
         const int beta = G::lo(P) + bit;
         const uint32_t m = 1u << (TLOG - 1 - beta);
+        int done = 0;
 #pragma unroll
         for (int r = 0; r < G::E; ++r) {
             if (r & (1 << bit)) continue;
             const Tw w = tw[bm * m + group_of<TLOG, P>(tau_x, r, beta)];
             gs_bfly(a[r], a[r | (1 << bit)], w, p, p2);
+            if (++done % (2 * kTwiddleChunk) == 0) __builtin_amdgcn_sched_barrier(0);
         }
-        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -433,6 +440,239 @@ __global__ void __launch_bounds__(kBlock) k_conv_tile(Tables tb, Job job, uint32
                     lds + tile * G::STRIDE, tau, valid);
 }
 
+// outer CT stages for one (poly, prime) channel, columns strided by `stride`
+template <int U>
+__device__ __forceinline__ void outer_forward_columns(const Tables& tb, uint32_t* base, uint32_t tlog,
+                                                      uint32_t prime, uint32_t x0, uint32_t stride) {
+    const uint32_t T = 1u << tlog;
+    const uint32_t p = tb.primes[prime], p2 = 2u * p;
+    const Tw* tw = tb.fwd + (size_t(prime) << tb.max_logn);
+    constexpr int R = 1 << U;
+    for (uint32_t x = x0; x < T; x += stride) {
+        uint32_t r[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) r[j] = base[(size_t(j) << tlog) + x];
+#pragma unroll
+        for (int bit = U - 1; bit >= 0; --bit) {
+            const uint32_t m = 1u << (U - 1 - bit);
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                if (j & (1 << bit)) continue;
+                ct_bfly(r[j], r[j | (1 << bit)], tw[m + (uint32_t(j) >> (bit + 1))], p, p2);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) base[(size_t(j) << tlog) + x] = r[j];
+    }
+}
+
+template <int U>
+__device__ __forceinline__ void outer_inverse_columns(const Tables& tb, uint32_t* base, uint32_t tlog,
+                                                      uint32_t prime, uint32_t x0, uint32_t stride) {
+    const uint32_t T = 1u << tlog;
+    const uint32_t p = tb.primes[prime], p2 = 2u * p;
+    const Tw* tw = tb.inv + (size_t(prime) << tb.max_logn);
+    constexpr int R = 1 << U;
+    for (uint32_t x = x0; x < T; x += stride) {
+        uint32_t r[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) r[j] = base[(size_t(j) << tlog) + x];
+#pragma unroll
+        for (int bit = 0; bit < U; ++bit) {
+            const uint32_t m = 1u << (U - 1 - bit);
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                if (j & (1 << bit)) continue;
+                gs_bfly(r[j], r[j | (1 << bit)], tw[m + (uint32_t(j) >> (bit + 1))], p, p2);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) base[(size_t(j) << tlog) + x] = csub(r[j], p);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// persistent batch kernel
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t conv_items_of(const Job& j) {
+    const uint32_t n = 1u << j.logn;
+    return 2u * ((n + kConvPerBlock - 1) / kConvPerBlock);
+}
+
+// Spin (thread 0 only) until *counter >= target; false on error or timeout.
+__device__ bool wait_for(uint32_t* counter, uint32_t target, const BatchArgs& ba) {
+    const uint64_t start = wall_clock64();
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (__hip_atomic_load(ba.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
+        if (wall_clock64() - start > ba.wait_ticks) {
+            atomicMax(ba.error, kErrTimeout);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+__device__ bool wait_item_deps(const Tables& tb, const BatchArgs& ba, const Item& it) {
+    const uint32_t k = tb.k;
+    switch (it.type) {
+        case kItemConv:
+            return true;
+        case kItemOuterFwd:
+            return wait_for(&ba.counters[it.job].conv, conv_items_of(ba.jobs[it.job]), ba);
+        case kItemTile: {
+            const uint32_t count = it.a0 >> 8;
+            uint32_t last = 0xFFFFFFFFu;
+            for (uint32_t i = 0; i < count; ++i) {
+                const uint32_t j = ba.units[it.a1 + i].job;
+                if (j == last) continue;
+                last = j;
+                const Job job = ba.jobs[j];
+                const bool ok = job.outer == 0 ? wait_for(&ba.counters[j].conv, conv_items_of(job), ba)
+                                               : wait_for(&ba.counters[j].fwd, 2u * k, ba);
+                if (!ok) return false;
+            }
+            return true;
+        }
+        case kItemOuterInv: {
+            const Job job = ba.jobs[it.job];
+            return wait_for(&ba.counters[it.job].tile, k << job.outer, ba);
+        }
+        case kItemCrt: {
+            const Job job = ba.jobs[it.job];
+            return job.outer == 0 ? wait_for(&ba.counters[it.job].tile, k, ba)
+                                  : wait_for(&ba.counters[it.job].inv, k, ba);
+        }
+    }
+    return false;
+}
+
+template <int TLOG>
+__device__ __forceinline__ void tile_item(const Tables& tb, const BatchArgs& ba, const Item& it, uint32_t* lds) {
+    using G = Geo<TLOG>;
+    const uint32_t tile = threadIdx.x >> G::TPTLOG, tau = threadIdx.x & (G::TPT - 1);
+    const uint32_t count = it.a0 >> 8;
+    const bool valid = tile < count;
+    const Unit unit = valid ? ba.units[it.a1 + tile] : Unit{0u, 0u};
+    const Job job = ba.jobs[unit.job];
+    const uint32_t prime = unit.prime_s >> 2, s = unit.prime_s & 3u;
+    const size_t off = (size_t(prime) << job.logn) + (size_t(s) << TLOG);
+    conv_tile<TLOG>(tb, job.ra + off, job.rb + off, job.ra + off, prime, (1u << job.outer) + s, job.logn,
+                    lds + tile * G::STRIDE, tau, valid);
+}
+
+// Item handlers are separate (not inlined) functions so that each gets the
+// full register budget instead of the union of all paths.
+__device__ __noinline__ void handle_tile(const Tables& tb, const BatchArgs& ba, const Item& it, uint32_t* lds) {
+    switch (it.a0 & 0xFFu) {
+        case 5: tile_item<5>(tb, ba, it, lds); break;
+        case 6: tile_item<6>(tb, ba, it, lds); break;
+        case 7: tile_item<7>(tb, ba, it, lds); break;
+        case 8: tile_item<8>(tb, ba, it, lds); break;
+        case 9: tile_item<9>(tb, ba, it, lds); break;
+        case 10: tile_item<10>(tb, ba, it, lds); break;
+        case 11: tile_item<11>(tb, ba, it, lds); break;
+        case 12: tile_item<12>(tb, ba, it, lds); break;
+        case 13: tile_item<13>(tb, ba, it, lds); break;
+        case 14: tile_item<14>(tb, ba, it, lds); break;
+        case 15: tile_item<15>(tb, ba, it, lds); break;
+        default: break;
+    }
+}
+
+__device__ __noinline__ void handle_conv(const Tables& tb, const Job& job, const Item& it, uint32_t* lds,
+                                         bool load_weights) {
+    uint4* lds_w = reinterpret_cast<uint4*>(lds);
+    const uint32_t tid = threadIdx.x;
+    if (load_weights) {
+        for (uint32_t i = tid; i < tb.quads * 128u; i += kBlock) lds_w[i] = tb.conv_w[i];
+        __syncthreads();
+    }
+    const uint32_t n = 1u << job.logn;
+    const uint32_t c0 = it.a1 * kConvPerBlock + (tid >> 6) * kConvPerWave;
+    if (c0 < n) convert_wave(tb, it.a0 ? job.b : job.a, n, c0, it.a0 ? job.rb : job.ra, lds_w, tid & 63u);
+}
+
+__device__ __noinline__ void handle_outer(const Tables& tb, const Job& job, const Item& it) {
+    const uint32_t tid = threadIdx.x, tlog = job.logn - job.outer;
+    if (it.type == kItemOuterFwd) {
+        const uint32_t poly = it.a0 / tb.k, prime = it.a0 % tb.k;
+        uint32_t* base = (poly ? job.rb : job.ra) + (size_t(prime) << job.logn);
+        if (job.outer == 1) outer_forward_columns<1>(tb, base, tlog, prime, tid, kBlock);
+        else outer_forward_columns<2>(tb, base, tlog, prime, tid, kBlock);
+    } else {
+        uint32_t* base = job.ra + (size_t(it.a0) << job.logn);
+        if (job.outer == 1) outer_inverse_columns<1>(tb, base, tlog, it.a0, tid, kBlock);
+        else outer_inverse_columns<2>(tb, base, tlog, it.a0, tid, kBlock);
+    }
+}
+
+__device__ __noinline__ void handle_crt(const Tables& tb, const Job& job, const Item& it) {
+    const uint32_t n = 1u << job.logn;
+    const uint32_t c = it.a0 * kBlock + threadIdx.x;
+    if (c < n) crt_one(tb, job.ra, n, c, job.c);
+}
+
+__global__ void __launch_bounds__(kBlock) k_persistent(Tables tb, BatchArgs ba) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_item, s_ok;
+    bool weights_in_lds = false;
+    const uint32_t tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) {
+            uint32_t i = atomicAdd(ba.next, 1u);
+            if (__hip_atomic_load(ba.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) i = ba.n_items;
+            s_item = i;
+        }
+        __syncthreads();
+        const uint32_t idx = s_item;
+        if (idx >= ba.n_items) break;
+        const Item it = ba.items[idx];
+        if (tid == 0) s_ok = wait_item_deps(tb, ba, it) ? 1u : 0u;
+        __syncthreads();
+        if (!s_ok) continue;  // error latched: drain the queue
+        const Job job = ba.jobs[it.type == kItemTile ? 0u : it.job];
+        switch (it.type) {
+            case kItemConv:
+                handle_conv(tb, job, it, lds, !weights_in_lds);
+                weights_in_lds = true;
+                break;
+            case kItemOuterFwd:
+            case kItemOuterInv:
+                handle_outer(tb, job, it);
+                break;
+            case kItemTile:
+                weights_in_lds = false;
+                handle_tile(tb, ba, it, lds);
+                break;
+            case kItemCrt:
+                handle_crt(tb, job, it);
+                break;
+        }
+        __syncthreads();  // all results of the item stored (and LDS free)
+        if (tid == 0) {
+            __threadfence();  // release the item's results at device scope
+            switch (it.type) {
+                case kItemConv: atomicAdd(&ba.counters[it.job].conv, 1u); break;
+                case kItemOuterFwd: atomicAdd(&ba.counters[it.job].fwd, 1u); break;
+                case kItemTile: {
+                    const uint32_t count = it.a0 >> 8;
+                    for (uint32_t i = 0; i < count; ++i) atomicAdd(&ba.counters[ba.units[it.a1 + i].job].tile, 1u);
+                    break;
+                }
+                case kItemOuterInv: atomicAdd(&ba.counters[it.job].inv, 1u); break;
+                default: break;
+            }
+        }
+    }
+}
+
+// clears the claim counter, error flag and per-job counters (a kernel, so it
+// can sit in a captured graph)
+__global__ void k_reset(uint32_t* words, uint32_t count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) words[i] = 0u;
+}
+
 // Outer forward CT stages (bits logn-1 .. logn-u) for a and b: one thread per
 // column x of the (2^u x T) view; grid.y = 2*k (poly, prime).
 template <int U>
@@ -570,6 +810,16 @@ hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t 
     return e;
 }
 
+hipError_t launch_persistent(const Tables& tb, const BatchArgs& ba, uint32_t* reset_words, uint32_t reset_count,
+                             uint32_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, reset_words, reset_count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_persistent, dim3(blocks), dim3(kBlock), size_t(kLdsWords) * sizeof(uint32_t), stream, tb,
+                       ba);
+    return hipGetLastError();
+}
+
 hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
     const uint32_t n = 1u << job.logn;
     const uint32_t threads = n < kCrtPerBlock ? ((n + 63) / 64) * 64 : kCrtPerBlock;
@@ -603,6 +853,8 @@ hipError_t configure_kernels() {
     if ((e = allow_lds<13>()) != hipSuccess) return e;
     if ((e = allow_lds<14>()) != hipSuccess) return e;
     if ((e = allow_lds<15>()) != hipSuccess) return e;
-    return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_persistent),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(kLdsWords * sizeof(uint32_t)));
 }
 }  // namespace pmd

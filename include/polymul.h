@@ -84,6 +84,26 @@ int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, ui
 /* CRT + mod q from scaled residues -- rns/solve.cu:336-420 */
 int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream);
 
+/* ---- batched products (no counterpart in the reference) --------------------
+ * Many products of uneven N (powers of two, 32 <= N <= max_n) under the
+ * context's q, run as one persistent kernel over a dependency-ordered work
+ * list, captured once in a hipGraph and replayed by pm_batch_run.  With
+ * d_a/d_b/d_c all non-NULL the caller owns the n_i*28-word device buffers of
+ * product i; otherwise the batch allocates them (pm_batch_io returns them). */
+typedef struct pm_batch pm_batch;
+int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
+                    const uint32_t* const* d_b, uint32_t* const* d_c, pm_batch** out);
+void pm_batch_destroy(pm_batch* batch);
+int pm_batch_io(pm_batch* batch, uint32_t i, uint32_t** d_a, uint32_t** d_b, uint32_t** d_c);
+/* synchronous host <-> device copies of product i's operands / result */
+int pm_batch_upload(pm_batch* batch, uint32_t i, const uint32_t* a, const uint32_t* b);
+int pm_batch_download(pm_batch* batch, uint32_t i, uint32_t* c);
+/* enqueue one replay (0 = context stream); pm_batch_wait syncs and reports
+ * PM_ETIMEOUT if a dependency wait inside the kernel hit its bound */
+int pm_batch_run(pm_batch* batch, void* hip_stream);
+int pm_batch_wait(pm_batch* batch, void* hip_stream);
+int pm_batch_info(const pm_batch* batch, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes);
+
 #ifdef __cplusplus
 }
 #endif

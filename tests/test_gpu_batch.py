@@ -1,0 +1,78 @@
+"""GPU parity of the persistent batched path: uneven N in one launch, each
+product bit-exact against the oracle / GMP fixtures, replays identical."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+
+
+def sha(x):
+    return hashlib.sha256(np.ascontiguousarray(x, dtype="<u4").tobytes()).hexdigest()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_batch_uneven_sizes(pm, orc, workload):
+    q = workload.modulus(1)
+    ctx = pm.Context(q, 1 << 17)
+    ns = [4096, 32768, 1024, 8192, 32, 16384, 2048, 1024, 65536, 131072, 512]
+    say("batch uneven", ns)
+    batch = pm.Batch(ctx, ns)
+    inputs = []
+    for i, n in enumerate(ns):
+        a, b = workload.family("dense-random-1", n, q, 100 + i)
+        batch.upload(i, a, b)
+        inputs.append((a, b))
+    batch.run()
+    batch.wait()
+    first = [batch.download(i) for i in range(len(ns))]
+    batch.run()
+    batch.wait()
+    for i, (n, (a, b)) in enumerate(zip(ns, inputs)):
+        say("check", i, n)
+        assert (batch.download(i) == first[i]).all(), "replay differs"
+        if n <= 32768:
+            want = orc.rns_multiply(q, a, b, orc.reference_primes_for(n, q))
+        else:
+            want = orc.rns_multiply(q, a, b, orc.primes_general(n, q))
+        assert (first[i] == want).all(), (i, n)
+    batch.close()
+    ctx.close()
+
+
+def test_batch_golden_and_identical_products(pm, workload):
+    """Each result is the same alone (golden) and inside a batch of copies."""
+    case = [c for c in GOLDEN["cases"] if c["n"] == 32768 and c["family"] == "limb-boundaries"
+            and c["modulus"] == "seed:1"][0]
+    q = workload.modulus(1)
+    a, b = workload.family(case["family"], 32768, q, case["seed"])
+    ctx = pm.Context(q, 1 << 15)
+    batch = pm.Batch(ctx, [32768] * 6)
+    say("batch of identical products", batch.info())
+    for i in range(6):
+        batch.upload(i, a, b)
+    batch.run()
+    batch.wait()
+    for i in range(6):
+        assert sha(batch.download(i)) == case["sha256"]["c"], i
+    batch.close()
+    ctx.close()
+
+
+def test_batch_rejects_small_n(pm, workload):
+    q = workload.modulus(1)
+    ctx = pm.Context(q, 1 << 10)
+    with pytest.raises(pm.PolymulError):
+        pm.Batch(ctx, [16, 1024])
+    with pytest.raises(pm.PolymulError):
+        pm.Batch(ctx, [2048])
+    ctx.close()
