@@ -66,7 +66,7 @@ def library():
     lib.pm_stage_convert.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
     lib.pm_stage_crt.argtypes = [P, u32, P, P, P]
-    lib.pm_batch_create.argtypes = [P, u32, P, P, P, P, ctypes.POINTER(P)]
+    lib.pm_batch_create.argtypes = [P, u32, P, P, P, P, u32, ctypes.POINTER(P)]
     lib.pm_batch_destroy.argtypes = [P]
     lib.pm_batch_destroy.restype = None
     lib.pm_batch_io.argtypes = [P, u32, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P)]
@@ -74,7 +74,14 @@ def library():
     lib.pm_batch_download.argtypes = [P, u32, P]
     lib.pm_batch_run.argtypes = [P, P]
     lib.pm_batch_wait.argtypes = [P, P]
+    lib.pm_plan_worklist.argtypes = [u32, u32, P, ctypes.POINTER(u32), P, ctypes.POINTER(u32), P]
     lib.pm_batch_info.argtypes = [P, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_uint64)]
+    lib.pm_server_create.argtypes = [P, ctypes.c_uint64, u32, u32, ctypes.POINTER(P)]
+    lib.pm_server_submit.argtypes = [P, u32, P, P, P, ctypes.POINTER(ctypes.c_uint64)]
+    lib.pm_server_wait.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    lib.pm_server_stats.argtypes = [P] + [ctypes.POINTER(ctypes.c_uint64)] * 4
+    lib.pm_server_destroy.argtypes = [P]
+    lib.pm_server_destroy.restype = None
     for name in EXPORTED:
         getattr(lib, name)  # every declared symbol must resolve
     _LIB = lib
@@ -85,7 +92,8 @@ def library():
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
             "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_ntt", "pm_stage_crt",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
-            "pm_batch_run", "pm_batch_wait", "pm_batch_info"]
+            "pm_batch_run", "pm_batch_wait", "pm_batch_info", "pm_plan_worklist",
+            "pm_server_create", "pm_server_submit", "pm_server_wait", "pm_server_stats", "pm_server_destroy"]
 
 
 def _ptr(x):
@@ -111,6 +119,23 @@ def plan_rns(q, max_n):
     if status != PM_OK:
         raise PolymulError("pm_plan_rns", status, lib.pm_last_error(None).decode())
     return k.value, primes[:k.value].copy(), psi[:k.value].copy(), bits.value
+
+
+def plan_worklist(k, ns):
+    """Host-only work list of a batch: (items [n,4] uint32, units [m,2] uint32)."""
+    lib = library()
+    ns_arr = np.ascontiguousarray(ns, dtype=np.uint32)
+    ni, nu = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    st = lib.pm_plan_worklist(k, len(ns_arr), ns_arr.ctypes.data, ctypes.byref(ni), None, ctypes.byref(nu), None)
+    if st != PM_OK:
+        raise PolymulError("pm_plan_worklist", st, "invalid sizes")
+    items = np.zeros((ni.value, 4), dtype=np.uint32)
+    units = np.zeros((max(nu.value, 1), 2), dtype=np.uint32)
+    st = lib.pm_plan_worklist(k, len(ns_arr), ns_arr.ctypes.data, ctypes.byref(ni), items.ctypes.data,
+                              ctypes.byref(nu), units.ctypes.data)
+    if st != PM_OK:
+        raise PolymulError("pm_plan_worklist", st, "invalid sizes")
+    return items, units[:nu.value]
 
 
 class Context:
@@ -196,7 +221,9 @@ class Context:
 class Batch:
     """Products of uneven N in one persistent, graph-captured launch."""
 
-    def __init__(self, ctx, ns, d_a=None, d_b=None, d_c=None):
+    PERSISTENT, PHASED = 1, 2
+
+    def __init__(self, ctx, ns, d_a=None, d_b=None, d_c=None, mode="phased"):
         self._ctx, self._lib = ctx, ctx._lib
         self.ns = [int(n) for n in ns]
         count = len(self.ns)
@@ -205,8 +232,10 @@ class Batch:
         if d_a is not None:
             ptrs = [(ctypes.c_void_p * count)(*[_ptr(x) for x in v]) for v in (d_a, d_b, d_c)]
         handle = ctypes.c_void_p()
+        flags = {"phased": self.PHASED, "persistent": self.PERSISTENT}[mode]
         ctx._check("pm_batch_create", self._lib.pm_batch_create(ctx.handle, count, arr, ptrs[0], ptrs[1], ptrs[2],
-                                                                 ctypes.byref(handle)))
+                                                                 flags, ctypes.byref(handle)))
+        self.mode = mode
         self._h = handle
 
     def close(self):
@@ -242,6 +271,54 @@ class Batch:
         self._ctx._check("pm_batch_info", self._lib.pm_batch_info(self._h, ctypes.byref(items), ctypes.byref(blocks),
                                                                    ctypes.byref(nbytes)))
         return {"items": items.value, "blocks": blocks.value, "device_bytes": nbytes.value}
+
+
+class Server:
+    """Continuous-batching request loop (pm_server_*)."""
+
+    def __init__(self, ctx, hbm_budget_bytes, max_batch=64, max_inflight=2):
+        self._ctx, self._lib = ctx, ctx._lib
+        handle = ctypes.c_void_p()
+        ctx._check("pm_server_create", self._lib.pm_server_create(ctx.handle, int(hbm_budget_bytes), max_batch,
+                                                                   max_inflight, ctypes.byref(handle)))
+        self._h = handle
+        self._keep = {}
+
+    def submit(self, a, b):
+        """Queue a*b; returns a request id.  The arrays stay referenced until wait()."""
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        c = np.empty_like(a)
+        rid = ctypes.c_uint64()
+        self._ctx._check("pm_server_submit", self._lib.pm_server_submit(self._h, a.shape[0], a.ctypes.data,
+                                                                         b.ctypes.data, c.ctypes.data,
+                                                                         ctypes.byref(rid)))
+        self._keep[rid.value] = (a, b, c)
+        return rid.value
+
+    def wait(self, rid):
+        """Block until request rid is done: (c, latency_seconds)."""
+        lat = ctypes.c_double()
+        st = self._lib.pm_server_wait(self._h, rid, ctypes.byref(lat))
+        a, b, c = self._keep.pop(rid)
+        self._ctx._check("pm_server_wait", st)
+        return c, lat.value
+
+    def stats(self):
+        vals = [ctypes.c_uint64() for _ in range(4)]
+        self._ctx._check("pm_server_stats", self._lib.pm_server_stats(self._h, *[ctypes.byref(v) for v in vals]))
+        return dict(zip(["completed", "launches", "max_concurrent", "peak_bytes"], [v.value for v in vals]))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Solver:

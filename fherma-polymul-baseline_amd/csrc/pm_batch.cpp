@@ -77,43 +77,10 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
         j.ra = res;
         j.rb = res + size_t(k) * n;
     }
-    // work list: conversions (largest N first), outer forward stages, tiles by
-    // size class, outer inverse stages, CRT blocks -- every dependency points
-    // to an earlier item, so in-order claiming cannot deadlock
-    std::vector<uint32_t> order(count);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return b->logn[x] > b->logn[y]; });
     std::vector<pmd::Item> items;
     std::vector<pmd::Unit> units;
-    for (uint32_t j : order) {
-        const uint32_t blocks = ceil_div(1u << b->logn[j], pmd::kConvPerBlock);
-        for (uint32_t blk = 0; blk < blocks; ++blk)
-            for (uint32_t poly = 0; poly < 2; ++poly) items.push_back({pmd::kItemConv, j, poly, blk});
-    }
-    for (uint32_t j : order)
-        if (jobs[j].outer)
-            for (uint32_t x = 0; x < 2 * k; ++x) items.push_back({pmd::kItemOuterFwd, j, x, 0});
-    for (uint32_t tlog = 15; tlog >= 5; --tlog) {
-        const uint32_t per_item = 1u << (15 - tlog);  // 1024 threads x 32 residues / tile size
-        std::vector<pmd::Unit> cls;
-        for (uint32_t j : order) {
-            if (jobs[j].logn - jobs[j].outer != tlog) continue;
-            for (uint32_t prime = 0; prime < k; ++prime)
-                for (uint32_t s = 0; s < (1u << jobs[j].outer); ++s) cls.push_back({j, (prime << 2) | s});
-        }
-        for (size_t first = 0; first < cls.size(); first += per_item) {
-            const uint32_t n_units = uint32_t(std::min<size_t>(per_item, cls.size() - first));
-            items.push_back({pmd::kItemTile, 0, tlog | (n_units << 8), uint32_t(units.size())});
-            units.insert(units.end(), cls.begin() + first, cls.begin() + first + n_units);
-        }
-    }
-    for (uint32_t j : order)
-        if (jobs[j].outer)
-            for (uint32_t prime = 0; prime < k; ++prime) items.push_back({pmd::kItemOuterInv, j, prime, 0});
-    for (uint32_t j : order) {
-        const uint32_t blocks = ceil_div(1u << b->logn[j], pmd::kBlock);
-        for (uint32_t blk = 0; blk < blocks; ++blk) items.push_back({pmd::kItemCrt, j, blk, 0});
-    }
+    pm_build_worklist(k, jobs.data(), count, true, items, units);
+    if (!pm_phase_ranges(items, b->ranges)) return pm_fail(ctx, PM_EINVAL, "work list phases are not contiguous");
     if (units.empty()) units.push_back({0, 0});
     b->n_items = uint32_t(items.size());
 
@@ -155,7 +122,9 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     if (e != hipSuccess) return pm_hip(ctx, e, "capture stream");
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipError_t le = pmd::launch_persistent(ctx->tables, ba, b->d_words, b->reset_count, b->blocks, cap);
+        hipError_t le = b->persistent
+                            ? pmd::launch_persistent(ctx->tables, ba, b->d_words, b->reset_count, b->blocks, cap)
+                            : pmd::launch_phased(ctx->tables, ba, b->ranges, cap);
         hipError_t ce = hipStreamEndCapture(cap, &b->graph);
         e = le != hipSuccess ? le : ce;
     }
@@ -165,11 +134,84 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     return PM_OK;
 }
 
+bool pm_phase_ranges(const std::vector<pmd::Item>& items, pmd::PhaseRanges& r) {
+    r = pmd::PhaseRanges{};
+    // each phase (and each tile class) must be one contiguous run of items
+    auto add = [](uint32_t i, uint32_t& first, uint32_t& count) {
+        if (count == 0) first = i;
+        else if (first + count != i) return false;
+        ++count;
+        return true;
+    };
+    for (uint32_t i = 0; i < items.size(); ++i) {
+        const pmd::Item& it = items[i];
+        bool ok = true;
+        switch (it.type) {
+            case pmd::kItemConv: ok = add(i, r.conv_first, r.conv_count); break;
+            case pmd::kItemOuterFwd: ok = add(i, r.fwd_first, r.fwd_count); break;
+            case pmd::kItemTile: {
+                const uint32_t tlog = it.a0 & 0xFFu;
+                ok = tlog >= 5 && tlog <= 15 && add(i, r.tile_first[tlog], r.tile_count[tlog]);
+                break;
+            }
+            case pmd::kItemOuterInv: ok = add(i, r.inv_first, r.inv_count); break;
+            case pmd::kItemCrt: ok = add(i, r.crt_first, r.crt_count); break;
+            default: ok = false;
+        }
+        if (!ok) return false;
+    }
+    return true;
+}
+
+void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool largest_first,
+                       std::vector<pmd::Item>& items, std::vector<pmd::Unit>& units) {
+    // Order: conversions, outer forward stages, fused tiles by size class,
+    // outer inverse stages, CRT blocks.  Every dependency of an item points to
+    // an earlier item, so claiming items in order cannot deadlock.
+    std::vector<uint32_t> order(count);
+    std::iota(order.begin(), order.end(), 0u);
+    if (largest_first)
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t x, uint32_t y) { return jobs[x].logn > jobs[y].logn; });
+    items.clear();
+    units.clear();
+    for (uint32_t j : order) {
+        const uint32_t blocks = ceil_div(1u << jobs[j].logn, pmd::kConvPerBlock);
+        for (uint32_t blk = 0; blk < blocks; ++blk)
+            for (uint32_t poly = 0; poly < 2; ++poly) items.push_back({pmd::kItemConv, j, poly, blk});
+    }
+    for (uint32_t j : order)
+        if (jobs[j].outer)
+            for (uint32_t x = 0; x < 2 * k; ++x) items.push_back({pmd::kItemOuterFwd, j, x, 0});
+    for (uint32_t tlog = 15; tlog >= 5; --tlog) {
+        const uint32_t per_item = 1u << (15 - tlog);  // 1024 threads x 32 residues / tile size
+        std::vector<pmd::Unit> cls;
+        for (uint32_t j : order) {
+            if (jobs[j].logn - jobs[j].outer != tlog) continue;
+            for (uint32_t prime = 0; prime < k; ++prime)
+                for (uint32_t s = 0; s < (1u << jobs[j].outer); ++s) cls.push_back({j, (prime << 2) | s});
+        }
+        for (size_t first = 0; first < cls.size(); first += per_item) {
+            const uint32_t n_units = uint32_t(std::min<size_t>(per_item, cls.size() - first));
+            items.push_back({pmd::kItemTile, 0, tlog | (n_units << 8), uint32_t(units.size())});
+            units.insert(units.end(), cls.begin() + first, cls.begin() + first + n_units);
+        }
+    }
+    for (uint32_t j : order)
+        if (jobs[j].outer)
+            for (uint32_t prime = 0; prime < k; ++prime) items.push_back({pmd::kItemOuterInv, j, prime, 0});
+    for (uint32_t j : order) {
+        const uint32_t blocks = ceil_div(1u << jobs[j].logn, pmd::kBlock);
+        for (uint32_t blk = 0; blk < blocks; ++blk) items.push_back({pmd::kItemCrt, j, blk, 0});
+    }
+}
+
 extern "C" {
 
 int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
-                    const uint32_t* const* d_b, uint32_t* const* d_c, pm_batch** out) {
+                    const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out) {
     if (!ctx || !ns || !out || count == 0) return PM_EINVAL;
+    if (flags & ~uint32_t(PM_BATCH_PERSISTENT | PM_BATCH_PHASED)) return PM_EINVAL;
     *out = nullptr;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return pm_hip(ctx, e, "set device");
@@ -177,6 +219,7 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
     if (!b) return pm_fail(ctx, PM_ENOMEM, "host allocation failed");
     b->ctx = ctx;
     b->count = count;
+    b->persistent = (flags & PM_BATCH_PERSISTENT) != 0;
     const int st = pm_batch_build(ctx, b, ns, d_a, d_b, d_c);
     if (st != PM_OK) {
         delete b;
@@ -233,9 +276,35 @@ int pm_batch_wait(pm_batch* b, void* hip_stream) {
     return PM_OK;
 }
 
+int pm_plan_worklist(uint32_t k, uint32_t count, const uint32_t* ns, uint32_t* n_items, uint32_t* items,
+                     uint32_t* n_units, uint32_t* units) {
+    if (!ns || !n_items || !n_units || k == 0 || k > 64) return PM_EINVAL;
+    std::vector<pmd::Job> jobs(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (ns[i] < 32 || (ns[i] & (ns[i] - 1)) || ns[i] > (1u << 17)) return PM_EUNSUPPORTED;
+        jobs[i].logn = uint32_t(__builtin_ctz(ns[i]));
+        jobs[i].outer = jobs[i].logn > 15 ? jobs[i].logn - 15 : 0;
+    }
+    std::vector<pmd::Item> it;
+    std::vector<pmd::Unit> un;
+    pm_build_worklist(k, jobs.data(), count, true, it, un);
+    pmd::PhaseRanges r;
+    if (!pm_phase_ranges(it, r)) return PM_EINVAL;
+    const uint32_t cap_items = *n_items, cap_units = *n_units;
+    *n_items = uint32_t(it.size());
+    *n_units = uint32_t(un.size());
+    if (items && cap_items >= it.size()) std::memcpy(items, it.data(), it.size() * sizeof(pmd::Item));
+    if (units && cap_units >= un.size()) std::memcpy(units, un.data(), un.size() * sizeof(pmd::Unit));
+    return PM_OK;
+}
+
 int pm_batch_info(const pm_batch* b, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes) {
     if (!b) return PM_EINVAL;
     if (items) *items = b->n_items;
+    if (blocks && !b->persistent) {
+        *blocks = 0;  // phased: one launch per phase, sized to its items
+        blocks = nullptr;
+    }
     if (blocks) *blocks = b->blocks;
     if (device_bytes) *device_bytes = b->device_bytes;
     return PM_OK;

@@ -6,8 +6,15 @@
 #include <vector>
 
 #include "pm_device.h"
+#include "pm_kernels.h"
 
 struct pm_context;
+
+// Dependency-ordered work list of a set of products (shared by the batch and
+// the serving loop).
+bool pm_phase_ranges(const std::vector<pmd::Item>& items, pmd::PhaseRanges& ranges);
+void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool largest_first,
+                       std::vector<pmd::Item>& items, std::vector<pmd::Unit>& units);
 
 struct pm_batch {
     pm_context* ctx = nullptr;
@@ -15,6 +22,8 @@ struct pm_batch {
     std::vector<uint32_t> logn;
     std::vector<pmd::Job> jobs;     // host copy (device pointers)
     pmd::BatchArgs args{};
+    pmd::PhaseRanges ranges{};
+    bool persistent = false;        // one persistent launch, else one launch per phase
     uint32_t* d_words = nullptr;    // claim counter, error flag, per-job counters
     std::vector<void*> allocations;
     uint64_t device_bytes = 0;

@@ -737,6 +737,53 @@ __global__ void __launch_bounds__(kCrtPerBlock) k_crt(Tables tb, Job job) {
 }
 
 // ---------------------------------------------------------------------------
+// phased batch: one launch per phase over a contiguous range of the work list
+// (the launches order the phases; block b runs item first + b)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, uint32_t first) {
+    __shared__ uint4 lds_w[16 * 2 * 64];
+    for (uint32_t i = threadIdx.x; i < tb.quads * 128u; i += blockDim.x) lds_w[i] = tb.conv_w[i];
+    __syncthreads();
+    const Item it = ba.items[first + blockIdx.x];
+    const Job job = ba.jobs[it.job];
+    const uint32_t n = 1u << job.logn;
+    const uint32_t c0 = it.a1 * kConvPerBlock + (threadIdx.x >> 6) * kConvPerWave;
+    if (c0 >= n) return;
+    convert_wave(tb, it.a0 ? job.b : job.a, n, c0, it.a0 ? job.rb : job.ra, lds_w, threadIdx.x & 63u);
+}
+
+__global__ void __launch_bounds__(kBlock) k_items_outer(Tables tb, BatchArgs ba, uint32_t first) {
+    const Item it = ba.items[first + blockIdx.x];
+    const Job job = ba.jobs[it.job];
+    const uint32_t tlog = job.logn - job.outer;
+    if (it.type == kItemOuterFwd) {
+        const uint32_t poly = it.a0 / tb.k, prime = it.a0 % tb.k;
+        uint32_t* base = (poly ? job.rb : job.ra) + (size_t(prime) << job.logn);
+        if (job.outer == 1) outer_forward_columns<1>(tb, base, tlog, prime, threadIdx.x, kBlock);
+        else outer_forward_columns<2>(tb, base, tlog, prime, threadIdx.x, kBlock);
+    } else {
+        uint32_t* base = job.ra + (size_t(it.a0) << job.logn);
+        if (job.outer == 1) outer_inverse_columns<1>(tb, base, tlog, it.a0, threadIdx.x, kBlock);
+        else outer_inverse_columns<2>(tb, base, tlog, it.a0, threadIdx.x, kBlock);
+    }
+}
+
+template <int TLOG>
+__global__ void __launch_bounds__(kBlock) k_items_tile(Tables tb, BatchArgs ba, uint32_t first) {
+    extern __shared__ uint32_t lds[];
+    const Item it = ba.items[first + blockIdx.x];
+    tile_item<TLOG>(tb, ba, it, lds);
+}
+
+__global__ void __launch_bounds__(kBlock) k_items_crt(Tables tb, BatchArgs ba, uint32_t first) {
+    const Item it = ba.items[first + blockIdx.x];
+    const Job job = ba.jobs[it.job];
+    const uint32_t n = 1u << job.logn;
+    const uint32_t c = it.a0 * kBlock + threadIdx.x;
+    if (c < n) crt_one(tb, job.ra, n, c, job.c);
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 template <int TLOG>
@@ -820,6 +867,38 @@ hipError_t launch_persistent(const Tables& tb, const BatchArgs& ba, uint32_t* re
     return hipGetLastError();
 }
 
+template <int TLOG>
+static void launch_items_tile(const Tables& tb, const BatchArgs& ba, uint32_t first, uint32_t count,
+                              hipStream_t stream) {
+    hipLaunchKernelGGL(k_items_tile<TLOG>, dim3(count), dim3(kBlock), size_t(kLdsWords) * sizeof(uint32_t), stream,
+                       tb, ba, first);
+}
+
+hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& r, hipStream_t stream) {
+    if (r.conv_count) hipLaunchKernelGGL(k_items_conv, dim3(r.conv_count), dim3(kBlock), 0, stream, tb, ba, r.conv_first);
+    if (r.fwd_count) hipLaunchKernelGGL(k_items_outer, dim3(r.fwd_count), dim3(kBlock), 0, stream, tb, ba, r.fwd_first);
+    for (uint32_t tlog = 15; tlog >= 5; --tlog) {
+        const uint32_t first = r.tile_first[tlog], count = r.tile_count[tlog];
+        if (!count) continue;
+        switch (tlog) {
+            case 5: launch_items_tile<5>(tb, ba, first, count, stream); break;
+            case 6: launch_items_tile<6>(tb, ba, first, count, stream); break;
+            case 7: launch_items_tile<7>(tb, ba, first, count, stream); break;
+            case 8: launch_items_tile<8>(tb, ba, first, count, stream); break;
+            case 9: launch_items_tile<9>(tb, ba, first, count, stream); break;
+            case 10: launch_items_tile<10>(tb, ba, first, count, stream); break;
+            case 11: launch_items_tile<11>(tb, ba, first, count, stream); break;
+            case 12: launch_items_tile<12>(tb, ba, first, count, stream); break;
+            case 13: launch_items_tile<13>(tb, ba, first, count, stream); break;
+            case 14: launch_items_tile<14>(tb, ba, first, count, stream); break;
+            case 15: launch_items_tile<15>(tb, ba, first, count, stream); break;
+        }
+    }
+    if (r.inv_count) hipLaunchKernelGGL(k_items_outer, dim3(r.inv_count), dim3(kBlock), 0, stream, tb, ba, r.inv_first);
+    if (r.crt_count) hipLaunchKernelGGL(k_items_crt, dim3(r.crt_count), dim3(kBlock), 0, stream, tb, ba, r.crt_first);
+    return hipGetLastError();
+}
+
 hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
     const uint32_t n = 1u << job.logn;
     const uint32_t threads = n < kCrtPerBlock ? ((n + 63) / 64) * 64 : kCrtPerBlock;
@@ -836,6 +915,13 @@ static hipError_t allow_lds() {
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                int(kLdsWords * sizeof(uint32_t)));
 }
+template <int TLOG>
+static hipError_t allow_items_lds() {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_tile<TLOG>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(kLdsWords * sizeof(uint32_t)));
+}
+
 hipError_t configure_kernels() {
     hipError_t e = hipSuccess;
     if ((e = allow_lds<1>()) != hipSuccess) return e;
@@ -853,6 +939,17 @@ hipError_t configure_kernels() {
     if ((e = allow_lds<13>()) != hipSuccess) return e;
     if ((e = allow_lds<14>()) != hipSuccess) return e;
     if ((e = allow_lds<15>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<5>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<6>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<7>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<8>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<9>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<10>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<11>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<12>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<13>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<14>()) != hipSuccess) return e;
+    if ((e = allow_items_lds<15>()) != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_persistent),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                int(kLdsWords * sizeof(uint32_t)));

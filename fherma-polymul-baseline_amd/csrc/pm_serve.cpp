@@ -1,0 +1,440 @@
+// Continuous-batching request loop (no counterpart in the reference, which
+// serves one product per fherma_run call, rns/solve.cu:721-853).
+//
+// Requests of different N under one q are submitted at any time.  A native
+// scheduler thread admits them in arrival order while their device memory
+// fits an HBM budget (first-fit over one arena), groups whatever has been
+// admitted into the next persistent launch (the batch kernel of pm_batch.cpp
+// fed with a freshly written work list), and keeps up to `max_inflight`
+// launches running at once on separate streams, so requests that arrive
+// while others are on the GPU start as soon as memory and a launch slot free
+// up, and their work overlaps on the device.  Tables stay resident in the
+// context.  Each result depends only on its own inputs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "pm_batch.h"
+#include "pm_device.h"
+#include "pm_kernels.h"
+#include "pm_runtime.h"
+#include "polymul.h"
+
+using Clock = std::chrono::steady_clock;
+using pmd::kLimbs;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over [0, size) with coalescing frees.
+class Arena {
+public:
+    void reset(size_t size) {
+        free_.clear();
+        free_[0] = size;
+        used_ = peak_ = 0;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = align_up(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes, start = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[start] = rest;
+            used_ += bytes;
+            peak_ = std::max(peak_, used_);
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align_up(bytes);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t peak() const { return peak_; }
+    size_t used() const { return used_; }
+
+private:
+    std::map<size_t, size_t> free_;
+    size_t used_ = 0, peak_ = 0;
+};
+
+struct Request {
+    uint64_t id = 0;
+    uint32_t logn = 0;
+    const uint32_t* a = nullptr;
+    const uint32_t* b = nullptr;
+    uint32_t* c = nullptr;
+    size_t offset = 0, bytes = 0;  // arena slice: a, b, c, ra, rb
+    Clock::time_point submitted, finished;
+    bool done = false;
+    int status = PM_OK;
+};
+
+struct Launch {
+    hipStream_t stream = nullptr;
+    hipEvent_t finished = nullptr;
+    pmd::Job* d_jobs = nullptr;
+    pmd::Item* d_items = nullptr;
+    pmd::Unit* d_units = nullptr;
+    uint32_t* d_words = nullptr;
+    // pinned staging of the work list (async upload)
+    pmd::Job* h_jobs = nullptr;
+    pmd::Item* h_items = nullptr;
+    pmd::Unit* h_units = nullptr;
+    std::vector<Request*> requests;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct pm_server {
+    pm_context* ctx = nullptr;
+    uint32_t max_batch = 0, max_inflight = 0, blocks = 0;
+    size_t max_items = 0, max_units = 0;
+    uint64_t budget = 0;
+    uint8_t* arena_base = nullptr;
+    Arena arena;
+    std::vector<Launch> launches;
+    uint64_t wait_ticks = 0;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;                   // submitted, not admitted
+    std::deque<Request*> admitted;                  // memory assigned, not launched
+    std::unordered_map<uint64_t, Request*> all;     // until waited for
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    // statistics
+    uint64_t completed = 0, launches_done = 0, max_concurrent = 0, failed = 0;
+    std::string error;
+
+    size_t request_bytes(uint32_t logn) const {
+        const size_t n = size_t(1) << logn;
+        return align_up(3 * n * kLimbs * 4) + align_up(2 * size_t(ctx->params.k) * n * 4);
+    }
+    void run();
+    bool start_launch(Launch& L, std::vector<Request*>& reqs);
+    void finish_launch(Launch& L);
+};
+
+bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
+    const uint32_t k = ctx->params.k;
+    std::vector<pmd::Item> items;
+    std::vector<pmd::Unit> units;
+    const uint32_t count = uint32_t(reqs.size());
+    for (uint32_t j = 0; j < count; ++j) {
+        Request* r = reqs[j];
+        const size_t n = size_t(1) << r->logn;
+        uint8_t* base = arena_base + r->offset;
+        pmd::Job& job = L.h_jobs[j];
+        job.logn = r->logn;
+        job.outer = r->logn > 15 ? r->logn - 15 : 0;
+        job.a = reinterpret_cast<const uint32_t*>(base);
+        job.b = job.a + n * kLimbs;
+        job.c = reinterpret_cast<uint32_t*>(base) + 2 * n * kLimbs;
+        job.ra = reinterpret_cast<uint32_t*>(base + align_up(3 * n * kLimbs * 4));
+        job.rb = job.ra + size_t(k) * n;
+    }
+    pm_build_worklist(k, L.h_jobs, count, false, items, units);
+    if (items.size() > max_items || units.size() > max_units) {
+        error = "work list exceeds the launch slot";
+        return false;
+    }
+    std::memcpy(L.h_items, items.data(), items.size() * sizeof(pmd::Item));
+    std::memcpy(L.h_units, units.data(), std::max<size_t>(units.size(), 1) * sizeof(pmd::Unit));
+    hipError_t e = hipSuccess;
+    for (uint32_t j = 0; j < count && e == hipSuccess; ++j) {
+        const size_t bytes = (size_t(1) << reqs[j]->logn) * kLimbs * 4;
+        e = hipMemcpyAsync(const_cast<uint32_t*>(L.h_jobs[j].a), reqs[j]->a, bytes, hipMemcpyHostToDevice, L.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(const_cast<uint32_t*>(L.h_jobs[j].b), reqs[j]->b, bytes, hipMemcpyHostToDevice,
+                               L.stream);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(L.d_jobs, L.h_jobs, count * sizeof(pmd::Job), hipMemcpyHostToDevice, L.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(L.d_items, L.h_items, items.size() * sizeof(pmd::Item), hipMemcpyHostToDevice, L.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(L.d_units, L.h_units, std::max<size_t>(units.size(), 1) * sizeof(pmd::Unit),
+                           hipMemcpyHostToDevice, L.stream);
+    pmd::BatchArgs ba{};
+    ba.jobs = L.d_jobs;
+    ba.items = L.d_items;
+    ba.units = L.d_units;
+    ba.next = L.d_words;
+    ba.error = L.d_words + 1;
+    ba.counters = reinterpret_cast<pmd::JobCounters*>(L.d_words + 2);
+    ba.n_items = uint32_t(items.size());
+    ba.wait_ticks = wait_ticks;
+    pmd::PhaseRanges ranges;
+    if (!pm_phase_ranges(items, ranges)) {
+        error = "work list phases are not contiguous";
+        return false;
+    }
+    if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, L.stream);
+    if (e == hipSuccess) e = hipEventRecord(L.finished, L.stream);
+    if (e != hipSuccess) {
+        error = std::string("launch: ") + hipGetErrorString(e);
+        return false;
+    }
+    L.requests = reqs;
+    L.busy = true;
+    return true;
+}
+
+void pm_server::finish_launch(Launch& L) {
+    // phased launches have no in-kernel waits, so no device error flag to read
+    const uint32_t err = 0;
+    hipError_t e = hipStreamSynchronize(L.stream);
+    // results to the callers' buffers (the kernel has finished: plain copies)
+    for (size_t j = 0; j < L.requests.size() && e == hipSuccess; ++j) {
+        const size_t bytes = (size_t(1) << L.requests[j]->logn) * kLimbs * 4;
+        e = hipMemcpy(L.requests[j]->c, L.h_jobs[j].c, bytes, hipMemcpyDeviceToHost);
+    }
+    const auto now = Clock::now();
+    std::lock_guard<std::mutex> lock(mu);
+    for (Request* r : L.requests) {
+        r->finished = now;
+        r->status = e != hipSuccess ? PM_EDEVICE : (err ? PM_ETIMEOUT : PM_OK);
+        if (r->status != PM_OK) ++failed;
+        r->done = true;
+        arena.release(r->offset, r->bytes);
+        ++completed;
+    }
+    L.requests.clear();
+    L.busy = false;
+    ++launches_done;
+    cv_done.notify_all();
+    cv_work.notify_all();
+}
+
+void pm_server::run() {
+    (void)hipSetDevice(ctx->device);
+    for (;;) {
+        std::vector<Request*> batch;
+        Launch* slot = nullptr;
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            // admission: arrival order, while the head fits the budget
+            while (!pending.empty()) {
+                Request* r = pending.front();
+                size_t off = 0;
+                if (!arena.alloc(r->bytes, &off)) break;
+                r->offset = off;
+                admitted.push_back(r);
+                pending.pop_front();
+            }
+            bool any_busy = false;
+            for (Launch& L : launches) {
+                if (L.busy) any_busy = true;
+                else if (!slot) slot = &L;
+            }
+            if (slot && !admitted.empty()) {
+                while (!admitted.empty() && batch.size() < max_batch) {
+                    batch.push_back(admitted.front());
+                    admitted.pop_front();
+                }
+            } else if (!any_busy) {
+                if (stop && pending.empty() && admitted.empty()) return;
+                cv_work.wait_for(lock, std::chrono::milliseconds(2));
+                continue;
+            }
+        }
+        if (!batch.empty()) {
+            if (!start_launch(*slot, batch)) {
+                std::lock_guard<std::mutex> lock(mu);
+                for (Request* r : batch) {
+                    r->status = PM_EDEVICE;
+                    r->done = true;
+                    r->finished = Clock::now();
+                    arena.release(r->offset, r->bytes);
+                    ++failed;
+                }
+                cv_done.notify_all();
+            } else {
+                uint64_t busy = 0;
+                for (Launch& L : launches) busy += L.busy;
+                std::lock_guard<std::mutex> lock(mu);
+                max_concurrent = std::max(max_concurrent, busy);
+            }
+            continue;  // admit / launch more before polling completions
+        }
+        // poll completions; yield briefly when nothing finished
+        bool progressed = false;
+        for (Launch& L : launches) {
+            if (!L.busy) continue;
+            if (hipEventQuery(L.finished) == hipSuccess) {
+                finish_launch(L);
+                progressed = true;
+            }
+        }
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+extern "C" {
+
+int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_batch, uint32_t max_inflight,
+                     pm_server** out) {
+    if (!ctx || !out || max_batch == 0 || max_inflight == 0) return PM_EINVAL;
+    *out = nullptr;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return pm_hip(ctx, e, "set device");
+    pm_server* s = new (std::nothrow) pm_server();
+    if (!s) return pm_fail(ctx, PM_ENOMEM, "host allocation failed");
+    s->ctx = ctx;
+    s->budget = hbm_budget_bytes;
+    s->max_batch = max_batch;
+    s->max_inflight = max_inflight;
+    if (s->request_bytes(ctx->max_logn) > hbm_budget_bytes) {
+        delete s;
+        return pm_fail(ctx, PM_EBUDGET, "HBM budget below one product of max_n");
+    }
+    e = hipMalloc(reinterpret_cast<void**>(&s->arena_base), hbm_budget_bytes);
+    if (e != hipSuccess) {
+        delete s;
+        return pm_hip(ctx, e, "allocate serving arena");
+    }
+    s->arena.reset(hbm_budget_bytes);
+    const uint32_t k = ctx->params.k, n = 1u << ctx->max_logn;
+    const size_t per_req_items = 3 * ((n + pmd::kBlock - 1) / pmd::kBlock) + 3 * k + size_t(k) * 4 + 1;
+    s->max_items = per_req_items * max_batch + 64;
+    s->max_units = size_t(k) * 4 * max_batch + 1;
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    s->wait_ticks = uint64_t(rate_khz) * 1000u * 10u;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, ctx->device);
+    s->blocks = e == hipSuccess ? uint32_t(prop.multiProcessorCount) : 256u;
+    s->launches.resize(max_inflight);
+    for (Launch& L : s->launches) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&L.finished, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L.d_jobs), max_batch * sizeof(pmd::Job));
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L.d_items), s->max_items * sizeof(pmd::Item));
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L.d_units), s->max_units * sizeof(pmd::Unit));
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L.d_words), (2 + 4 * size_t(max_batch)) * 4);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_jobs), max_batch * sizeof(pmd::Job));
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_items), s->max_items * sizeof(pmd::Item));
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_units), s->max_units * sizeof(pmd::Unit));
+    }
+    if (e != hipSuccess) {
+        const int st = pm_hip(ctx, e, "serving launch slots");
+        pm_server_destroy(s);
+        return st;
+    }
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return PM_OK;
+}
+
+int pm_server_submit(pm_server* s, uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t* c, uint64_t* id) {
+    if (!s || !a || !b || !c || !id) return PM_EINVAL;
+    if (n < 32 || (n & (n - 1))) return pm_fail(s->ctx, PM_EUNSUPPORTED, "served sizes are powers of two >= 32");
+    const uint32_t logn = uint32_t(__builtin_ctz(n));
+    if (logn > s->ctx->max_logn) return pm_fail(s->ctx, PM_EUNSUPPORTED, "N exceeds the context's max_n");
+    auto* r = new (std::nothrow) Request();
+    if (!r) return PM_ENOMEM;
+    r->logn = logn;
+    r->a = a;
+    r->b = b;
+    r->c = c;
+    r->bytes = s->request_bytes(logn);
+    if (r->bytes > s->budget) {
+        delete r;
+        return pm_fail(s->ctx, PM_EBUDGET, "request exceeds the HBM budget");
+    }
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        r->id = s->next_id++;
+        r->submitted = Clock::now();
+        s->pending.push_back(r);
+        s->all[r->id] = r;
+    }
+    s->cv_work.notify_all();
+    *id = r->id;
+    return PM_OK;
+}
+
+int pm_server_wait(pm_server* s, uint64_t id, double* latency_seconds) {
+    if (!s) return PM_EINVAL;
+    std::unique_lock<std::mutex> lock(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) return PM_EINVAL;
+    Request* r = it->second;
+    s->cv_done.wait(lock, [r] { return r->done; });
+    if (latency_seconds) *latency_seconds = std::chrono::duration<double>(r->finished - r->submitted).count();
+    const int st = r->status;
+    s->all.erase(it);
+    delete r;
+    if (st != PM_OK) s->ctx->last_error = "served product failed";
+    return st;
+}
+
+int pm_server_stats(pm_server* s, uint64_t* completed, uint64_t* launches, uint64_t* max_concurrent,
+                    uint64_t* peak_bytes) {
+    if (!s) return PM_EINVAL;
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (completed) *completed = s->completed;
+    if (launches) *launches = s->launches_done;
+    if (max_concurrent) *max_concurrent = s->max_concurrent;
+    if (peak_bytes) *peak_bytes = s->arena.peak();
+    return PM_OK;
+}
+
+void pm_server_destroy(pm_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (Launch& L : s->launches) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+        if (L.finished) (void)hipEventDestroy(L.finished);
+        (void)hipFree(L.d_jobs);
+        (void)hipFree(L.d_items);
+        (void)hipFree(L.d_units);
+        (void)hipFree(L.d_words);
+        (void)hipHostFree(L.h_jobs);
+        (void)hipHostFree(L.h_items);
+        (void)hipHostFree(L.h_units);
+    }
+    (void)hipFree(s->arena_base);
+    for (auto& kv : s->all) delete kv.second;
+    delete s;
+}
+
+}  // extern "C"

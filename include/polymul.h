@@ -91,8 +91,15 @@ int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c
  * d_a/d_b/d_c all non-NULL the caller owns the n_i*28-word device buffers of
  * product i; otherwise the batch allocates them (pm_batch_io returns them). */
 typedef struct pm_batch pm_batch;
+/* flags: PM_BATCH_PHASED (default, 0 also selects it) runs one launch per
+ * phase -- conversion, outer stages, one per tile size class, CRT -- inside
+ * the captured graph; PM_BATCH_PERSISTENT runs the whole work list as one
+ * persistent kernel whose blocks claim items in order and wait on per-product
+ * completion counters. */
+#define PM_BATCH_PERSISTENT 1u
+#define PM_BATCH_PHASED 2u
 int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
-                    const uint32_t* const* d_b, uint32_t* const* d_c, pm_batch** out);
+                    const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out);
 void pm_batch_destroy(pm_batch* batch);
 int pm_batch_io(pm_batch* batch, uint32_t i, uint32_t** d_a, uint32_t** d_b, uint32_t** d_c);
 /* synchronous host <-> device copies of product i's operands / result */
@@ -103,6 +110,34 @@ int pm_batch_download(pm_batch* batch, uint32_t i, uint32_t* c);
 int pm_batch_run(pm_batch* batch, void* hip_stream);
 int pm_batch_wait(pm_batch* batch, void* hip_stream);
 int pm_batch_info(const pm_batch* batch, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes);
+/* Host-only: the work list a batch of these sizes runs (k primes).  On entry
+ * *n_items / *n_units hold the capacities of items (4 words each: type, job,
+ * a0, a1) and units (2 words each: job, prime<<2|sub-block); on return the
+ * sizes needed.  Arrays are filled only when large enough. */
+int pm_plan_worklist(uint32_t k, uint32_t count, const uint32_t* ns, uint32_t* n_items, uint32_t* items,
+                     uint32_t* n_units, uint32_t* units);
+
+/* ---- serving (no counterpart in the reference) ------------------------------
+ * Continuous batching: requests of any N (powers of two, 32 <= N <= max_n)
+ * are submitted at any time; a native scheduler thread admits them in
+ * arrival order while their device memory (3*N*112 + 2*k*N*4 bytes) fits
+ * hbm_budget_bytes, groups what has been admitted into the next persistent
+ * launch (at most max_batch products), and keeps up to max_inflight launches
+ * running concurrently on separate streams (each launch runs the phased
+ * kernels of the batch path on the launch's own stream).  a, b, c are host buffers of
+ * n*28 words that must stay valid until pm_server_wait returns for the id. */
+typedef struct pm_server pm_server;
+int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_batch, uint32_t max_inflight,
+                     pm_server** out);
+/* non-blocking; PM_EBUDGET if the request alone exceeds the budget */
+int pm_server_submit(pm_server* server, uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                     uint64_t* id);
+/* blocks until request id is done; latency = completion - submission */
+int pm_server_wait(pm_server* server, uint64_t id, double* latency_seconds);
+int pm_server_stats(pm_server* server, uint64_t* completed, uint64_t* launches, uint64_t* max_concurrent,
+                    uint64_t* peak_bytes);
+/* finishes queued work, then releases the server */
+void pm_server_destroy(pm_server* server);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,64 @@
+"""GPU tests of the continuous-batching server: every result is the same as
+the product computed alone (and as the oracle), whatever it was mixed with;
+admission respects the HBM budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def request_bytes(n, k):
+    al = lambda x: (x + 255) // 256 * 256
+    return al(3 * n * 112) + al(2 * k * n * 4)
+
+
+def test_served_results_equal_alone(pm, orc, workload):
+    q = workload.modulus(3)
+    ctx = pm.Context(q, 1 << 16)
+    rng = np.random.default_rng(5)
+    ns = [1 << int(x) for x in rng.integers(10, 17, size=40)]
+    say("serve mix", ns)
+    reqs = []
+    for i, n in enumerate(ns):
+        a, b = workload.family("dense-random-1", n, q, 1000 + i)
+        reqs.append((n, a, b))
+    server = pm.Server(ctx, 4 << 30, max_batch=8, max_inflight=2)
+    ids = [server.submit(a, b) for _, a, b in reqs]
+    results = [server.wait(r) for r in ids]
+    stats = server.stats()
+    say("stats", stats)
+    server.close()
+    assert stats["completed"] == len(ns)
+    for i, ((n, a, b), (c, lat)) in enumerate(zip(reqs, results)):
+        alone = ctx.multiply(a, b)
+        assert (c == alone).all(), (i, n)
+        assert lat > 0
+        if i < 4 and n <= 32768:
+            assert (c == orc.rns_multiply(q, a, b, orc.reference_primes_for(n, q))).all(), i
+    ctx.close()
+
+
+def test_budget_admission_queues(pm, workload):
+    q = workload.modulus(4)
+    ctx = pm.Context(q, 1 << 16)
+    k = ctx.info()["k"]
+    budget = request_bytes(1 << 16, k) + request_bytes(1 << 12, k)
+    server = pm.Server(ctx, budget, max_batch=4, max_inflight=2)
+    reqs = []
+    for i, n in enumerate([1 << 16, 1 << 16, 1 << 12, 1 << 16, 1 << 12]):
+        a, b = workload.family("dense-random-1", n, q, 50 + i)
+        reqs.append((a, b, server.submit(a, b)))
+    for a, b, rid in reqs:
+        c, _ = server.wait(rid)
+        assert (c == ctx.multiply(a, b)).all()
+    stats = server.stats()
+    assert stats["peak_bytes"] <= budget
+    assert stats["launches"] >= 3  # two N=2^16 products never fit together
+    server.close()
+    with pytest.raises(pm.PolymulError):
+        pm.Server(ctx, request_bytes(1 << 12, k), max_batch=4, max_inflight=1)
+    ctx.close()
