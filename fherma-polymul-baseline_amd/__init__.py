@@ -64,6 +64,7 @@ def library():
     lib.pm_multiply_device.argtypes = [P, u32, P, P, P, P]
     lib.pm_synchronize.argtypes = [P]
     lib.pm_stage_convert.argtypes = [P, u32, P, P, P, P, P]
+    lib.pm_stage_convert_scalar.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
     lib.pm_stage_crt.argtypes = [P, u32, P, P, P]
     lib.pm_batch_create.argtypes = [P, u32, P, P, P, P, u32, ctypes.POINTER(P)]
@@ -74,6 +75,7 @@ def library():
     lib.pm_batch_download.argtypes = [P, u32, P]
     lib.pm_batch_run.argtypes = [P, P]
     lib.pm_batch_wait.argtypes = [P, P]
+    lib.pm_batch_phase_times.argtypes = [P, P, u32, P, P, ctypes.POINTER(u32)]
     lib.pm_plan_worklist.argtypes = [u32, u32, P, ctypes.POINTER(u32), P, ctypes.POINTER(u32), P]
     lib.pm_batch_info.argtypes = [P, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_uint64)]
     lib.pm_server_create.argtypes = [P, ctypes.c_uint64, u32, u32, ctypes.POINTER(P)]
@@ -90,9 +92,9 @@ def library():
 
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
-            "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_ntt", "pm_stage_crt",
+            "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
-            "pm_batch_run", "pm_batch_wait", "pm_batch_info", "pm_plan_worklist",
+            "pm_batch_run", "pm_batch_wait", "pm_batch_info", "pm_plan_worklist", "pm_batch_phase_times",
             "pm_server_create", "pm_server_submit", "pm_server_wait", "pm_server_stats", "pm_server_destroy"]
 
 
@@ -211,6 +213,10 @@ class Context:
         self._check("pm_stage_convert",
                     self._lib.pm_stage_convert(self._h, n, _ptr(a), _ptr(b), _ptr(ra), _ptr(rb), stream))
 
+    def stage_convert_scalar(self, n, a, b, ra, rb, stream=None):
+        self._check("pm_stage_convert_scalar",
+                    self._lib.pm_stage_convert_scalar(self._h, n, _ptr(a), _ptr(b), _ptr(ra), _ptr(rb), stream))
+
     def stage_ntt(self, n, ra, rb, outer=0, stream=None):
         self._check("pm_stage_ntt", self._lib.pm_stage_ntt(self._h, n, outer, _ptr(ra), _ptr(rb), stream))
 
@@ -221,7 +227,7 @@ class Context:
 class Batch:
     """Products of uneven N in one persistent, graph-captured launch."""
 
-    PERSISTENT, PHASED = 1, 2
+    PERSISTENT, PHASED, SCALAR_CONVERSION = 1, 2, 4
 
     def __init__(self, ctx, ns, d_a=None, d_b=None, d_c=None, mode="phased"):
         self._ctx, self._lib = ctx, ctx._lib
@@ -232,7 +238,8 @@ class Batch:
         if d_a is not None:
             ptrs = [(ctypes.c_void_p * count)(*[_ptr(x) for x in v]) for v in (d_a, d_b, d_c)]
         handle = ctypes.c_void_p()
-        flags = {"phased": self.PHASED, "persistent": self.PERSISTENT}[mode]
+        flags = {"phased": self.PHASED, "persistent": self.PERSISTENT,
+                 "phased-scalar": self.PHASED | self.SCALAR_CONVERSION}[mode]
         ctx._check("pm_batch_create", self._lib.pm_batch_create(ctx.handle, count, arr, ptrs[0], ptrs[1], ptrs[2],
                                                                  flags, ctypes.byref(handle)))
         self.mode = mode
@@ -265,6 +272,22 @@ class Batch:
 
     def wait(self, stream=None):
         self._ctx._check("pm_batch_wait", self._lib.pm_batch_wait(self._h, stream))
+
+    PHASE_NAMES = {0: "conversion", 1: "outer_forward", 3: "outer_inverse", 4: "crt", 5: "conversion"}
+
+    def phase_times(self, stream=None):
+        """One direct (ungraphed) run with events between phases: [(name, ms)]."""
+        kinds = np.zeros(32, dtype=np.uint32)
+        ms = np.zeros(32, dtype=np.float64)
+        count = ctypes.c_uint32()
+        self._ctx._check("pm_batch_phase_times", self._lib.pm_batch_phase_times(
+            self._h, stream, 32, kinds.ctypes.data, ms.ctypes.data, ctypes.byref(count)))
+        out = []
+        for kd, t in zip(kinds[:count.value], ms[:count.value]):
+            kd = int(kd)
+            name = f"tile_2^{kd >> 8}" if (kd & 0xFF) == 2 else self.PHASE_NAMES[kd & 0xFF]
+            out.append((name, float(t)))
+        return out
 
     def info(self):
         items, blocks, nbytes = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()

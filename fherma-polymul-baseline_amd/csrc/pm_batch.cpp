@@ -124,7 +124,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     if (e == hipSuccess) {
         hipError_t le = b->persistent
                             ? pmd::launch_persistent(ctx->tables, ba, b->d_words, b->reset_count, b->blocks, cap)
-                            : pmd::launch_phased(ctx->tables, ba, b->ranges, cap);
+                            : pmd::launch_phased(ctx->tables, ba, b->ranges, b->scalar_conversion, cap);
         hipError_t ce = hipStreamEndCapture(cap, &b->graph);
         e = le != hipSuccess ? le : ce;
     }
@@ -211,7 +211,8 @@ extern "C" {
 int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
                     const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out) {
     if (!ctx || !ns || !out || count == 0) return PM_EINVAL;
-    if (flags & ~uint32_t(PM_BATCH_PERSISTENT | PM_BATCH_PHASED)) return PM_EINVAL;
+    if (flags & ~uint32_t(PM_BATCH_PERSISTENT | PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION)) return PM_EINVAL;
+    if ((flags & PM_BATCH_PERSISTENT) && (flags & (PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION))) return PM_EINVAL;
     *out = nullptr;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return pm_hip(ctx, e, "set device");
@@ -220,6 +221,7 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
     b->ctx = ctx;
     b->count = count;
     b->persistent = (flags & PM_BATCH_PERSISTENT) != 0;
+    b->scalar_conversion = (flags & PM_BATCH_SCALAR_CONVERSION) != 0;
     const int st = pm_batch_build(ctx, b, ns, d_a, d_b, d_c);
     if (st != PM_OK) {
         delete b;
@@ -296,6 +298,35 @@ int pm_plan_worklist(uint32_t k, uint32_t count, const uint32_t* ns, uint32_t* n
     if (items && cap_items >= it.size()) std::memcpy(items, it.data(), it.size() * sizeof(pmd::Item));
     if (units && cap_units >= un.size()) std::memcpy(units, un.data(), un.size() * sizeof(pmd::Unit));
     return PM_OK;
+}
+
+int pm_batch_phase_times(pm_batch* b, void* hip_stream, uint32_t max, uint32_t* kinds, double* ms,
+                         uint32_t* count) {
+    if (!b || !count) return PM_EINVAL;
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->ctx->stream;
+    const std::vector<pmd::Phase> phases = pmd::phase_list(b->ranges, b->scalar_conversion);
+    *count = uint32_t(phases.size());
+    if (phases.size() > max || !kinds || !ms) return PM_OK;
+    std::vector<hipEvent_t> ev(phases.size() + 1, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto& x : ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    for (size_t i = 0; i < phases.size() && e == hipSuccess; ++i) {
+        e = hipEventRecord(ev[i], stream);
+        if (e == hipSuccess)
+            e = pmd::launch_phase(b->ctx->tables, b->args, phases[i].kind, phases[i].first, phases[i].count, stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev[phases.size()], stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[phases.size()]);
+    for (size_t i = 0; i < phases.size() && e == hipSuccess; ++i) {
+        float t = 0;
+        e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+        kinds[i] = phases[i].kind;
+        ms[i] = t;
+    }
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    return pm_hip(b->ctx, e, "time batch phases");
 }
 
 int pm_batch_info(const pm_batch* b, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes) {

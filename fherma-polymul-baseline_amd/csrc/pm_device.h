@@ -29,6 +29,7 @@ struct Tables {
     const uint4* conv_w;          // [quads][2][64] int8x16 weight operands
     const int32_t* conv_corr;     // [quads*4][4]
     const uint32_t* conv_sh16;    // [k] floor(2^48/p)
+    const Tw* limb_pow;           // [k][28] 2^(32j) mod p (scalar conversion)
     const uint32_t* crt_m30;      // [k][29]
     const uint32_t* crt_round;    // [k] floor(2^42/p)
     const uint32_t* crt_negkp;    // [k+1][28]

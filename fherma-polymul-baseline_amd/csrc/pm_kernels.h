@@ -3,12 +3,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "pm_device.h"
 
 namespace pmd {
 
 // residues of a and b for all primes (MFMA conversion)
 hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream);
+// residues by scalar limb reduction (comparison path, same output contract)
+hipError_t launch_convert_scalar(const Tables& tb, const Job& job, hipStream_t stream);
 // forward NTTs, pointwise product, inverse NTT, scale; u outer radix-2 stages
 // run as separate passes (tile size 2^(logn-u) <= 2^15)
 hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream);
@@ -25,8 +29,19 @@ struct PhaseRanges {
     uint32_t inv_first = 0, inv_count = 0;
     uint32_t crt_first = 0, crt_count = 0;
 };
+// phase kinds (tile phases carry the tile size class: kPhaseTile | tlog << 8)
+enum PhaseKind : uint32_t {
+    kPhaseConv = 0, kPhaseOuterFwd = 1, kPhaseTile = 2, kPhaseOuterInv = 3, kPhaseCrt = 4, kPhaseConvScalar = 5
+};
+struct Phase {
+    uint32_t kind, first, count;
+};
+std::vector<Phase> phase_list(const PhaseRanges& ranges, bool scalar_conversion);
+hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, uint32_t first, uint32_t count,
+                        hipStream_t stream);
 // one launch per phase (and per tile size class) over the same work list
-hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& ranges, hipStream_t stream);
+hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& ranges, bool scalar_conversion,
+                         hipStream_t stream);
 // raise the dynamic LDS limit of the tile kernels (call once per device)
 hipError_t configure_kernels();
 

@@ -313,6 +313,31 @@ __device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* _
     }
 }
 
+// Scalar limb reduction, the comparison path of the MFMA conversion (role of
+// rns/solve.cu:138-157): one thread per coefficient, every prime,
+// r = sum_j w_j * (2^(32j) mod p) with Shoup products, kept below 2p.
+__device__ __forceinline__ void convert_scalar_one(const Tables& tb, const uint32_t* __restrict__ coef,
+                                                   uint32_t n, uint32_t c, uint32_t* __restrict__ res) {
+    uint32_t w[kLimbs];
+    const uint4* src = reinterpret_cast<const uint4*>(coef + size_t(c) * kLimbs);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+        const uint4 x = src[v];
+        w[4 * v] = x.x;
+        w[4 * v + 1] = x.y;
+        w[4 * v + 2] = x.z;
+        w[4 * v + 3] = x.w;
+    }
+    for (uint32_t i = 0; i < tb.k; ++i) {
+        const uint32_t p = tb.primes[i], p2 = 2u * p;
+        const Tw* pw = tb.limb_pow + size_t(i) * kLimbs;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int j = 0; j < int(kLimbs); ++j) acc = csub(acc + shoup(w[j], pw[j], p), p2);
+        res[size_t(i) * n + c] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // CRT: one output coefficient per thread
 // ---------------------------------------------------------------------------
@@ -324,9 +349,14 @@ __device__ __forceinline__ void crt_one(const Tables& tb, const uint32_t* __rest
     uint64_t est = 0;
     const uint32_t k = tb.k;
     for (uint32_t i0 = 0; i0 < k; i0 += 15) {
-        const uint32_t i1 = i0 + 15 < k ? i0 + 15 : k;
-        for (uint32_t i = i0; i < i1; ++i) {
-            const uint32_t t = t_res[size_t(i) * n + c];  // canonical, < p < 2^30
+        // issue the group's 15 residue loads together, then accumulate
+        uint32_t tv[15];
+#pragma unroll
+        for (int j = 0; j < 15; ++j) tv[j] = i0 + j < k ? t_res[size_t(i0 + j) * n + c] : 0u;  // < p < 2^30
+#pragma unroll
+        for (int j = 0; j < 15; ++j) {
+            if (i0 + j >= k) break;
+            const uint32_t i = i0 + j, t = tv[j];
             est += uint64_t(t) * tb.crt_round[i];
             const uint32_t* m = tb.crt_m30 + size_t(i) * kCrtLimbs;
 #pragma unroll
@@ -422,6 +452,14 @@ __global__ void __launch_bounds__(kBlock) k_convert(Tables tb, Job job) {
     const uint32_t c0 = blockIdx.x * kConvPerBlock + wave * kConvPerWave;
     if (c0 >= n) return;
     convert_wave(tb, poly ? job.b : job.a, n, c0, poly ? job.rb : job.ra, lds_w, lane);
+}
+
+__global__ void __launch_bounds__(256) k_convert_scalar(Tables tb, Job job) {
+    const uint32_t n = 1u << job.logn;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const bool poly = blockIdx.y != 0;
+    convert_scalar_one(tb, poly ? job.b : job.a, n, c, poly ? job.rb : job.ra);
 }
 
 // units = k << u sub-blocks; unit -> (prime = unit >> u, s = unit & (2^u-1))
@@ -752,6 +790,15 @@ __global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, 
     convert_wave(tb, it.a0 ? job.b : job.a, n, c0, it.a0 ? job.rb : job.ra, lds_w, threadIdx.x & 63u);
 }
 
+__global__ void __launch_bounds__(kBlock) k_items_conv_scalar(Tables tb, BatchArgs ba, uint32_t first) {
+    const Item it = ba.items[first + blockIdx.x];
+    const Job job = ba.jobs[it.job];
+    const uint32_t n = 1u << job.logn;
+    const uint32_t c = it.a1 * kConvPerBlock + threadIdx.x;
+    if (c >= n) return;
+    convert_scalar_one(tb, it.a0 ? job.b : job.a, n, c, it.a0 ? job.rb : job.ra);
+}
+
 __global__ void __launch_bounds__(kBlock) k_items_outer(Tables tb, BatchArgs ba, uint32_t first) {
     const Item it = ba.items[first + blockIdx.x];
     const Job job = ba.jobs[it.job];
@@ -874,28 +921,68 @@ static void launch_items_tile(const Tables& tb, const BatchArgs& ba, uint32_t fi
                        tb, ba, first);
 }
 
-hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& r, hipStream_t stream) {
-    if (r.conv_count) hipLaunchKernelGGL(k_items_conv, dim3(r.conv_count), dim3(kBlock), 0, stream, tb, ba, r.conv_first);
-    if (r.fwd_count) hipLaunchKernelGGL(k_items_outer, dim3(r.fwd_count), dim3(kBlock), 0, stream, tb, ba, r.fwd_first);
-    for (uint32_t tlog = 15; tlog >= 5; --tlog) {
-        const uint32_t first = r.tile_first[tlog], count = r.tile_count[tlog];
-        if (!count) continue;
-        switch (tlog) {
-            case 5: launch_items_tile<5>(tb, ba, first, count, stream); break;
-            case 6: launch_items_tile<6>(tb, ba, first, count, stream); break;
-            case 7: launch_items_tile<7>(tb, ba, first, count, stream); break;
-            case 8: launch_items_tile<8>(tb, ba, first, count, stream); break;
-            case 9: launch_items_tile<9>(tb, ba, first, count, stream); break;
-            case 10: launch_items_tile<10>(tb, ba, first, count, stream); break;
-            case 11: launch_items_tile<11>(tb, ba, first, count, stream); break;
-            case 12: launch_items_tile<12>(tb, ba, first, count, stream); break;
-            case 13: launch_items_tile<13>(tb, ba, first, count, stream); break;
-            case 14: launch_items_tile<14>(tb, ba, first, count, stream); break;
-            case 15: launch_items_tile<15>(tb, ba, first, count, stream); break;
-        }
+hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, uint32_t first, uint32_t count,
+                        hipStream_t stream) {
+    if (!count) return hipSuccess;
+    switch (kind & 0xFFu) {
+        case kPhaseConv:
+            hipLaunchKernelGGL(k_items_conv, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
+            break;
+        case kPhaseConvScalar:
+            hipLaunchKernelGGL(k_items_conv_scalar, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
+            break;
+        case kPhaseOuterFwd:
+        case kPhaseOuterInv:
+            hipLaunchKernelGGL(k_items_outer, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
+            break;
+        case kPhaseCrt:
+            hipLaunchKernelGGL(k_items_crt, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
+            break;
+        case kPhaseTile:
+            switch (kind >> 8) {
+                case 5: launch_items_tile<5>(tb, ba, first, count, stream); break;
+                case 6: launch_items_tile<6>(tb, ba, first, count, stream); break;
+                case 7: launch_items_tile<7>(tb, ba, first, count, stream); break;
+                case 8: launch_items_tile<8>(tb, ba, first, count, stream); break;
+                case 9: launch_items_tile<9>(tb, ba, first, count, stream); break;
+                case 10: launch_items_tile<10>(tb, ba, first, count, stream); break;
+                case 11: launch_items_tile<11>(tb, ba, first, count, stream); break;
+                case 12: launch_items_tile<12>(tb, ba, first, count, stream); break;
+                case 13: launch_items_tile<13>(tb, ba, first, count, stream); break;
+                case 14: launch_items_tile<14>(tb, ba, first, count, stream); break;
+                case 15: launch_items_tile<15>(tb, ba, first, count, stream); break;
+                default: return hipErrorInvalidValue;
+            }
+            break;
+        default:
+            return hipErrorInvalidValue;
     }
-    if (r.inv_count) hipLaunchKernelGGL(k_items_outer, dim3(r.inv_count), dim3(kBlock), 0, stream, tb, ba, r.inv_first);
-    if (r.crt_count) hipLaunchKernelGGL(k_items_crt, dim3(r.crt_count), dim3(kBlock), 0, stream, tb, ba, r.crt_first);
+    return hipGetLastError();
+}
+
+std::vector<Phase> phase_list(const PhaseRanges& r, bool scalar_conversion) {
+    std::vector<Phase> out;
+    if (r.conv_count) out.push_back({scalar_conversion ? kPhaseConvScalar : kPhaseConv, r.conv_first, r.conv_count});
+    if (r.fwd_count) out.push_back({kPhaseOuterFwd, r.fwd_first, r.fwd_count});
+    for (uint32_t tlog = 15; tlog >= 5; --tlog)
+        if (r.tile_count[tlog]) out.push_back({kPhaseTile | (tlog << 8), r.tile_first[tlog], r.tile_count[tlog]});
+    if (r.inv_count) out.push_back({kPhaseOuterInv, r.inv_first, r.inv_count});
+    if (r.crt_count) out.push_back({kPhaseCrt, r.crt_first, r.crt_count});
+    return out;
+}
+
+hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& r, bool scalar_conversion,
+                         hipStream_t stream) {
+    for (const Phase& ph : phase_list(r, scalar_conversion)) {
+        const hipError_t e = launch_phase(tb, ba, ph.kind, ph.first, ph.count, stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_convert_scalar(const Tables& tb, const Job& job, hipStream_t stream) {
+    const uint32_t n = 1u << job.logn;
+    hipLaunchKernelGGL(k_convert_scalar, dim3((n + 255) / 256, 2), dim3(256), 0, stream, tb, job);
     return hipGetLastError();
 }
 

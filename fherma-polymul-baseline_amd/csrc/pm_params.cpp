@@ -302,6 +302,15 @@ std::string build_params(uint32_t max_logn, const uint32_t* q_words, uint32_t li
         }
         s.conv_shoup16[i] = uint32_t((uint64_t(1) << 48) / p);
     }
+    s.limb_pow.resize(size_t(k) * kLimbs);
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint32_t p = s.primes[i], base = uint32_t((uint64_t(1) << 32) % p);
+        uint32_t v = 1;
+        for (uint32_t j = 0; j < kLimbs; ++j) {
+            s.limb_pow[size_t(i) * kLimbs + j] = make_twiddle(v, p);
+            v = mulmod(v, base, p);
+        }
+    }
     return std::string();
 }
 

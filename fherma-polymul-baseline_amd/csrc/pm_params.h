@@ -46,11 +46,12 @@ struct Params {
     std::vector<int8_t> conv_digits;             // k * 112 * 4  [prime][byte j][digit b]
     std::vector<int32_t> conv_correction;        // k * 4: 128 * sum_j digit
     std::vector<uint32_t> conv_shoup16;          // k: floor(2^16 * 2^32 / p)
+    std::vector<Twiddle> limb_pow;               // k * 28: 2^(32j) mod p (scalar conversion)
     // CRT
     std::vector<uint32_t> crt_bases30;           // k * 29: M_i mod q in 30-bit limbs
     std::vector<uint32_t> crt_round;             // k: floor(2^42 / p_i)
     std::vector<uint32_t> crt_neg_kp;            // (k+1) * 28: (-K * P) mod q
-    std::vector<uint32_t> barrett_mu;            // 2 words: floor(2^906 / q)
+    std::vector<uint32_t> barrett_mu;            // 2 words: floor(2^908 / q)
     uint32_t product_bits = 0;
 };
 

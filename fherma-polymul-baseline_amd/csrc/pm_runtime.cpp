@@ -112,7 +112,9 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
                 }
     std::vector<int32_t> conv_corr(size_t(quads) * 4 * 4, 0);
     std::copy(s.conv_correction.begin(), s.conv_correction.end(), conv_corr.begin());
-    std::vector<pmd::Tw> fwd(s.forward.size()), inv(s.inverse.size()), scale(s.scale.size());
+    std::vector<pmd::Tw> fwd(s.forward.size()), inv(s.inverse.size()), scale(s.scale.size()),
+        limb_pow(s.limb_pow.size());
+    for (size_t i = 0; i < limb_pow.size(); ++i) limb_pow[i] = {s.limb_pow[i].value, s.limb_pow[i].shoup};
     for (size_t i = 0; i < fwd.size(); ++i) fwd[i] = {s.forward[i].value, s.forward[i].shoup};
     for (size_t i = 0; i < inv.size(); ++i) inv[i] = {s.inverse[i].value, s.inverse[i].shoup};
     for (size_t i = 0; i < scale.size(); ++i) scale[i] = {s.scale[i].value, s.scale[i].shoup};
@@ -126,6 +128,7 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     tb.conv_w = reinterpret_cast<const uint4*>(upload(ctx, conv_w, &status));
     tb.conv_corr = upload(ctx, conv_corr, &status);
     tb.conv_sh16 = upload(ctx, s.conv_shoup16, &status);
+    tb.limb_pow = upload(ctx, limb_pow, &status);
     tb.crt_m30 = upload(ctx, s.crt_bases30, &status);
     tb.crt_round = upload(ctx, s.crt_round, &status);
     tb.crt_negkp = upload(ctx, s.crt_neg_kp, &status);
@@ -276,6 +279,17 @@ int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uin
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, d_a, d_b, nullptr, d_ra, d_rb};
     return pm_hip(ctx, pmd::launch_convert(ctx->tables, job, stream), "launch conversion");
+}
+
+int pm_stage_convert_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
+                            uint32_t* d_rb, void* hip_stream) {
+    if (!ctx || !d_a || !d_b || !d_ra || !d_rb) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
+    pmd::Job job{logn, 0, d_a, d_b, nullptr, d_ra, d_rb};
+    return pm_hip(ctx, pmd::launch_convert_scalar(ctx->tables, job, stream), "launch scalar conversion");
 }
 
 int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, uint32_t* d_rb, void* hip_stream) {

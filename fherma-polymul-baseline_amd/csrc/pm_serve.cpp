@@ -200,7 +200,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
         error = "work list phases are not contiguous";
         return false;
     }
-    if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, L.stream);
+    if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, false, L.stream);
     if (e == hipSuccess) e = hipEventRecord(L.finished, L.stream);
     if (e != hipSuccess) {
         error = std::string("launch: ") + hipGetErrorString(e);

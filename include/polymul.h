@@ -78,6 +78,11 @@ int pm_synchronize(pm_context* ctx);
 /* residues of a and b: ra[i*n+j] = a_j mod p_i (value below 2*p_i) -- rns/solve.cu:138-157 */
 int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
                      uint32_t* d_rb, void* hip_stream);
+/* same contract by scalar limb reduction (sum of 28 Shoup products with
+ * 2^(32j) mod p_i, rns/solve.cu:138-157) -- the comparison path of the MFMA
+ * conversion */
+int pm_stage_convert_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
+                            uint32_t* d_rb, void* hip_stream);
 /* forward NTTs, pointwise, inverse, scale: ra <- N^-1 (M_i mod p_i)^-1 * (a*b mod p_i) --
  * rns/solve.cu:225-333; outer = number of radix-2 stages run as separate passes */
 int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, uint32_t* d_rb, void* hip_stream);
@@ -98,6 +103,7 @@ typedef struct pm_batch pm_batch;
  * completion counters. */
 #define PM_BATCH_PERSISTENT 1u
 #define PM_BATCH_PHASED 2u
+#define PM_BATCH_SCALAR_CONVERSION 4u /* phased: scalar limb reduction instead of int8 MFMA */
 int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
                     const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out);
 void pm_batch_destroy(pm_batch* batch);
@@ -110,6 +116,12 @@ int pm_batch_download(pm_batch* batch, uint32_t i, uint32_t* c);
 int pm_batch_run(pm_batch* batch, void* hip_stream);
 int pm_batch_wait(pm_batch* batch, void* hip_stream);
 int pm_batch_info(const pm_batch* batch, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes);
+/* Runs the batch's phases once as direct launches on a stream with HIP events
+ * between them and returns each phase's kind (0 conversion, 1 outer forward,
+ * 2 | tlog<<8 tile class, 3 outer inverse, 4 CRT) and milliseconds.  On entry
+ * max is the capacity of kinds/ms; *count returns the number of phases. */
+int pm_batch_phase_times(pm_batch* batch, void* hip_stream, uint32_t max, uint32_t* kinds, double* ms,
+                         uint32_t* count);
 /* Host-only: the work list a batch of these sizes runs (k primes).  On entry
  * *n_items / *n_units hold the capacities of items (4 words each: type, job,
  * a0, a1) and units (2 words each: job, prime<<2|sub-block); on return the

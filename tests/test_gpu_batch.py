@@ -23,7 +23,7 @@ def say(*a):
 
 @pytest.mark.skip(reason="a modified persistent kernel faulted the GPU on this uneven mix and the one run "
                          "testing a fix faulted again; the kernel was reverted to the last version this test "
-                         "passed with, and the case stays off the GPU for the rest of the round")
+                         "passed with, and the case stays off the GPU until a cause is shown")
 def test_batch_uneven_sizes(pm, orc, workload):
     q = workload.modulus(1)
     ctx = pm.Context(q, 1 << 17)
@@ -78,7 +78,7 @@ def test_batch_uneven_sizes_phased(pm, orc, workload):
     ctx.close()
 
 
-@pytest.mark.parametrize("mode", ["persistent", "phased"])
+@pytest.mark.parametrize("mode", ["persistent", "phased", "phased-scalar"])
 def test_batch_golden_and_identical_products(pm, workload, mode):
     """Each result is the same alone (golden) and inside a batch of copies."""
     case = [c for c in GOLDEN["cases"] if c["n"] == 32768 and c["family"] == "limb-boundaries"

@@ -93,6 +93,28 @@ def test_convert_stage(pm, orc, workload, torch_cuda, n):
     assert ((gb % primes[:, None]) == orc.residues(b, primes)).all()
 
 
+@pytest.mark.parametrize("n", [32, 4096])
+def test_convert_stage_scalar_matches_mfma(pm, orc, workload, torch_cuda, n):
+    """Scalar limb reduction and int8 MFMA conversion agree mod p (and with the oracle)."""
+    say("convert scalar", n)
+    q = workload.modulus(11)
+    a, b = workload.family("limb-boundaries", n, q, 2)
+    ctx = context(pm, q, 1 << 15)
+    primes = np.array([p for p, _ in ctx.primes()], dtype=np.uint32)
+    k = len(primes)
+    da, db = to_dev(torch_cuda, a), to_dev(torch_cuda, b)
+    bufs = [torch_cuda.zeros(k * n, dtype=torch_cuda.int32, device="cuda") for _ in range(4)]
+    ctx.stage_convert(n, da, db, bufs[0], bufs[1])
+    ctx.stage_convert_scalar(n, da, db, bufs[2], bufs[3])
+    ctx.synchronize()
+    g = [to_host(x).reshape(k, n) for x in bufs]
+    assert all((x < 2 * primes[:, None]).all() for x in g)
+    assert ((g[2] % primes[:, None]) == orc.residues(a, primes)).all()
+    assert ((g[3] % primes[:, None]) == orc.residues(b, primes)).all()
+    assert ((g[0] % primes[:, None]) == (g[2] % primes[:, None])).all()
+    assert ((g[1] % primes[:, None]) == (g[3] % primes[:, None])).all()
+
+
 @pytest.mark.parametrize("n,outer", [(32, 0), (1024, 0), (8192, 0), (32768, 0), (32768, 1), (32768, 2)])
 def test_ntt_stage(pm, orc, workload, torch_cuda, n, outer):
     say("ntt", n, outer)
