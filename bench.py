@@ -203,6 +203,17 @@ def run_headline(args):
         },
         "bit_exact": bool(exact and gpu_consistent and done > 0),
     }
+    if args.copy_mib:
+        # a plain streaming kernel of known size (int32 negation: reads and
+        # writes copy_mib MiB) for scaling the HBM byte counters of a profile
+        words = args.copy_mib << 18
+        x = torch.ones(words, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        line["config"]["copy_calibration"] = {"kernel": "torch int32 neg", "bytes_read": words * 4,
+                                              "bytes_written": words * 4, "launches": 3}
     print(json.dumps(line), flush=True)
     batch.close()
     ctx.close()
@@ -346,16 +357,28 @@ def run_large(args):
 def run_serve(args):
     """Poisson arrivals, N = 2^U with U uniform on {10..16}, 64 GB budget:
     continuous batching vs one-at-a-time launches on the same request stream."""
+    import torch  # before the library touches HIP, so torch sees the device
     pm = package()
     rng = np.random.default_rng(args.seed)
     count = args.count or 400
     ns = [1 << int(u) for u in rng.integers(10, 17, size=count)]
     q = pm.workload.modulus(args.seed)
     ctx = pm.Context(q, 1 << 16)
+    torch.zeros(1, device="cuda")  # initialise torch's HIP state before pinning host memory
     inputs = make_inputs(pm, q, ns, args.seed, pinned=True)
-    print(f"[bench] serve inputs ready: {count} requests", file=sys.stderr, flush=True)
+    outs = [torch.empty(a.size, dtype=torch.int32).pin_memory().numpy().view(np.uint32).reshape(a.shape)
+            for a, _ in inputs]
+    print(f"[bench] serve inputs ready: {count} requests (page-locked in/out)", file=sys.stderr, flush=True)
     out = {"workload": f"serving: Poisson arrivals, N log-uniform 2^10..2^16, {count} requests, 64 GB HBM budget",
-           "rates": {}}
+           "io": "page-locked host inputs and outputs (PCIe-inclusive latency)", "rates": {}}
+    # warm-up: one request per size so module loading is not in the timed runs
+    warm = pm.Server(ctx, 64 << 30, max_batch=64, max_inflight=3)
+    seen = {}
+    for (a, b), n in zip(inputs, ns):
+        seen.setdefault(n, (a, b))
+    for rid in [warm.submit(a, b) for a, b in seen.values()]:
+        warm.wait(rid)
+    warm.close()
     reference = None
     for rate in args.rates:
         gaps = rng.exponential(1.0 / rate, size=count)
@@ -368,7 +391,7 @@ def run_serve(args):
                 t_next += gaps[i]
                 while time.perf_counter() < t_next:
                     pass
-                ids.append(server.submit(a, b))
+                ids.append(server.submit(a, b, out=outs[i]))
             results = [server.wait(r) for r in ids]
             wall = time.perf_counter() - t_start
             lats = [lat * 1e3 for _, lat in results]
@@ -403,6 +426,8 @@ def main():
     ap.add_argument("--workload", default="headline", choices=["headline", "uneven", "large", "serve"])
     ap.add_argument("--count", type=int, default=0, help="products / requests for the extra workloads")
     ap.add_argument("--rates", type=float, nargs="*", default=[2000.0, 8000.0])
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a known-size streaming kernel (for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "

@@ -49,6 +49,14 @@ def library():
     path = library_path()
     if not os.path.exists(path):
         raise OSError(f"{path} is missing: build it with __graft_entry__.build() or make -C {_HERE}")
+    # PyTorch-ROCm ships its own libamdhip64 and asks for it by a name that
+    # the system runtime does not answer to; loaded in that order the process
+    # ends up with two HIP runtimes and torch then finds no GPU.  Importing
+    # torch first lets this library bind to torch's runtime (same soname).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     P = ctypes.c_void_p
     u32 = ctypes.c_uint32
@@ -307,11 +315,19 @@ class Server:
         self._h = handle
         self._keep = {}
 
-    def submit(self, a, b):
-        """Queue a*b; returns a request id.  The arrays stay referenced until wait()."""
+    def submit(self, a, b, out=None):
+        """Queue a*b; returns a request id.  The arrays stay referenced until
+        wait().  `out` (optional, same shape as a, uint32, C-contiguous) receives
+        the product; page-locked memory there lets the result come back
+        asynchronously behind the kernels."""
         a = np.ascontiguousarray(a, dtype=np.uint32)
         b = np.ascontiguousarray(b, dtype=np.uint32)
-        c = np.empty_like(a)
+        if out is None:
+            c = np.empty_like(a)
+        else:
+            c = out
+            if c.dtype != np.uint32 or c.shape != a.shape or not c.flags.c_contiguous or not c.flags.writeable:
+                raise ValueError("out must be a writeable C-contiguous uint32 array shaped like a")
         rid = ctypes.c_uint64()
         self._ctx._check("pm_server_submit", self._lib.pm_server_submit(self._h, a.shape[0], a.ctypes.data,
                                                                          b.ctypes.data, c.ctypes.data,

@@ -185,12 +185,15 @@ void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool la
             for (uint32_t x = 0; x < 2 * k; ++x) items.push_back({pmd::kItemOuterFwd, j, x, 0});
     for (uint32_t tlog = 15; tlog >= 5; --tlog) {
         const uint32_t per_item = 1u << (15 - tlog);  // 1024 threads x 32 residues / tile size
+        // prime-major: blocks that run at the same time (and the blocks one
+        // XCD receives round-robin) share a prime, so its twiddle tables stay
+        // in that XCD's L2 instead of every block streaming its own
         std::vector<pmd::Unit> cls;
-        for (uint32_t j : order) {
-            if (jobs[j].logn - jobs[j].outer != tlog) continue;
-            for (uint32_t prime = 0; prime < k; ++prime)
+        for (uint32_t prime = 0; prime < k; ++prime)
+            for (uint32_t j : order) {
+                if (jobs[j].logn - jobs[j].outer != tlog) continue;
                 for (uint32_t s = 0; s < (1u << jobs[j].outer); ++s) cls.push_back({j, (prime << 2) | s});
-        }
+            }
         for (size_t first = 0; first < cls.size(); first += per_item) {
             const uint32_t n_units = uint32_t(std::min<size_t>(per_item, cls.size() - first));
             items.push_back({pmd::kItemTile, 0, tlog | (n_units << 8), uint32_t(units.size())});

@@ -126,19 +126,45 @@ __device__ __forceinline__ uint32_t padded(uint32_t x) { return x + (x >> 5); }
 #endif
 constexpr int kTwiddleChunk = PM_TWIDDLE_CHUNK;
 
-template <int TLOG, int PFROM, int PTO>
+// An exchange keeps every element inside its wave when the wave bits of the
+// thread index (thread bits 6 and up) sit on the same index bits in both
+// phases; then no block barrier is needed: LDS instructions of one wave
+// execute in order, so its reads see its own writes.
+template <int TLOG>
+__host__ __device__ constexpr bool wave_local(int pfrom, int pto) {
+    using G = Geo<TLOG>;
+    for (int j = 6; j < G::TPTLOG; ++j)
+        if (G::thr_pos(pfrom, j) != G::thr_pos(pto, j)) return false;
+    return true;
+}
+
+// LEAD: a wave-local exchange came before this block-wide one, so other waves
+// may still be reading their part of the buffer.
+template <int TLOG, int PFROM, int PTO, bool LEAD>
 __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* lds, uint32_t tau) {
     using G = Geo<TLOG>;
+    constexpr bool kWave = wave_local<TLOG>(PFROM, PTO);
     // thread and register bits are disjoint, so padded(x|c) = padded(x)+padded(c):
     // one base address per thread plus immediate offsets
     uint32_t* wbase = lds + padded(thr_scatter<TLOG, PFROM>(tau));
     uint32_t* rbase = lds + padded(thr_scatter<TLOG, PTO>(tau));
+    if constexpr (LEAD && !kWave) __syncthreads();
 #pragma unroll
     for (int r = 0; r < G::E; ++r) wbase[padded(G::reg_scatter(PFROM, r))] = v[r];
-    __syncthreads();
+    if constexpr (kWave) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
 #pragma unroll
     for (int r = 0; r < G::E; ++r) v[r] = rbase[padded(G::reg_scatter(PTO, r))];
-    __syncthreads();
+    if constexpr (kWave) {
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
 }
 
 // Group index (index bits above beta) of register r in phase P.  When every
@@ -205,8 +231,9 @@ __device__ __forceinline__ void forward_from(uint32_t (&a)[Geo<TLOG>::E], uint32
                                              uint32_t p, uint32_t p2) {
     using G = Geo<TLOG>;
     if constexpr (P > 0) {
-        exchange<TLOG, P - 1, P>(a, lds, tau);
-        exchange<TLOG, P - 1, P>(b, lds, tau);
+        constexpr bool lead = P >= 2 && wave_local<TLOG>(P - 2, P - 1);
+        exchange<TLOG, P - 1, P, lead>(a, lds, tau);
+        exchange<TLOG, P - 1, P, false>(b, lds, tau);
     }
     fwd_phase2<TLOG, P>(a, b, tw, bm, thr_scatter<TLOG, P>(tau), p, p2);
     if constexpr (P + 1 < G::NPH) forward_from<TLOG, P + 1>(a, b, lds, tau, tw, bm, p, p2);
@@ -216,7 +243,12 @@ template <int TLOG, int P>
 __device__ __forceinline__ void inverse_from(uint32_t (&a)[Geo<TLOG>::E], uint32_t* lds, uint32_t tau,
                                              const Tw* tw, uint32_t bm, uint32_t p, uint32_t p2) {
     using G = Geo<TLOG>;
-    if constexpr (P + 1 < G::NPH) exchange<TLOG, P + 1, P>(a, lds, tau);
+    if constexpr (P + 1 < G::NPH) {
+        // the exchange before this one: the inverse step above, or the last
+        // forward step when this is the first inverse exchange
+        constexpr bool lead = P + 2 < G::NPH ? wave_local<TLOG>(P + 2, P + 1) : wave_local<TLOG>(P, P + 1);
+        exchange<TLOG, P + 1, P, lead>(a, lds, tau);
+    }
     inv_phase<TLOG, P>(a, tw, bm, thr_scatter<TLOG, P>(tau), p, p2);
     if constexpr (P > 0) inverse_from<TLOG, P - 1>(a, lds, tau, tw, bm, p, p2);
 }

@@ -94,6 +94,7 @@ struct Request {
     uint32_t* c = nullptr;
     size_t offset = 0, bytes = 0;  // arena slice: a, b, c, ra, rb
     Clock::time_point submitted, finished;
+    bool c_pinned = false;  // c is page-locked host memory: copy back on the launch stream
     bool done = false;
     int status = PM_OK;
 };
@@ -201,6 +202,13 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
         return false;
     }
     if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, false, L.stream);
+    // results into page-locked caller buffers go back asynchronously, behind
+    // the kernels on the same stream; other buffers are copied on completion
+    for (uint32_t j = 0; j < count && e == hipSuccess; ++j) {
+        if (!reqs[j]->c_pinned) continue;
+        const size_t bytes = (size_t(1) << reqs[j]->logn) * kLimbs * 4;
+        e = hipMemcpyAsync(reqs[j]->c, L.h_jobs[j].c, bytes, hipMemcpyDeviceToHost, L.stream);
+    }
     if (e == hipSuccess) e = hipEventRecord(L.finished, L.stream);
     if (e != hipSuccess) {
         error = std::string("launch: ") + hipGetErrorString(e);
@@ -215,8 +223,9 @@ void pm_server::finish_launch(Launch& L) {
     // phased launches have no in-kernel waits, so no device error flag to read
     const uint32_t err = 0;
     hipError_t e = hipStreamSynchronize(L.stream);
-    // results to the callers' buffers (the kernel has finished: plain copies)
+    // results to pageable callers' buffers (the kernels have finished)
     for (size_t j = 0; j < L.requests.size() && e == hipSuccess; ++j) {
+        if (L.requests[j]->c_pinned) continue;
         const size_t bytes = (size_t(1) << L.requests[j]->logn) * kLimbs * 4;
         e = hipMemcpy(L.requests[j]->c, L.h_jobs[j].c, bytes, hipMemcpyDeviceToHost);
     }
@@ -370,6 +379,9 @@ int pm_server_submit(pm_server* s, uint32_t n, const uint32_t* a, const uint32_t
     r->b = b;
     r->c = c;
     r->bytes = s->request_bytes(logn);
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, c) == hipSuccess) r->c_pinned = attr.type == hipMemoryTypeHost;
+    else (void)hipGetLastError();  // pageable memory: not an error
     if (r->bytes > s->budget) {
         delete r;
         return pm_fail(s->ctx, PM_EBUDGET, "request exceeds the HBM budget");

@@ -62,3 +62,35 @@ def test_budget_admission_queues(pm, workload):
     with pytest.raises(pm.PolymulError):
         pm.Server(ctx, request_bytes(1 << 12, k), max_batch=4, max_inflight=1)
     ctx.close()
+
+
+def test_served_page_locked_outputs(pm, workload):
+    """Outputs in page-locked memory come back on the launch stream; mixed
+    with pageable outputs in the same launches, every result is the alone one."""
+    import torch
+    q = workload.modulus(6)
+    ctx = pm.Context(q, 1 << 15)
+    ns = [1 << 15, 1 << 11, 1 << 13, 1 << 15, 1 << 12, 1 << 14]
+    server = pm.Server(ctx, 2 << 30, max_batch=4, max_inflight=2)
+    reqs = []
+    for i, n in enumerate(ns):
+        a, b = workload.family("dense-random-1", n, q, 70 + i)
+        out = None
+        if i % 2 == 0:
+            out = torch.empty(n * 28, dtype=torch.int32).pin_memory().numpy().view(np.uint32).reshape(n, 28)
+            out.fill(0xFFFFFFFF)
+        reqs.append((a, b, out, server.submit(a, b, out=out)))
+    for i, (a, b, out, rid) in enumerate(reqs):
+        c, _ = server.wait(rid)
+        if out is not None:
+            assert c is out
+        assert (c == ctx.multiply(a, b)).all(), i
+    server.close()
+    with pytest.raises(ValueError):
+        server2 = pm.Server(ctx, 1 << 30, max_batch=1, max_inflight=1)
+        try:
+            a, b = workload.family("dense-random-1", 64, q, 1)
+            server2.submit(a, b, out=np.empty((64, 27), dtype=np.uint32))
+        finally:
+            server2.close()
+    ctx.close()
