@@ -1,6 +1,6 @@
 """Throughput of the exact 868-bit negacyclic product on one MI355X.
 
-Workload (one step): one persistent, graph-captured launch over a batch of B
+Workload (one step): one graph replay (one launch per phase) over a batch of B
 independent products c = a*b mod (X^N+1, q), N = 32768, q a seeded random
 odd 868-bit modulus, coefficients uniform in [0,q) as 28 u32 limbs, inputs
 resident in HBM.  `value` is batched products/s over K timed steps; the plain
@@ -252,8 +252,6 @@ def run_uneven(args):
     print(f"[bench] uneven inputs ready: {len(ns)} products", file=sys.stderr, flush=True)
     results = {}
     outputs = {}
-    # the persistent mode is not run on uneven mixes (its uneven-mix test is
-    # skipped after a GPU fault of a since-removed variant)
     for mode in ("phased",):
         batch = pm.Batch(ctx, ns, mode=mode)
         for i, (a, b) in enumerate(inputs):
@@ -421,8 +419,8 @@ def main():
     ap.add_argument("--plain", type=int, default=200, help="plain single-product launches to time")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", default="phased", choices=["phased", "persistent"],
-                    help="batched launch: one launch per phase, or one persistent kernel")
+    ap.add_argument("--mode", default="phased", choices=["phased", "phased-scalar"],
+                    help="batched launch: one launch per phase (int8 MFMA or scalar residue conversion)")
     ap.add_argument("--workload", default="headline", choices=["headline", "uneven", "large", "serve"])
     ap.add_argument("--count", type=int, default=0, help="products / requests for the extra workloads")
     ap.add_argument("--rates", type=float, nargs="*", default=[2000.0, 8000.0])

@@ -217,6 +217,11 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
     if (flags & ~uint32_t(PM_BATCH_PERSISTENT | PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION)) return PM_EINVAL;
     if ((flags & PM_BATCH_PERSISTENT) && (flags & (PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION))) return PM_EINVAL;
     *out = nullptr;
+    // The persistent kernel hung the GPU on a uniform batch after a change to
+    // the tile code it shares; no cause was found, so the mode is switched off.
+    if (flags & PM_BATCH_PERSISTENT)
+        return pm_fail(ctx, PM_EUNSUPPORTED, "persistent batch mode is disabled (it hung the GPU in testing); "
+                                             "use the phased mode");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return pm_hip(ctx, e, "set device");
     pm_batch* b = new (std::nothrow) pm_batch();

@@ -35,7 +35,7 @@ struct Tables {
     const uint32_t* crt_negkp;    // [k+1][28]
     const uint32_t* q;            // [28]
     uint32_t mu0, mu1;            // floor(2^908/q)
-    uint32_t k, quads, max_logn, pad;
+    uint32_t k, quads, max_logn, cus;  // cus: compute units of the device
 };
 
 // One product of the batch / the plain path.

@@ -269,10 +269,13 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
     const uint32_t x0 = thr_scatter<TLOG, 0>(tau);
     const uint32_t* a_base = a_src + x0;
     const uint32_t* b_base = b_src + x0;
+    // unconditional loads (a padding tile points at a real tile of the same
+    // size and only its stores are skipped): per-element conditions would
+    // turn the loads into a chain of branches
 #pragma unroll
     for (int r = 0; r < G::E; ++r) {
-        a[r] = valid ? a_base[G::reg_scatter(0, r)] : 0u;
-        b[r] = valid ? b_base[G::reg_scatter(0, r)] : 0u;
+        a[r] = a_base[G::reg_scatter(0, r)];
+        b[r] = b_base[G::reg_scatter(0, r)];
     }
     forward_from<TLOG, 0>(a, b, lds, tau, tb.fwd + table, bm, p, p2);
 #pragma unroll
@@ -623,7 +626,7 @@ __device__ __forceinline__ void tile_item(const Tables& tb, const BatchArgs& ba,
     const uint32_t tile = threadIdx.x >> G::TPTLOG, tau = threadIdx.x & (G::TPT - 1);
     const uint32_t count = it.a0 >> 8;
     const bool valid = tile < count;
-    const Unit unit = valid ? ba.units[it.a1 + tile] : Unit{0u, 0u};
+    const Unit unit = ba.units[it.a1 + (valid ? tile : 0u)];  // padding reads the item's first tile
     const Job job = ba.jobs[unit.job];
     const uint32_t prime = unit.prime_s >> 2, s = unit.prime_s & 3u;
     const size_t off = (size_t(prime) << job.logn) + (size_t(s) << TLOG);

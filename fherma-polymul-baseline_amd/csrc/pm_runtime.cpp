@@ -138,6 +138,9 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     tb.k = k;
     tb.quads = quads;
     tb.max_logn = ctx->max_logn;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    tb.cus = uint32_t(cus);
     if (status != PM_OK) {
         const std::string reason = ctx->last_error;
         pm_destroy(ctx);

@@ -91,16 +91,16 @@ int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c
 
 /* ---- batched products (no counterpart in the reference) --------------------
  * Many products of uneven N (powers of two, 32 <= N <= max_n) under the
- * context's q, run as one persistent kernel over a dependency-ordered work
- * list, captured once in a hipGraph and replayed by pm_batch_run.  With
+ * context's q, run from a dependency-ordered work list whose launches are
+ * captured once in a hipGraph and replayed by pm_batch_run.  With
  * d_a/d_b/d_c all non-NULL the caller owns the n_i*28-word device buffers of
  * product i; otherwise the batch allocates them (pm_batch_io returns them). */
 typedef struct pm_batch pm_batch;
 /* flags: PM_BATCH_PHASED (default, 0 also selects it) runs one launch per
  * phase -- conversion, outer stages, one per tile size class, CRT -- inside
- * the captured graph; PM_BATCH_PERSISTENT runs the whole work list as one
- * persistent kernel whose blocks claim items in order and wait on per-product
- * completion counters. */
+ * the captured graph; PM_BATCH_PERSISTENT (one persistent kernel whose blocks
+ * claim items in order and wait on per-product completion counters) is
+ * currently disabled and returns PM_EUNSUPPORTED: it hung the GPU in testing. */
 #define PM_BATCH_PERSISTENT 1u
 #define PM_BATCH_PHASED 2u
 #define PM_BATCH_SCALAR_CONVERSION 4u /* phased: scalar limb reduction instead of int8 MFMA */
