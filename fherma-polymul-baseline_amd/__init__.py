@@ -75,6 +75,7 @@ def library():
     lib.pm_stage_convert_scalar.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
     lib.pm_stage_crt.argtypes = [P, u32, P, P, P]
+    lib.pm_stage_crt_scalar.argtypes = [P, u32, P, P, P]
     lib.pm_batch_create.argtypes = [P, u32, P, P, P, P, u32, ctypes.POINTER(P)]
     lib.pm_batch_destroy.argtypes = [P]
     lib.pm_batch_destroy.restype = None
@@ -100,7 +101,8 @@ def library():
 
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
-            "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt",
+            "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_convert_scalar", "pm_stage_ntt",
+            "pm_stage_crt", "pm_stage_crt_scalar",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
             "pm_batch_run", "pm_batch_wait", "pm_batch_info", "pm_plan_worklist", "pm_batch_phase_times",
             "pm_server_create", "pm_server_submit", "pm_server_wait", "pm_server_stats", "pm_server_destroy"]
@@ -231,11 +233,14 @@ class Context:
     def stage_crt(self, n, t, c, stream=None):
         self._check("pm_stage_crt", self._lib.pm_stage_crt(self._h, n, _ptr(t), _ptr(c), stream))
 
+    def stage_crt_scalar(self, n, t, c, stream=None):
+        self._check("pm_stage_crt_scalar", self._lib.pm_stage_crt_scalar(self._h, n, _ptr(t), _ptr(c), stream))
+
 
 class Batch:
     """Products of uneven N in one persistent, graph-captured launch."""
 
-    PERSISTENT, PHASED, SCALAR_CONVERSION = 1, 2, 4
+    PERSISTENT, PHASED, SCALAR_CONVERSION, SCALAR_CRT = 1, 2, 4, 8
 
     def __init__(self, ctx, ns, d_a=None, d_b=None, d_c=None, mode="phased"):
         self._ctx, self._lib = ctx, ctx._lib
@@ -247,7 +252,7 @@ class Batch:
             ptrs = [(ctypes.c_void_p * count)(*[_ptr(x) for x in v]) for v in (d_a, d_b, d_c)]
         handle = ctypes.c_void_p()
         flags = {"phased": self.PHASED, "persistent": self.PERSISTENT,
-                 "phased-scalar": self.PHASED | self.SCALAR_CONVERSION}[mode]
+                 "phased-scalar": self.PHASED | self.SCALAR_CONVERSION | self.SCALAR_CRT}[mode]
         ctx._check("pm_batch_create", self._lib.pm_batch_create(ctx.handle, count, arr, ptrs[0], ptrs[1], ptrs[2],
                                                                  flags, ctypes.byref(handle)))
         self.mode = mode
@@ -281,7 +286,7 @@ class Batch:
     def wait(self, stream=None):
         self._ctx._check("pm_batch_wait", self._lib.pm_batch_wait(self._h, stream))
 
-    PHASE_NAMES = {0: "conversion", 1: "outer_forward", 3: "outer_inverse", 4: "crt", 5: "conversion"}
+    PHASE_NAMES = {0: "conversion", 1: "outer_forward", 3: "outer_inverse", 4: "crt", 5: "conversion", 6: "crt"}
 
     def phase_times(self, stream=None):
         """One direct (ungraphed) run with events between phases: [(name, ms)]."""

@@ -124,7 +124,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     if (e == hipSuccess) {
         hipError_t le = b->persistent
                             ? pmd::launch_persistent(ctx->tables, ba, b->d_words, b->reset_count, b->blocks, cap)
-                            : pmd::launch_phased(ctx->tables, ba, b->ranges, b->scalar_conversion, cap);
+                            : pmd::launch_phased(ctx->tables, ba, b->ranges, b->scalar_conversion, b->scalar_crt, cap);
         hipError_t ce = hipStreamEndCapture(cap, &b->graph);
         e = le != hipSuccess ? le : ce;
     }
@@ -214,8 +214,11 @@ extern "C" {
 int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
                     const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out) {
     if (!ctx || !ns || !out || count == 0) return PM_EINVAL;
-    if (flags & ~uint32_t(PM_BATCH_PERSISTENT | PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION)) return PM_EINVAL;
-    if ((flags & PM_BATCH_PERSISTENT) && (flags & (PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION))) return PM_EINVAL;
+    if (flags & ~uint32_t(PM_BATCH_PERSISTENT | PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION | PM_BATCH_SCALAR_CRT))
+        return PM_EINVAL;
+    if ((flags & PM_BATCH_PERSISTENT) &&
+        (flags & (PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION | PM_BATCH_SCALAR_CRT)))
+        return PM_EINVAL;
     *out = nullptr;
     // The persistent kernel hung the GPU on a uniform batch after a change to
     // the tile code it shares; no cause was found, so the mode is switched off.
@@ -230,6 +233,7 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
     b->count = count;
     b->persistent = (flags & PM_BATCH_PERSISTENT) != 0;
     b->scalar_conversion = (flags & PM_BATCH_SCALAR_CONVERSION) != 0;
+    b->scalar_crt = (flags & PM_BATCH_SCALAR_CRT) != 0;
     const int st = pm_batch_build(ctx, b, ns, d_a, d_b, d_c);
     if (st != PM_OK) {
         delete b;
@@ -312,7 +316,7 @@ int pm_batch_phase_times(pm_batch* b, void* hip_stream, uint32_t max, uint32_t* 
                          uint32_t* count) {
     if (!b || !count) return PM_EINVAL;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->ctx->stream;
-    const std::vector<pmd::Phase> phases = pmd::phase_list(b->ranges, b->scalar_conversion);
+    const std::vector<pmd::Phase> phases = pmd::phase_list(b->ranges, b->scalar_conversion, b->scalar_crt);
     *count = uint32_t(phases.size());
     if (phases.size() > max || !kinds || !ms) return PM_OK;
     std::vector<hipEvent_t> ev(phases.size() + 1, nullptr);

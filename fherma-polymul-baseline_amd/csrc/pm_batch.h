@@ -25,6 +25,7 @@ struct pm_batch {
     pmd::PhaseRanges ranges{};
     bool persistent = false;        // one persistent launch, else one launch per phase
     bool scalar_conversion = false; // phased only: scalar limb reduction instead of MFMA
+    bool scalar_crt = false;        // phased only: vector-ALU CRT sums instead of MFMA
     uint32_t* d_words = nullptr;    // claim counter, error flag, per-job counters
     std::vector<void*> allocations;
     uint64_t device_bytes = 0;

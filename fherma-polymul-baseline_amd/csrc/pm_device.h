@@ -14,6 +14,13 @@ constexpr uint32_t kLdsWords = kTileElems + kTileElems / 32;  // padded exchange
 constexpr uint32_t kConvPerWave = 64;      // coefficients per wave in the conversion
 constexpr uint32_t kConvPerBlock = kConvPerWave * (kBlock / kWaveSize);  // 1024
 constexpr uint32_t kCrtPerBlock = 256;
+// MFMA CRT: groups of 64 coefficients per wave, 8 waves per block; per wave
+// the LDS holds 31 words (low 32 bits + signed high 16 bits) per coefficient
+constexpr uint32_t kCrtWaveCoefs = 64;
+constexpr uint32_t kCrtMfmaWaves = 8;
+constexpr uint32_t kCrtWords = 31;
+constexpr uint32_t kCrtLdsStride = 68;  // words per row (64 + 4: spreads the banks)
+constexpr uint32_t kCrtWaveLds = kCrtWords * kCrtLdsStride * 6;  // bytes: u32 low + i16 high
 
 struct Tw {
     uint32_t value, shoup;
@@ -33,6 +40,8 @@ struct Tables {
     const uint32_t* crt_m30;      // [k][29]
     const uint32_t* crt_round;    // [k] floor(2^42/p)
     const uint32_t* crt_negkp;    // [k+1][28]
+    const uint4* crt_a;           // [8 row tiles][4 k steps][64 lanes] int8x16 MFMA operands
+    const int4* crt_corr;         // [32]: row corrections of rows 4r..4r+3
     const uint32_t* q;            // [28]
     uint32_t mu0, mu1;            // floor(2^908/q)
     uint32_t k, quads, max_logn, cus;  // cus: compute units of the device

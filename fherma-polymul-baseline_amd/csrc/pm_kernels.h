@@ -16,8 +16,10 @@ hipError_t launch_convert_scalar(const Tables& tb, const Job& job, hipStream_t s
 // forward NTTs, pointwise product, inverse NTT, scale; u outer radix-2 stages
 // run as separate passes (tile size 2^(logn-u) <= 2^15)
 hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream);
-// CRT and reduction modulo q into 28-limb coefficients
+// CRT and reduction modulo q into 28-limb coefficients (int8 MFMA sums)
 hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream);
+// the same by 64-bit multiply-accumulate on the vector ALUs (comparison path)
+hipError_t launch_crt_scalar(const Tables& tb, const Job& job, hipStream_t stream);
 // reset counters, then one persistent launch over the batch work list
 hipError_t launch_persistent(const Tables& tb, const BatchArgs& ba, uint32_t* reset_words, uint32_t reset_count,
                              uint32_t blocks, hipStream_t stream);
@@ -31,17 +33,18 @@ struct PhaseRanges {
 };
 // phase kinds (tile phases carry the tile size class: kPhaseTile | tlog << 8)
 enum PhaseKind : uint32_t {
-    kPhaseConv = 0, kPhaseOuterFwd = 1, kPhaseTile = 2, kPhaseOuterInv = 3, kPhaseCrt = 4, kPhaseConvScalar = 5
+    kPhaseConv = 0, kPhaseOuterFwd = 1, kPhaseTile = 2, kPhaseOuterInv = 3, kPhaseCrt = 4, kPhaseConvScalar = 5,
+    kPhaseCrtScalar = 6
 };
 struct Phase {
     uint32_t kind, first, count;
 };
-std::vector<Phase> phase_list(const PhaseRanges& ranges, bool scalar_conversion);
+std::vector<Phase> phase_list(const PhaseRanges& ranges, bool scalar_conversion, bool scalar_crt);
 hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, uint32_t first, uint32_t count,
                         hipStream_t stream);
 // one launch per phase (and per tile size class) over the same work list
 hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& ranges, bool scalar_conversion,
-                         hipStream_t stream);
+                         bool scalar_crt, hipStream_t stream);
 // raise the dynamic LDS limit of the tile kernels (call once per device)
 hipError_t configure_kernels();
 

@@ -376,6 +376,9 @@ __device__ __forceinline__ void convert_scalar_one(const Tables& tb, const uint3
 // ---------------------------------------------------------------------------
 // CRT: one output coefficient per thread
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], uint64_t est,
+                                           uint32_t* __restrict__ out);
+
 __device__ __forceinline__ void crt_one(const Tables& tb, const uint32_t* __restrict__ t_res, uint32_t n,
                                         uint32_t c, uint32_t* __restrict__ out) {
     uint64_t acc[31];
@@ -419,59 +422,169 @@ __device__ __forceinline__ void crt_one(const Tables& tb, const uint32_t* __rest
         const int L = (32 * w) / 30, s = (32 * w) % 30;
         x[w] = (limb[L] >> s) | (L + 1 < 31 ? (limb[L + 1] << (30 - s)) : 0u);
     }
+    crt_finish(tb, x, est, out + size_t(c) * kLimbs);
+}
+
+// From the 29 words of A = sum_i t_i (M_i mod q) and the rounding estimate
+// est = sum_i t_i floor(2^42/p_i): K = nearest integer of sum_i t_i/p_i,
+// X = A + (-K*P mod q) < 2^904, then Barrett reduction modulo q and the
+// 28-word store (role of rns/solve.cu:336-420).
+__device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], uint64_t est,
+                                           uint32_t* __restrict__ out) {
+    const uint32_t k = tb.k;
     // nearest integer quotient of the CRT sum by P; add (-K*P) mod q
     uint32_t K = uint32_t((est + (uint64_t(1) << 41)) >> 42);
     K = K <= k ? K : k;  // canonical t_i give K <= k; clamp keeps the table read in bounds
     const uint32_t* nk = tb.crt_negkp + size_t(K) * kLimbs;
-    uint64_t carry = 0;
+    uint32_t cin = 0;
 #pragma unroll
-    for (int w = 0; w < 28; ++w) {
-        const uint64_t s = uint64_t(x[w]) + nk[w] + carry;
-        x[w] = uint32_t(s);
-        carry = s >> 32;
-    }
-    x[28] += uint32_t(carry);
-    // Barrett: X < 2^904, Qh = floor(floor(X/2^864) * floor(2^908/q) / 2^44)
+    for (int w = 0; w < 28; ++w) x[w] = __builtin_addc(x[w], nk[w], cin, &cin);
+    x[28] += cin;
+    // Barrett: X < 2^904, Qh = floor(floor(X/2^864) * floor(2^908/q) / 2^44) < 2^37
     const uint64_t xh = uint64_t(x[27]) | (uint64_t(x[28]) << 32);
     const uint64_t mu = uint64_t(tb.mu0) | (uint64_t(tb.mu1) << 32);
     const uint64_t plo = xh * mu, phi = __umul64hi(xh, mu);
     const uint64_t qh = (phi << 20) | (plo >> 44);
     const uint32_t q0 = uint32_t(qh), q1 = uint32_t(qh >> 32);
-    // X -= qh * q
+    // X -= q0*q and X -= q1*q*2^32: two multiply chains (each word product
+    // plus the previous carry fits 64 bits) with their own borrow chains; the
+    // true result is in [0, 2q), so the wrapped 29-word arithmetic is exact
     const uint32_t* qw = tb.q;
-    uint64_t mc = 0;          // carry of qh*q
-    int64_t borrow = 0;
-    uint32_t prev_hi1 = 0;    // high part of q1*q[w-1]
+    uint64_t acc0 = 0, acc1 = 0;
+    uint32_t b0 = 0, b1 = 0;
 #pragma unroll
     for (int w = 0; w < 29; ++w) {
-        const uint32_t qwd = w < 28 ? qw[w] : 0u;
-        const uint32_t qwp = (w >= 1 && w - 1 < 28) ? qw[w - 1] : 0u;
-        const uint64_t p0 = uint64_t(q0) * qwd;
-        const uint64_t p1 = uint64_t(q1) * qwp;
-        const uint64_t sum = (p0 & 0xFFFFFFFFull) + (p1 & 0xFFFFFFFFull) + mc + prev_hi1;
-        const uint32_t prod_w = uint32_t(sum);
-        mc = (sum >> 32) + (p0 >> 32);
-        prev_hi1 = uint32_t(p1 >> 32);
-        const int64_t d = int64_t(x[w]) - int64_t(prod_w) - borrow;
-        x[w] = uint32_t(d);
-        borrow = d < 0 ? 1 : 0;
+        acc0 = uint64_t(q0) * (w < 28 ? qw[w] : 0u) + (acc0 >> 32);
+        x[w] = __builtin_subc(x[w], uint32_t(acc0), b0, &b0);
+        acc1 = uint64_t(q1) * (w >= 1 ? qw[w - 1] : 0u) + (acc1 >> 32);
+        x[w] = __builtin_subc(x[w], uint32_t(acc1), b1, &b1);
     }
     // result in [0, 2q): one conditional subtraction
     uint32_t y[28];
-    borrow = 0;
+    uint32_t bq = 0;
 #pragma unroll
-    for (int w = 0; w < 28; ++w) {
-        const int64_t d = int64_t(x[w]) - int64_t(qw[w]) - borrow;
-        y[w] = uint32_t(d);
-        borrow = d < 0 ? 1 : 0;
-    }
-    const int64_t top = int64_t(x[28]) - borrow;
-    const bool ge = top >= 0;
-    uint4* dst = reinterpret_cast<uint4*>(out + size_t(c) * kLimbs);
+    for (int w = 0; w < 28; ++w) y[w] = __builtin_subc(x[w], qw[w], bq, &bq);
+    const bool ge = x[28] >= bq;
+    uint4* dst = reinterpret_cast<uint4*>(out);
 #pragma unroll
     for (int w = 0; w < 28; w += 4)
         dst[w / 4] = ge ? make_uint4(y[w], y[w + 1], y[w + 2], y[w + 3])
                         : make_uint4(x[w], x[w + 1], x[w + 2], x[w + 3]);
+}
+
+// CRT on int8 MFMA, one wave per group of 64 coefficients [c0, c0+64).  The
+// sums A = sum_i t_i (M_i mod q) and est = sum_i t_i floor(2^42/p_i) are one
+// exact integer GEMM D[j][c] = sum_{i,u} A[j][4i+u] * byte_u(t_i(c)) over byte
+// positions j (rows, 8 tiles of 16) and coefficients (4 column tiles of 16),
+// K = 64 primes x 4 bytes in 4 steps; the unsigned bytes enter as byte - 128
+// and the row sums of A (the accumulators' start values) restore them.  Lane l
+// of a tile holds rows 4(l/16)..+3 of coefficient l%16: one 32-bit word
+// position of the sum, kept as a 47-bit signed partial in LDS (low 32 bits,
+// high 16 bits); then each lane takes one coefficient, carries through its 29
+// words and finishes as the scalar CRT does, so the output is bit-identical.
+
+// B operands of a group: lane holds t of primes 16s+4g+{0..3} for coefficient
+// c0+16ct+col; indices are clamped (padding primes meet zero columns of A,
+// coefficients past n are never stored).
+typedef const __attribute__((address_space(1))) uint32_t* global_u32;
+
+__device__ __forceinline__ void crt_load_group(const uint32_t* t_res_any, uint32_t n, uint32_t k,
+                                               uint32_t c0, uint32_t lane, v4i (&bop)[4][4]) {
+    const global_u32 t_res = (global_u32)t_res_any;  // device memory: global (not flat) loads
+    const uint32_t g = lane >> 4, col = lane & 15u;
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            uint32_t c = c0 + 16u * ct + col;
+            c = c < n ? c : n - 1u;
+            uint32_t v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                uint32_t i = 16u * s + 4u * g + j;
+                i = i < k ? i : k - 1u;
+                // < 64 * 2^17 words: a 32-bit byte offset from a scalar base
+                v[j] = *reinterpret_cast<global_u32>(reinterpret_cast<const __attribute__((address_space(1))) char*>(t_res) +
+                                                     ((i * n + c) << 2));
+            }
+            bop[s][ct] = v4i{int(v[0]), int(v[1]), int(v[2]), int(v[3])};
+        }
+}
+
+__device__ __forceinline__ void crt_mfma_group(const Tables& tb, const v4i (&bop)[4][4], const uint4* a_lds,
+                                               const int4* corr_lds, uint8_t* wave_lds, uint32_t lane, uint32_t n,
+                                               uint32_t c0, uint32_t* __restrict__ out) {
+    const uint32_t g = lane >> 4, col = lane & 15u;
+    uint32_t* lo = reinterpret_cast<uint32_t*>(wave_lds);
+    int16_t* hi = reinterpret_cast<int16_t*>(wave_lds + kCrtWords * kCrtLdsStride * 4);
+    v4i b[4][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+            b[s][ct] = bop[s][ct] ^ v4i{int(0x80808080u), int(0x80808080u), int(0x80808080u), int(0x80808080u)};
+#pragma unroll
+    for (int rt = 0; rt < 8; ++rt) {
+        const uint32_t word = 4u * rt + g;
+        const int4 corr = corr_lds[word < kCrtWords ? word : 0u];
+        v4i acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = v4i{corr.x, corr.y, corr.z, corr.w};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const uint4 au = a_lds[(rt * 4 + s) * 64 + lane];
+            const v4i a = v4i{int(au.x), int(au.y), int(au.z), int(au.w)};
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b[s][ct], acc[ct], 0, 0, 0);
+        }
+        if (word < kCrtWords) {
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                // |D| <= 64 primes * 128 * (3*255 + 63) < 2^22.7 (top byte of t < p < 2^30 is
+                // below 64), so d0 + 256 d1 and d2 + 256 d3 fit in int32 and the word
+                // partial u + 2^16 w stays below 2^47 in magnitude
+                const int32_t u = acc[ct][0] + acc[ct][1] * 256;
+                const int32_t w = acc[ct][2] + acc[ct][3] * 256;
+                const int64_t part = int64_t(u) + int64_t(w) * 65536;
+                const uint32_t at = word * kCrtLdsStride + 16u * ct + col;
+                lo[at] = uint32_t(part);
+                hi[at] = int16_t(part >> 32);
+            }
+        }
+    }
+    // the same wave reads back: LDS instructions of one wave execute in order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t c = c0 + lane;
+    uint32_t x[29];
+    int64_t carry = 0;
+#pragma unroll
+    for (int w = 0; w < 29; ++w) {
+        const int64_t v = int64_t(uint64_t(lo[w * kCrtLdsStride + lane])) +
+                          (int64_t(hi[w * kCrtLdsStride + lane]) << 32) + carry;
+        x[w] = uint32_t(v);
+        carry = v >> 32;  // arithmetic: partials are signed
+    }
+    const int64_t e0 = int64_t(uint64_t(lo[29 * kCrtLdsStride + lane])) + (int64_t(hi[29 * kCrtLdsStride + lane]) << 32);
+    const int64_t e1 = int64_t(uint64_t(lo[30 * kCrtLdsStride + lane])) + (int64_t(hi[30 * kCrtLdsStride + lane]) << 32);
+    const uint64_t est = uint64_t(e0 + (e1 << 32));
+    if (c < n) crt_finish(tb, x, est, out + size_t(c) * kLimbs);
+    // the next group of this wave rewrites the same LDS rows
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Shared operand tables to LDS (whole block), then a barrier.
+__device__ __forceinline__ void crt_tables_to_lds(const Tables& tb, uint4* a_lds, int4* corr_lds) {
+    for (uint32_t i = threadIdx.x; i < 8u * 4u * 64u; i += blockDim.x) a_lds[i] = tb.crt_a[i];
+    for (uint32_t i = threadIdx.x; i < 32u; i += blockDim.x) corr_lds[i] = tb.crt_corr[i];
+    __syncthreads();
+}
+
+// LDS of a CRT block with W waves: A operands, row corrections, per-wave rows
+template <int W>
+constexpr size_t crt_block_lds() {
+    return 8u * 4u * 64u * sizeof(uint4) + 32u * sizeof(int4) + size_t(W) * kCrtWaveLds;
 }
 
 // ---------------------------------------------------------------------------
@@ -857,6 +970,79 @@ __global__ void __launch_bounds__(kBlock) k_items_tile(Tables tb, BatchArgs ba, 
     tile_item<TLOG>(tb, ba, it, lds);
 }
 
+// Groups of 64 coefficients, looped by W-wave blocks (wave w of block b takes
+// groups b*W + w, + grid*W, ...); the next group's residues load while the
+// current one is computed.
+template <int W, typename GroupFn>
+__device__ __forceinline__ void crt_group_loop(const Tables& tb, uint32_t groups, GroupFn where) {
+    extern __shared__ uint8_t crt_lds[];
+    uint4* a_lds = reinterpret_cast<uint4*>(crt_lds);
+    int4* corr_lds = reinterpret_cast<int4*>(crt_lds + 8u * 4u * 64u * sizeof(uint4));
+    uint8_t* rows = crt_lds + 8u * 4u * 64u * sizeof(uint4) + 32u * sizeof(int4);
+    crt_tables_to_lds(tb, a_lds, corr_lds);
+    // the wave index is the same in every lane: make it (and all that follows
+    // from it: group, product, pointers) scalar
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * W;
+    uint32_t gi = blockIdx.x * W + wave;
+    if (gi >= groups) return;  // whole wave; no block barriers follow
+    const uint32_t* t;
+    uint32_t* out;
+    uint32_t n, c0;
+    where(gi, t, out, n, c0);
+    v4i cur[4][4];
+    crt_load_group(t, n, tb.k, c0, lane, cur);
+    uint8_t* my_rows = rows + wave * kCrtWaveLds;
+    for (;;) {
+        const uint32_t gn = gi + stride;
+        const bool more = gn < groups;
+        const uint32_t* t2;
+        uint32_t* out2;
+        uint32_t n2, c2;
+        where(more ? gn : gi, t2, out2, n2, c2);
+        v4i nxt[4][4];
+        crt_load_group(t2, n2, tb.k, c2, lane, nxt);  // a repeat of this group when none is left
+        crt_mfma_group(tb, cur, a_lds, corr_lds, my_rows, lane, n, c0, out);
+        if (!more) break;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) cur[s][ct] = nxt[s][ct];
+        gi = gn;
+        t = t2;
+        out = out2;
+        n = n2;
+        c0 = c2;
+    }
+}
+
+__global__ void __launch_bounds__(kCrtMfmaWaves * 64) k_crt_mfma(Tables tb, Job job) {
+    const uint32_t n = 1u << job.logn;
+    crt_group_loop<kCrtMfmaWaves>(tb, (n + kCrtWaveCoefs - 1) / kCrtWaveCoefs,
+                                  [&](uint32_t gi, const uint32_t*& t, uint32_t*& out, uint32_t& nn, uint32_t& c0) {
+                                      t = job.ra;
+                                      out = job.c;
+                                      nn = n;
+                                      c0 = gi * kCrtWaveCoefs;
+                                  });
+}
+
+// CRT items cover kBlock coefficients = kBlock / 64 groups each; groups past a
+// product's n (small N) load clamped data and store nothing
+__global__ void __launch_bounds__(kCrtMfmaWaves * 64) k_items_crt_mfma(Tables tb, BatchArgs ba, uint32_t first,
+                                                                       uint32_t count) {
+    constexpr uint32_t per_item = kBlock / kCrtWaveCoefs;
+    crt_group_loop<kCrtMfmaWaves>(tb, count * per_item,
+                                  [&](uint32_t gi, const uint32_t*& t, uint32_t*& out, uint32_t& nn, uint32_t& c0) {
+                                      const Item it = ba.items[first + gi / per_item];
+                                      const Job job = ba.jobs[it.job];
+                                      t = job.ra;
+                                      out = job.c;
+                                      nn = 1u << job.logn;
+                                      c0 = it.a0 * kBlock + (gi % per_item) * kCrtWaveCoefs;
+                                  });
+}
+
 __global__ void __launch_bounds__(kBlock) k_items_crt(Tables tb, BatchArgs ba, uint32_t first) {
     const Item it = ba.items[first + blockIdx.x];
     const Job job = ba.jobs[it.job];
@@ -970,7 +1156,15 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
         case kPhaseOuterInv:
             hipLaunchKernelGGL(k_items_outer, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
             break;
-        case kPhaseCrt:
+        case kPhaseCrt: {
+            const uint32_t groups = count * (kBlock / kCrtWaveCoefs);
+            uint32_t grid = (groups + kCrtMfmaWaves - 1) / kCrtMfmaWaves;
+            grid = grid < tb.cus ? grid : tb.cus;
+            hipLaunchKernelGGL(k_items_crt_mfma, dim3(grid), dim3(kCrtMfmaWaves * 64), crt_block_lds<kCrtMfmaWaves>(),
+                               stream, tb, ba, first, count);
+            break;
+        }
+        case kPhaseCrtScalar:
             hipLaunchKernelGGL(k_items_crt, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
             break;
         case kPhaseTile:
@@ -995,20 +1189,20 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
     return hipGetLastError();
 }
 
-std::vector<Phase> phase_list(const PhaseRanges& r, bool scalar_conversion) {
+std::vector<Phase> phase_list(const PhaseRanges& r, bool scalar_conversion, bool scalar_crt) {
     std::vector<Phase> out;
     if (r.conv_count) out.push_back({scalar_conversion ? kPhaseConvScalar : kPhaseConv, r.conv_first, r.conv_count});
     if (r.fwd_count) out.push_back({kPhaseOuterFwd, r.fwd_first, r.fwd_count});
     for (uint32_t tlog = 15; tlog >= 5; --tlog)
         if (r.tile_count[tlog]) out.push_back({kPhaseTile | (tlog << 8), r.tile_first[tlog], r.tile_count[tlog]});
     if (r.inv_count) out.push_back({kPhaseOuterInv, r.inv_first, r.inv_count});
-    if (r.crt_count) out.push_back({kPhaseCrt, r.crt_first, r.crt_count});
+    if (r.crt_count) out.push_back({scalar_crt ? kPhaseCrtScalar : kPhaseCrt, r.crt_first, r.crt_count});
     return out;
 }
 
 hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& r, bool scalar_conversion,
-                         hipStream_t stream) {
-    for (const Phase& ph : phase_list(r, scalar_conversion)) {
+                         bool scalar_crt, hipStream_t stream) {
+    for (const Phase& ph : phase_list(r, scalar_conversion, scalar_crt)) {
         const hipError_t e = launch_phase(tb, ba, ph.kind, ph.first, ph.count, stream);
         if (e != hipSuccess) return e;
     }
@@ -1021,10 +1215,20 @@ hipError_t launch_convert_scalar(const Tables& tb, const Job& job, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
+hipError_t launch_crt_scalar(const Tables& tb, const Job& job, hipStream_t stream) {
     const uint32_t n = 1u << job.logn;
     const uint32_t threads = n < kCrtPerBlock ? ((n + 63) / 64) * 64 : kCrtPerBlock;
     hipLaunchKernelGGL(k_crt, dim3((n + threads - 1) / threads), dim3(threads), 0, stream, tb, job);
+    return hipGetLastError();
+}
+
+hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
+    const uint32_t n = 1u << job.logn;
+    const uint32_t groups = (n + kCrtWaveCoefs - 1) / kCrtWaveCoefs;
+    uint32_t grid = (groups + kCrtMfmaWaves - 1) / kCrtMfmaWaves;
+    grid = grid < tb.cus ? grid : tb.cus;
+    hipLaunchKernelGGL(k_crt_mfma, dim3(grid), dim3(kCrtMfmaWaves * 64), crt_block_lds<kCrtMfmaWaves>(), stream, tb,
+                       job);
     return hipGetLastError();
 }
 
@@ -1046,6 +1250,14 @@ static hipError_t allow_items_lds() {
 
 hipError_t configure_kernels() {
     hipError_t e = hipSuccess;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_crt_mfma),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 int(crt_block_lds<kCrtMfmaWaves>()))) != hipSuccess)
+        return e;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_crt_mfma),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 int(crt_block_lds<kCrtMfmaWaves>()))) != hipSuccess)
+        return e;
     if ((e = allow_lds<1>()) != hipSuccess) return e;
     if ((e = allow_lds<2>()) != hipSuccess) return e;
     if ((e = allow_lds<3>()) != hipSuccess) return e;

@@ -256,6 +256,36 @@ std::string build_params(uint32_t max_logn, const uint32_t* q_words, uint32_t li
         }
         basis_inverse[i] = pow_mod(mod_small(basis, p), p - 2, p);
         s.crt_round[i] = uint32_t((uint64_t(1) << kRoundShift) / p);
+        if (i >= 64) return "too many primes for the MFMA CRT (K = 4 bytes x 64 primes)";
+        // balanced base-256 digits of M_i mod q and of floor(2^42/p_i)
+        int8_t dig[kCrtDigits];
+        int64_t carry = 0;
+        for (uint32_t v = 0; v < kCrtDigits; ++v) {
+            const uint32_t word = v / 4 < reduced.size() ? reduced[v / 4] : 0u;
+            const int64_t x = int64_t((word >> (8 * (v % 4))) & 255u) + carry;
+            const int64_t d = ((x + 128) & 255) - 128;
+            dig[v] = int8_t(d);
+            carry = (x - d) / 256;
+        }
+        if (carry != 0) return "balanced digits of a CRT basis overflow";
+        int64_t rv = s.crt_round[i];
+        const int64_t e0 = ((rv + 128) & 255) - 128, e1 = (rv - e0) / 256;
+        if (e1 < -128 || e1 > 127) return "rounding constant out of digit range";
+        if (s.crt_mfma_a.empty()) {
+            s.crt_mfma_a.assign(8 * 4 * 64 * 16, 0);
+            s.crt_mfma_corr.assign(128, 0);
+        }
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t kk = 4 * i + u;
+            auto put = [&](uint32_t row, int8_t value) {
+                const uint32_t rt = row / 16, s_ = kk / 64, lane = (row % 16) + 16 * ((kk % 64) / 16);
+                s.crt_mfma_a[((size_t(rt) * 4 + s_) * 64 + lane) * 16 + kk % 16] = value;
+                s.crt_mfma_corr[row] += 128 * int32_t(value);
+            };
+            for (uint32_t v = 0; v < kCrtDigits; ++v) put(u + v, dig[v]);
+            put(kCrtRowE + u, int8_t(e0));
+            put(kCrtRowE + u + 1, int8_t(e1));
+        }
     }
     Big product_mod_q;
     divmod(product, q, nullptr, &product_mod_q);

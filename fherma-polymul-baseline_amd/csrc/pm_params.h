@@ -24,6 +24,8 @@ constexpr uint32_t kMaxPrimes = 64;
 constexpr uint32_t kCrtLimbs = 29;       // 30-bit limbs of M_i mod q (870 bits)
 constexpr uint32_t kConvBytes = 112;     // bytes of an ABI coefficient
 constexpr uint32_t kRoundShift = 42;     // fixed point of the CRT rounding estimate
+constexpr uint32_t kCrtDigits = 110;     // balanced base-256 digits of M_i mod q (< 2^868)
+constexpr uint32_t kCrtRowE = 116;       // first row of the rounding-estimate digits
 
 struct Twiddle {
     uint32_t value;
@@ -52,6 +54,12 @@ struct Params {
     std::vector<uint32_t> crt_round;             // k: floor(2^42 / p_i)
     std::vector<uint32_t> crt_neg_kp;            // (k+1) * 28: (-K * P) mod q
     std::vector<uint32_t> barrett_mu;            // 2 words: floor(2^908 / q)
+    // CRT on int8 MFMA: the 128 x 256 digit matrix A[j][4i+u] (row j = byte
+    // position of the sum; balanced base-256 digit j-u of M_i mod q for
+    // j < 113, of floor(2^42/p_i) at rows 116 + (j-u)) laid out as the A
+    // operand of v_mfma_i32_16x16x64_i8: [row tile 8][k step 4][lane 64][16]
+    std::vector<int8_t> crt_mfma_a;              // 8*4*64*16
+    std::vector<int32_t> crt_mfma_corr;          // 128: 128 * sum_k A[j][k] (unsigned-byte correction)
     uint32_t product_bits = 0;
 };
 

@@ -132,6 +132,11 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     tb.crt_m30 = upload(ctx, s.crt_bases30, &status);
     tb.crt_round = upload(ctx, s.crt_round, &status);
     tb.crt_negkp = upload(ctx, s.crt_neg_kp, &status);
+    tb.crt_a = reinterpret_cast<const uint4*>(upload(ctx, s.crt_mfma_a, &status));
+    {
+        std::vector<int32_t> corr(s.crt_mfma_corr);  // [row] -> int4 per 4 rows
+        tb.crt_corr = reinterpret_cast<const int4*>(upload(ctx, corr, &status));
+    }
     tb.q = upload(ctx, s.q, &status);
     tb.mu0 = s.barrett_mu[0];
     tb.mu1 = s.barrett_mu[1];
@@ -305,6 +310,16 @@ int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, ui
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, nullptr, nullptr, nullptr, d_ra, d_rb};
     return pm_hip(ctx, pmd::launch_ntt(ctx->tables, job, outer, stream), "launch NTT convolution");
+}
+
+int pm_stage_crt_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream) {
+    if (!ctx || !d_t || !d_c) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
+    pmd::Job job{logn, 0, nullptr, nullptr, d_c, const_cast<uint32_t*>(d_t), nullptr};
+    return pm_hip(ctx, pmd::launch_crt_scalar(ctx->tables, job, stream), "launch scalar CRT");
 }
 
 int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream) {

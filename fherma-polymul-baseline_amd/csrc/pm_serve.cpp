@@ -201,7 +201,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
         error = "work list phases are not contiguous";
         return false;
     }
-    if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, false, L.stream);
+    if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, false, false, L.stream);
     // results into page-locked caller buffers go back asynchronously, behind
     // the kernels on the same stream; other buffers are copied on completion
     for (uint32_t j = 0; j < count && e == hipSuccess; ++j) {

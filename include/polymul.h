@@ -86,8 +86,10 @@ int pm_stage_convert_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_a, co
 /* forward NTTs, pointwise, inverse, scale: ra <- N^-1 (M_i mod p_i)^-1 * (a*b mod p_i) --
  * rns/solve.cu:225-333; outer = number of radix-2 stages run as separate passes */
 int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, uint32_t* d_rb, void* hip_stream);
-/* CRT + mod q from scaled residues -- rns/solve.cu:336-420 */
+/* CRT + mod q from scaled residues -- rns/solve.cu:336-420 (int8 MFMA sums) */
 int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream);
+/* the same with the sums on the vector ALUs (comparison path, identical output) */
+int pm_stage_crt_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream);
 
 /* ---- batched products (no counterpart in the reference) --------------------
  * Many products of uneven N (powers of two, 32 <= N <= max_n) under the
@@ -104,6 +106,7 @@ typedef struct pm_batch pm_batch;
 #define PM_BATCH_PERSISTENT 1u
 #define PM_BATCH_PHASED 2u
 #define PM_BATCH_SCALAR_CONVERSION 4u /* phased: scalar limb reduction instead of int8 MFMA */
+#define PM_BATCH_SCALAR_CRT 8u        /* phased: CRT sums on the vector ALUs instead of int8 MFMA */
 int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
                     const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out);
 void pm_batch_destroy(pm_batch* batch);

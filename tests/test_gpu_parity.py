@@ -144,6 +144,33 @@ def test_crt_stage(pm, orc, workload, torch_cuda, n):
         assert (to_host(c).reshape(n, 28) == want).all(), family
 
 
+@pytest.mark.parametrize("n", [2, 64, 32768])
+def test_crt_mfma_matches_scalar(pm, workload, torch_cuda, n):
+    """The int8-MFMA CRT and the vector-ALU CRT give identical words for
+    canonical residues: random, all p_i - 1 (largest sums and quotient),
+    all zero, and per-coefficient mixtures of both extremes."""
+    say("crt mfma vs scalar", n)
+    q = workload.modulus(17)
+    ctx = context(pm, q, max(n, 1 << 15))
+    primes = np.asarray([p for p, _ in ctx.primes()], dtype=np.uint64)
+    k = len(primes)
+    rng = np.random.default_rng(n)
+    cases = {
+        "random": (rng.integers(0, 1 << 62, size=(k, n), dtype=np.uint64) % primes[:, None]),
+        "max": np.repeat((primes - 1)[:, None], n, axis=1),
+        "zero": np.zeros((k, n), dtype=np.uint64),
+        "mixed": np.where(rng.integers(0, 2, size=(k, n)) == 1, (primes - 1)[:, None], 0),
+    }
+    for name, t in cases.items():
+        t_dev = to_dev(torch_cuda, np.ascontiguousarray(t.astype(np.uint32)))
+        c_mfma = torch_cuda.zeros(n * 28, dtype=torch_cuda.int32, device="cuda")
+        c_scalar = torch_cuda.ones(n * 28, dtype=torch_cuda.int32, device="cuda")
+        ctx.stage_crt(n, t_dev, c_mfma)
+        ctx.stage_crt_scalar(n, t_dev, c_scalar)
+        ctx.synchronize()
+        assert (to_host(c_mfma) == to_host(c_scalar)).all(), name
+
+
 @pytest.mark.parametrize("n", [32, 64, 1024, 4096, 8192, 16384, 32768, 65536, 131072])
 def test_product_vs_golden(pm, workload, n):
     for case in GOLDEN["cases"]:
