@@ -71,6 +71,8 @@ def library():
     lib.pm_multiply.argtypes = [P, u32, P, P, P]
     lib.pm_multiply_device.argtypes = [P, u32, P, P, P, P]
     lib.pm_synchronize.argtypes = [P]
+    lib.pm_set_outer.argtypes = [P, u32, u32]
+    lib.pm_get_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_stage_convert.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_convert_scalar.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
@@ -101,8 +103,8 @@ def library():
 
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
-            "pm_multiply_device", "pm_synchronize", "pm_stage_convert", "pm_stage_convert_scalar", "pm_stage_ntt",
-            "pm_stage_crt", "pm_stage_crt_scalar",
+            "pm_multiply_device", "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_stage_convert",
+            "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
             "pm_batch_run", "pm_batch_wait", "pm_batch_info", "pm_plan_worklist", "pm_batch_phase_times",
             "pm_server_create", "pm_server_submit", "pm_server_wait", "pm_server_stats", "pm_server_destroy"]
@@ -218,6 +220,33 @@ class Context:
 
     def synchronize(self):
         self._check("pm_synchronize", self._lib.pm_synchronize(self._h))
+
+    def set_outer(self, n, outer):
+        """Plain-path schedule: outer radix-2 passes for products of size n."""
+        self._check("pm_set_outer", self._lib.pm_set_outer(self._h, int(n), int(outer)))
+
+    def get_outer(self, n):
+        v = ctypes.c_uint32()
+        self._check("pm_get_outer", self._lib.pm_get_outer(self._h, int(n), ctypes.byref(v)))
+        return v.value
+
+    def schedule(self):
+        """{n: outer} for every power of two up to max_n."""
+        return {1 << lg: self.get_outer(1 << lg) for lg in range(1, self.info()["max_n"].bit_length())}
+
+    def load_schedule(self, path):
+        """Apply a schedule written by `autotune_plain` (sizes this context
+        cannot hold are skipped)."""
+        import json
+        with open(path) as f:
+            table = json.load(f)["outer"]
+        applied = {}
+        for n_str, outer in table.items():
+            n = int(n_str)
+            if n <= self.info()["max_n"]:
+                self.set_outer(n, outer)
+                applied[n] = outer
+        return applied
 
     def stage_convert(self, n, a, b, ra, rb, stream=None):
         self._check("pm_stage_convert",
@@ -363,6 +392,50 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def autotune_plain(ctx, ns, reps=20, path=None, stream=None):
+    """Per-N schedule of the plain path: times one product of each size with
+    0, 1 and 2 outer radix-2 passes (where the tile stays within 2..2^15) on
+    the GPU, keeps the fastest in the context and returns
+    {n: {"outer": u, "ms": {u: median ms}}}; with `path` the choice is also
+    written as JSON ({"device": ..., "outer": {n: u}}) for `load_schedule`."""
+    import json
+    import torch
+    from . import workload as wl
+    s = stream if stream is not None else torch.cuda.Stream()
+    result = {}
+    for n in sorted(set(int(x) for x in ns)):
+        logn = n.bit_length() - 1
+        q = ctx.q
+        a = torch.from_numpy(wl.poly(n, 0, n, q).view(np.int32)).cuda()
+        b = torch.from_numpy(wl.poly(n, 1, n, q).view(np.int32)).cuda()
+        c = torch.empty_like(a)
+        times = {}
+        for outer in (0, 1, 2):
+            if outer > logn or logn - outer > 15 or logn - outer < 1:
+                continue
+            ctx.set_outer(n, outer)
+            with torch.cuda.stream(s):
+                for _ in range(3):
+                    ctx.multiply_device(n, a, b, c, s.cuda_stream)
+                ms = []
+                for _ in range(reps):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(s)
+                    ctx.multiply_device(n, a, b, c, s.cuda_stream)
+                    e1.record(s)
+                    e1.synchronize()
+                    ms.append(e0.elapsed_time(e1))
+            times[outer] = float(np.median(ms))
+        best = min(times, key=times.get)
+        ctx.set_outer(n, best)
+        result[n] = {"outer": best, "ms": times}
+    if path is not None:
+        with open(path, "w") as f:
+            json.dump({"device": torch.cuda.get_device_name(0),
+                       "outer": {str(n): r["outer"] for n, r in result.items()}}, f, indent=1)
+    return result
 
 
 class Solver:

@@ -491,7 +491,29 @@ typedef const __attribute__((address_space(1))) uint32_t* global_u32;
 __device__ __forceinline__ void crt_load_group(const uint32_t* t_res_any, uint32_t n, uint32_t k,
                                                uint32_t c0, uint32_t lane, v4i (&bop)[4][4]) {
     const global_u32 t_res = (global_u32)t_res_any;  // device memory: global (not flat) loads
+    typedef const __attribute__((address_space(1))) char* global_bytes;
     const uint32_t g = lane >> 4, col = lane & 15u;
+    if (c0 + kCrtWaveCoefs <= n) {
+        // whole group in range: one offset per prime, the 4 column tiles at
+        // immediate offsets of 64 bytes (< 64 * 2^17 words: 32-bit byte offsets)
+        uint32_t v[4][4][4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                uint32_t i = 16u * s + 4u * g + j;
+                i = i < k ? i : k - 1u;
+                const global_bytes row = reinterpret_cast<global_bytes>(t_res) + ((i * n + c0 + col) << 2);
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) v[s][ct][j] = *reinterpret_cast<global_u32>(row + 64 * ct);
+            }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+                bop[s][ct] = v4i{int(v[s][ct][0]), int(v[s][ct][1]), int(v[s][ct][2]), int(v[s][ct][3])};
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -503,9 +525,7 @@ __device__ __forceinline__ void crt_load_group(const uint32_t* t_res_any, uint32
             for (int j = 0; j < 4; ++j) {
                 uint32_t i = 16u * s + 4u * g + j;
                 i = i < k ? i : k - 1u;
-                // < 64 * 2^17 words: a 32-bit byte offset from a scalar base
-                v[j] = *reinterpret_cast<global_u32>(reinterpret_cast<const __attribute__((address_space(1))) char*>(t_res) +
-                                                     ((i * n + c) << 2));
+                v[j] = *reinterpret_cast<global_u32>(reinterpret_cast<global_bytes>(t_res) + ((i * n + c) << 2));
             }
             bop[s][ct] = v4i{int(v[0]), int(v[1]), int(v[2]), int(v[3])};
         }

@@ -281,6 +281,8 @@ int pm_multiply(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* 
 int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
                      uint32_t* d_rb, void* hip_stream) {
     if (!ctx || !d_a || !d_b || !d_ra || !d_rb) return PM_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(d_ra) | reinterpret_cast<uintptr_t>(d_rb)) & 15u)
+        return pm_fail(ctx, PM_EINVAL, "residue buffers must be 16-byte aligned");
     uint32_t logn = 0;
     int st = pm_check_n(ctx, n, &logn);
     if (st != PM_OK) return st;
@@ -292,6 +294,8 @@ int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uin
 int pm_stage_convert_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
                             uint32_t* d_rb, void* hip_stream) {
     if (!ctx || !d_a || !d_b || !d_ra || !d_rb) return PM_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(d_ra) | reinterpret_cast<uintptr_t>(d_rb)) & 15u)
+        return pm_fail(ctx, PM_EINVAL, "residue buffers must be 16-byte aligned");
     uint32_t logn = 0;
     int st = pm_check_n(ctx, n, &logn);
     if (st != PM_OK) return st;
@@ -310,6 +314,26 @@ int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, ui
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, nullptr, nullptr, nullptr, d_ra, d_rb};
     return pm_hip(ctx, pmd::launch_ntt(ctx->tables, job, outer, stream), "launch NTT convolution");
+}
+
+int pm_set_outer(pm_context* ctx, uint32_t n, uint32_t outer) {
+    if (!ctx) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    if (outer > 2 || outer > logn || logn - outer > 15 || logn - outer < 1)
+        return pm_fail(ctx, PM_EUNSUPPORTED, "outer stages must leave a tile of 2..2^15 (outer <= 2)");
+    ctx->outer[logn] = outer;
+    return PM_OK;
+}
+
+int pm_get_outer(pm_context* ctx, uint32_t n, uint32_t* outer) {
+    if (!ctx || !outer) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    *outer = ctx->outer[logn];
+    return PM_OK;
 }
 
 int pm_stage_crt_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream) {

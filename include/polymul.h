@@ -71,10 +71,17 @@ int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const u
                        void* hip_stream);
 /* Waits for the context's stream. */
 int pm_synchronize(pm_context* ctx);
+/* Plain-path schedule (no counterpart in the reference): the number of outer
+ * radix-2 stages (0..2) run as separate passes before/after the fused tiles
+ * for products of size n; the default is max(0, log2 n - 15).  More outer
+ * stages give more, smaller tiles -- more blocks for a single product.  Set by
+ * the autotuner in the Python mirror (`autotune_plain`). */
+int pm_set_outer(pm_context* ctx, uint32_t n, uint32_t outer);
+int pm_get_outer(pm_context* ctx, uint32_t n, uint32_t* outer);
 
 /* ---- stage entry points (device buffers, context stream) ------------------
  * The three kernels of the plain path, exposed so each can be checked alone.
- * Residue buffers are [k][n] uint32. */
+ * Residue buffers are [k][n] uint32, 16-byte aligned (PM_EINVAL otherwise). */
 /* residues of a and b: ra[i*n+j] = a_j mod p_i (value below 2*p_i) -- rns/solve.cu:138-157 */
 int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
                      uint32_t* d_rb, void* hip_stream);

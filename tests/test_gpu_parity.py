@@ -264,3 +264,33 @@ def test_solution_harness(workload, tmp_path):
     for i, case in enumerate(cases):
         got = (tmp_path / "out" / f"{i:06d}" / "c.bin").read_bytes()
         assert hashlib.sha256(got).hexdigest() == case["sha256"]["c"]
+
+
+def test_plain_schedule_and_autotune(pm, workload, tmp_path):
+    """Every plain-path schedule (outer passes 0..2) gives the same product;
+    the autotuner keeps the fastest per size and its JSON loads back."""
+    q = workload.modulus(23)
+    ctx = pm.Context(q, 1 << 15)
+    say("schedules")
+    for n in (64, 4096, 32768):
+        a, b = workload.family("dense-random-1", n, q, 9)
+        want = None
+        for outer in (0, 1, 2):
+            ctx.set_outer(n, outer)
+            assert ctx.get_outer(n) == outer
+            c = ctx.multiply(a, b)
+            want = c if want is None else want
+            assert (c == want).all(), (n, outer)
+    with pytest.raises(pm.PolymulError):
+        ctx.set_outer(32768, 3)
+    with pytest.raises(pm.PolymulError):
+        ctx.set_outer(2, 1)  # would leave a tile of one element
+    path = tmp_path / "schedule.json"
+    res = pm.autotune_plain(ctx, [1024, 32768], reps=3, path=str(path))
+    assert set(res) == {1024, 32768}
+    for n, r in res.items():
+        assert r["outer"] == ctx.get_outer(n) and r["outer"] in r["ms"]
+    ctx.set_outer(32768, 0 if res[32768]["outer"] else 1)
+    applied = ctx.load_schedule(str(path))
+    assert applied[32768] == res[32768]["outer"] == ctx.get_outer(32768)
+    ctx.close()
