@@ -146,7 +146,9 @@ def run_headline(args):
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_batched_n{n}_b{B}.json")
     if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        # HBM bytes per launch of the dominant phase from the FETCH_SIZE /
+        # WRITE_SIZE profile runs, scaled by a copy kernel of known size
+        traffic = json.load(open(tfile)).get("phases", {}).get(phase_key(dominant), {}).get("hbm_bytes_per_launch")
 
     # ---- CPU baseline: the oracle port on a bounded sample -------------------
     sys.path.insert(0, os.path.join(ROOT, "tests"))
