@@ -118,14 +118,21 @@ def run_headline(args):
     torch.cuda.synchronize()
     plain_rate = args.plain / (time.perf_counter() - t0)
     plain_c = dc.cpu().numpy().view(np.uint32).reshape(n, 28)
-    # drop-in path with host buffers (PCIe-inclusive), as fherma_run times it
+    # drop-in path with host buffers (PCIe-inclusive): submit + collect into a
+    # reused output array
     host_ms = []
+    host_c = np.empty_like(inputs[0][0])
     for _ in range(20):
         t1 = time.perf_counter()
-        host_c = ctx.multiply(inputs[0][0], inputs[0][1])
+        ctx.submit(inputs[0][0], inputs[0][1])
+        ctx.collect(out=host_c)
         host_ms.append((time.perf_counter() - t1) * 1e3)
+    # and as the reference's harness times it: the `solution` program calls
+    # fherma_init once and times each fherma_run (fresh output vector) alone
+    harness_ms, harness_exact = dropin_harness(pm, q, n, inputs[:args.harness_cases], plain_c)
     print(f"[bench] plain: {plain_rate:.0f} products/s, p50 {percentile(plain_ms, 50):.3f} ms, "
-          f"host drop-in p50 {percentile(host_ms, 50):.3f} ms", file=sys.stderr, flush=True)
+          f"host drop-in p50 {percentile(host_ms, 50):.3f} ms, fherma_run p50 {percentile(harness_ms, 50):.3f} ms",
+          file=sys.stderr, flush=True)
 
     batch_c = [batch.download(i) for i in range(min(len(inputs), B))]
     gpu_consistent = bool(np.array_equal(batch_c[0], plain_c) and np.array_equal(host_c, plain_c))
@@ -187,7 +194,10 @@ def run_headline(args):
             "batched_over_plain": round(batched_rate / plain_rate, 3),
             "dropin_host_buffers_p50_ms": round(percentile(host_ms, 50), 4),
             "dropin_host_buffers_p99_ms": round(percentile(host_ms, 99), 4),
-            "gpu_paths_agree": gpu_consistent,
+            "dropin_fherma_run_p50_ms": round(percentile(harness_ms, 50), 4),
+            "dropin_fherma_run_p99_ms": round(percentile(harness_ms, 99), 4),
+            "dropin_fherma_run_cases": len(harness_ms),
+            "gpu_paths_agree": gpu_consistent and harness_exact,
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -220,6 +230,34 @@ def run_headline(args):
     batch.close()
     ctx.close()
     return 0
+
+
+def dropin_harness(pm, q, n, cases, first_c):
+    """Runs the `solution` harness (fherma_init once, each fherma_run timed
+    alone, as main.cpp:181-186 does) over a point directory of these cases;
+    returns the per-case milliseconds and whether case 0 equals first_c."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory(prefix="polymul_point_") as d:
+        os.makedirs(os.path.join(d, "point"))
+        with open(os.path.join(d, "point", "q.bin"), "wb") as f:
+            f.write(np.asarray(q, dtype="<u4").tobytes())
+        for i, (a, b) in enumerate(cases):
+            cd = os.path.join(d, "cases", f"{i:06d}")
+            os.makedirs(cd)
+            with open(os.path.join(cd, "a.bin"), "wb") as f:
+                f.write(a.astype("<u4").tobytes())
+            with open(os.path.join(cd, "b.bin"), "wb") as f:
+                f.write(b.astype("<u4").tobytes())
+        with open(os.path.join(d, "manifest.json"), "w") as f:
+            json.dump({"point": {"N": n, "W": 868, "L": 28}, "cases": len(cases)}, f)
+        subprocess.run([os.path.join(PKG_DIR, "solution"), d], check=True, timeout=300,
+                       stdout=subprocess.DEVNULL)
+        report = json.load(open(os.path.join(d, "out", "results.json")))
+        ms = [c["seconds"] * 1e3 for c in report["cases"] if c["status"] == "ok"]
+        with open(os.path.join(d, "out", "000000", "c.bin"), "rb") as f:
+            c0 = np.frombuffer(f.read(), dtype="<u4").reshape(n, 28)
+    return ms, bool(len(ms) == len(cases) and np.array_equal(c0, first_c))
 
 
 def make_inputs(pm, q, ns, seed, pinned=False):
@@ -420,6 +458,8 @@ def main():
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--plain", type=int, default=200, help="plain single-product launches to time")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--harness-cases", type=int, default=30,
+                    help="cases of the fherma_run drop-in timing through the solution harness (<= 64)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", default="phased", choices=["phased", "phased-scalar"],
                     help="batched launch: one launch per phase (int8 MFMA or scalar residue conversion)")

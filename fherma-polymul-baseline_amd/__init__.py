@@ -69,6 +69,9 @@ def library():
     lib.pm_get_info.argtypes = [P, ctypes.POINTER(_Info)]
     lib.pm_get_prime.argtypes = [P, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]
     lib.pm_multiply.argtypes = [P, u32, P, P, P]
+    lib.pm_multiply_submit.argtypes = [P, u32, P, P]
+    lib.pm_multiply_collect.argtypes = [P, P]
+    lib.pm_set_io_chunks.argtypes = [P, u32, u32]
     lib.pm_multiply_device.argtypes = [P, u32, P, P, P, P]
     lib.pm_synchronize.argtypes = [P]
     lib.pm_set_outer.argtypes = [P, u32, u32]
@@ -103,7 +106,7 @@ def library():
 
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
-            "pm_multiply_device", "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_stage_convert",
+            "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_multiply_device", "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
             "pm_batch_run", "pm_batch_wait", "pm_batch_info", "pm_plan_worklist", "pm_batch_phase_times",
@@ -213,6 +216,33 @@ class Context:
         self._check("pm_multiply", self._lib.pm_multiply(self._h, a.shape[0], a.ctypes.data, b.ctypes.data,
                                                           c.ctypes.data))
         return c
+
+    def submit(self, a, b):
+        """First half of `multiply` (pm_multiply_submit): a and b may be
+        reused once this returns."""
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        if a.shape != b.shape or a.ndim != 2 or a.shape[1] != LIMBS:
+            raise ValueError("a and b must both be [n, 28] uint32")
+        self._check("pm_multiply_submit", self._lib.pm_multiply_submit(self._h, a.shape[0], a.ctypes.data,
+                                                                        b.ctypes.data))
+        self._pending_n = a.shape[0]
+
+    def collect(self, out=None):
+        """Second half (pm_multiply_collect): the product of the last submit."""
+        n = getattr(self, "_pending_n", None)
+        if n is None:
+            raise PolymulError("pm_multiply_collect", 1, "no submitted product to collect")
+        c = np.empty((n, LIMBS), dtype=np.uint32) if out is None else out
+        if c.shape != (n, LIMBS) or c.dtype != np.uint32 or not c.flags.c_contiguous:
+            raise ValueError("out must be a contiguous [n, 28] uint32 array")
+        self._pending_n = None
+        self._check("pm_multiply_collect", self._lib.pm_multiply_collect(self._h, c.ctypes.data))
+        return c
+
+    def set_io_chunks(self, n, chunks):
+        """Chunks of the host-buffer path for products of size n (0 = default)."""
+        self._check("pm_set_io_chunks", self._lib.pm_set_io_chunks(self._h, int(n), int(chunks)))
 
     def multiply_device(self, n, a, b, c, stream=None):
         self._check("pm_multiply_device",

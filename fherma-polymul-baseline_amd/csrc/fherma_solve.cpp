@@ -42,10 +42,14 @@ fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
     if (words == 0 || words % PM_LIMBS != 0 || in.b.data.size() != words)
         throw std::runtime_error("fherma_run: a and b must both hold N x 28 words");
     const uint32_t n = uint32_t(words / PM_LIMBS);
+    int status = pm_multiply_submit(ctx, n, in.a.data.data(), in.b.data.data());
+    if (status != PM_OK) raise("fherma_run", ctx, status);
+    // the fresh output vector is allocated (and zero-filled) while the
+    // device computes
     fherma::Outputs out;
     out.c.shape = {int64_t(n), int64_t(PM_LIMBS)};
     out.c.data.resize(words);
-    const int status = pm_multiply(ctx, n, in.a.data.data(), in.b.data.data(), out.c.data.data());
+    status = pm_multiply_collect(ctx, out.c.data.data());
     if (status != PM_OK) raise("fherma_run", ctx, status);
     return out;
 }

@@ -9,8 +9,15 @@
 
 namespace pmd {
 
+// words 32-bit words from src to dst (16-byte aligned; either may be mapped
+// page-locked host memory)
+hipError_t launch_copy(const uint32_t* src, uint32_t* dst, size_t words, hipStream_t stream);
 // residues of a and b for all primes (MFMA conversion)
 hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream);
+// the same for coefficients [c_first, c_end) of polynomials poly_first ..
+// poly_first + polys - 1 (0 = a, 1 = b)
+hipError_t launch_convert_range(const Tables& tb, const Job& job, uint32_t c_first, uint32_t c_end,
+                                uint32_t poly_first, uint32_t polys, hipStream_t stream);
 // residues by scalar limb reduction (comparison path, same output contract)
 hipError_t launch_convert_scalar(const Tables& tb, const Job& job, hipStream_t stream);
 // forward NTTs, pointwise product, inverse NTT, scale; u outer radix-2 stages
@@ -18,6 +25,10 @@ hipError_t launch_convert_scalar(const Tables& tb, const Job& job, hipStream_t s
 hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream);
 // CRT and reduction modulo q into 28-limb coefficients (int8 MFMA sums)
 hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream);
+// the same for coefficients [c_first, c_end): c_first a multiple of 64, c_end a
+// multiple of 64 or n (c may be mapped host memory)
+hipError_t launch_crt_range(const Tables& tb, const Job& job, uint32_t c_first, uint32_t c_end,
+                            hipStream_t stream);
 // the same by 64-bit multiply-accumulate on the vector ALUs (comparison path)
 hipError_t launch_crt_scalar(const Tables& tb, const Job& job, hipStream_t stream);
 // reset counters, then one persistent launch over the batch work list

@@ -297,16 +297,19 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
 // coefficient c0 + 16*tile + l%16; the weight operand of quad Q holds rows
 // (prime 4Q + i/4, digit i%4).  Output lane l gets the 4 digit sums of
 // (prime 4Q + l/16, coefficient c0 + 16*tile + l%16).
+// Coefficients at or past c_end are neither read nor stored; n is the row
+// stride of the residue array.
 __device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* __restrict__ coef, uint32_t n,
-                                             uint32_t c0, uint32_t* __restrict__ res,
-                                             const uint4* __restrict__ lds_w, uint32_t lane) {
+                                             uint32_t c_end, uint32_t c0, uint32_t* __restrict__ res,
+                                             const uint4* __restrict__ lds_w, uint32_t lane,
+                                             uint32_t quad_begin = 0, uint32_t quad_end = 0xFFFFFFFFu) {
     const uint32_t col = lane & 15u, kg = lane >> 4;
     v4i b0[4], b1[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
         const uint32_t c = c0 + 16u * ct + col;
         uint4 v0 = make_uint4(0, 0, 0, 0), v1 = make_uint4(0, 0, 0, 0);
-        if (c < n) {
+        if (c < c_end) {
             const uint4* src = reinterpret_cast<const uint4*>(coef + size_t(c) * kLimbs);
             v0 = src[kg];
             if (kg < 3) v1 = src[4 + kg];
@@ -318,7 +321,8 @@ __device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* _
                               int(v1.w ^ 0x80808080u)}
                         : v4i{0, 0, 0, 0};
     }
-    for (uint32_t quad = 0; quad < tb.quads; ++quad) {
+    quad_end = quad_end < tb.quads ? quad_end : tb.quads;
+    for (uint32_t quad = quad_begin; quad < quad_end; ++quad) {
         const uint4 w0u = lds_w[(quad * 2 + 0) * 64 + lane];
         const uint4 w1u = lds_w[(quad * 2 + 1) * 64 + lane];
         const v4i w0 = v4i{int(w0u.x), int(w0u.y), int(w0u.z), int(w0u.w)};
@@ -343,7 +347,7 @@ __device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* _
             uint32_t x = csub(shoup(uint32_t(w) + p2, c16, p), p);   // w * 2^16 mod p, [0,p)
             uint32_t r = uint32_t(u) + p2 + x;                          // < 2^32, > 0
             r = csub(csub(r, p2), p2);                                  // [0,2p)
-            if (live && c < n) res[size_t(prime) * n + c] = r;
+            if (live && c < c_end) res[size_t(prime) * n + c] = r;
         }
     }
 }
@@ -610,16 +614,23 @@ constexpr size_t crt_block_lds() {
 // ---------------------------------------------------------------------------
 // standalone kernels (plain path)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_convert(Tables tb, Job job) {
+// Coefficients [c_first, c_end) of a (poly 0) and/or b (1): blockIdx.y is
+// poly - poly_first; blockIdx.z picks one of gridDim.z slices of the primes,
+// so a single product still spreads over many waves.
+__global__ void __launch_bounds__(kBlock) k_convert(Tables tb, Job job, uint32_t c_first, uint32_t c_end,
+                                                    uint32_t poly_first) {
     __shared__ uint4 lds_w[16 * 2 * 64];
-    for (uint32_t i = threadIdx.x; i < tb.quads * 128u; i += blockDim.x) lds_w[i] = tb.conv_w[i];
+    const uint32_t per = (tb.quads + gridDim.z - 1) / gridDim.z;
+    const uint32_t quad_begin = blockIdx.z * per, quad_end = quad_begin + per;
+    for (uint32_t i = threadIdx.x + quad_begin * 128u; i < tb.quads * 128u && i < quad_end * 128u; i += blockDim.x)
+        lds_w[i] = tb.conv_w[i];
     __syncthreads();
     const uint32_t n = 1u << job.logn;
-    const uint32_t poly = blockIdx.y;
+    const uint32_t poly = poly_first + blockIdx.y;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t c0 = blockIdx.x * kConvPerBlock + wave * kConvPerWave;
-    if (c0 >= n) return;
-    convert_wave(tb, poly ? job.b : job.a, n, c0, poly ? job.rb : job.ra, lds_w, lane);
+    const uint32_t c0 = c_first + blockIdx.x * blockDim.x + wave * kConvPerWave;  // one coefficient per thread
+    if (c0 >= c_end) return;
+    convert_wave(tb, poly ? job.b : job.a, n, c_end, c0, poly ? job.rb : job.ra, lds_w, lane, quad_begin, quad_end);
 }
 
 __global__ void __launch_bounds__(256) k_convert_scalar(Tables tb, Job job) {
@@ -796,7 +807,7 @@ __device__ __noinline__ void handle_conv(const Tables& tb, const Job& job, const
     }
     const uint32_t n = 1u << job.logn;
     const uint32_t c0 = it.a1 * kConvPerBlock + (tid >> 6) * kConvPerWave;
-    if (c0 < n) convert_wave(tb, it.a0 ? job.b : job.a, n, c0, it.a0 ? job.rb : job.ra, lds_w, tid & 63u);
+    if (c0 < n) convert_wave(tb, it.a0 ? job.b : job.a, n, n, c0, it.a0 ? job.rb : job.ra, lds_w, tid & 63u);
 }
 
 __device__ __noinline__ void handle_outer(const Tables& tb, const Job& job, const Item& it) {
@@ -955,7 +966,7 @@ __global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, 
     const uint32_t n = 1u << job.logn;
     const uint32_t c0 = it.a1 * kConvPerBlock + (threadIdx.x >> 6) * kConvPerWave;
     if (c0 >= n) return;
-    convert_wave(tb, it.a0 ? job.b : job.a, n, c0, it.a0 ? job.rb : job.ra, lds_w, threadIdx.x & 63u);
+    convert_wave(tb, it.a0 ? job.b : job.a, n, n, c0, it.a0 ? job.rb : job.ra, lds_w, threadIdx.x & 63u);
 }
 
 __global__ void __launch_bounds__(kBlock) k_items_conv_scalar(Tables tb, BatchArgs ba, uint32_t first) {
@@ -1036,14 +1047,18 @@ __device__ __forceinline__ void crt_group_loop(const Tables& tb, uint32_t groups
     }
 }
 
-__global__ void __launch_bounds__(kCrtMfmaWaves * 64) k_crt_mfma(Tables tb, Job job) {
+// coefficients [c_first, c_end) with c_first a multiple of 64 and c_end a
+// multiple of 64 or n; job.c may be device memory or mapped page-locked host
+// memory
+__global__ void __launch_bounds__(kCrtMfmaWaves * 64) k_crt_mfma(Tables tb, Job job, uint32_t c_first,
+                                                                 uint32_t c_end) {
     const uint32_t n = 1u << job.logn;
-    crt_group_loop<kCrtMfmaWaves>(tb, (n + kCrtWaveCoefs - 1) / kCrtWaveCoefs,
+    crt_group_loop<kCrtMfmaWaves>(tb, (c_end - c_first + kCrtWaveCoefs - 1) / kCrtWaveCoefs,
                                   [&](uint32_t gi, const uint32_t*& t, uint32_t*& out, uint32_t& nn, uint32_t& c0) {
                                       t = job.ra;
                                       out = job.c;
                                       nn = n;
-                                      c0 = gi * kCrtWaveCoefs;
+                                      c0 = c_first + gi * kCrtWaveCoefs;
                                   });
 }
 
@@ -1080,6 +1095,9 @@ static hipError_t launch_tile(const Tables& tb, const Job& job, uint32_t u, hipS
     const uint32_t units = tb.k << u;
     uint32_t tiles = kBlock >> G::TPTLOG;
     if (tiles > units) tiles = units;
+    // fewer tiles per block while that leaves CUs idle: one product has only
+    // k << u tiles
+    while (tiles > 1 && (units + tiles - 1) / tiles < tb.cus && ((tiles >> 1) << G::TPTLOG) >= 64) tiles >>= 1;
     uint32_t threads = tiles << G::TPTLOG;
     if (threads < 64) {  // whole waves
         tiles = (64u >> G::TPTLOG) ? (64u >> G::TPTLOG) : 1u;
@@ -1113,13 +1131,49 @@ static hipError_t launch_tile_dispatch(uint32_t tlog, const Tables& tb, const Jo
     }
 }
 
-hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream) {
-    const uint32_t n = 1u << job.logn;
-    const uint32_t blocks = (n + kConvPerBlock - 1) / kConvPerBlock;
-    uint32_t threads = kBlock;
-    if (n < kConvPerBlock) threads = ((n + kConvPerWave - 1) / kConvPerWave) * kWaveSize;
-    hipLaunchKernelGGL(k_convert, dim3(blocks, 2), dim3(threads), 0, stream, tb, job);
+// words / 4 16-byte pieces, then the remaining words; either side may be
+// page-locked host memory (the loads and stores then cross PCIe)
+__global__ void __launch_bounds__(256) k_copy(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                              size_t words) {
+    const size_t vecs = words / 4, stride = size_t(gridDim.x) * blockDim.x;
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    const size_t tid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t i = tid; i < vecs; i += stride) d4[i] = s4[i];
+    for (size_t i = vecs * 4 + tid; i < words; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const uint32_t* src, uint32_t* dst, size_t words, hipStream_t stream) {
+    if (!words) return hipSuccess;
+    if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) return hipErrorInvalidValue;
+    // ~128 blocks keep a PCIe transfer at its rate; fewer for small pieces
+    size_t blocks = (words / 4 + 1023) / 1024;
+    blocks = blocks < 1 ? 1 : (blocks > 128 ? 128 : blocks);
+    hipLaunchKernelGGL(k_copy, dim3(uint32_t(blocks)), dim3(256), 0, stream, src, dst, words);
     return hipGetLastError();
+}
+
+hipError_t launch_convert_range(const Tables& tb, const Job& job, uint32_t c_first, uint32_t c_end,
+                                uint32_t poly_first, uint32_t polys, hipStream_t stream) {
+    const uint32_t n = 1u << job.logn;
+    if (c_first >= c_end || c_end > n || polys < 1 || poly_first + polys > 2) return hipErrorInvalidValue;
+    const uint32_t count = c_end - c_first;
+    // 4-wave blocks, and the primes split over up to 4 slices while that
+    // leaves fewer than ~4 waves per CU: a single product (or one chunk of it)
+    // then still spreads over the whole device
+    uint32_t threads = kConvSmallBlock;
+    if (count < threads) threads = ((count + kConvPerWave - 1) / kConvPerWave) * kWaveSize;
+    const uint32_t blocks = (count + threads - 1) / threads;
+    const uint32_t waves = blocks * polys * (threads / kWaveSize);
+    uint32_t slices = 1;
+    while (slices < 4 && slices < tb.quads && waves * slices < 4 * tb.cus) slices *= 2;
+    hipLaunchKernelGGL(k_convert, dim3(blocks, polys, slices), dim3(threads), 0, stream, tb, job, c_first, c_end,
+                       poly_first);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream) {
+    return launch_convert_range(tb, job, 0, 1u << job.logn, 0, 2, stream);
 }
 
 hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream) {
@@ -1242,14 +1296,21 @@ hipError_t launch_crt_scalar(const Tables& tb, const Job& job, hipStream_t strea
     return hipGetLastError();
 }
 
-hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
+hipError_t launch_crt_range(const Tables& tb, const Job& job, uint32_t c_first, uint32_t c_end,
+                            hipStream_t stream) {
     const uint32_t n = 1u << job.logn;
-    const uint32_t groups = (n + kCrtWaveCoefs - 1) / kCrtWaveCoefs;
+    if (c_first >= c_end || c_end > n || c_first % kCrtWaveCoefs || (c_end != n && c_end % kCrtWaveCoefs))
+        return hipErrorInvalidValue;
+    const uint32_t groups = (c_end - c_first + kCrtWaveCoefs - 1) / kCrtWaveCoefs;
     uint32_t grid = (groups + kCrtMfmaWaves - 1) / kCrtMfmaWaves;
     grid = grid < tb.cus ? grid : tb.cus;
     hipLaunchKernelGGL(k_crt_mfma, dim3(grid), dim3(kCrtMfmaWaves * 64), crt_block_lds<kCrtMfmaWaves>(), stream, tb,
-                       job);
+                       job, c_first, c_end);
     return hipGetLastError();
+}
+
+hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
+    return launch_crt_range(tb, job, 0, 1u << job.logn, stream);
 }
 
 }  // namespace pmd

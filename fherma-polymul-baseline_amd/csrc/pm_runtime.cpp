@@ -1,6 +1,7 @@
 // Context, resident tables and the plain product path behind include/polymul.h.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -60,7 +61,22 @@ static const T* upload(pm_context* ctx, const std::vector<T>& v, int* status) {
     return static_cast<const T*>(p);
 }
 
-uint32_t pm_default_outer(uint32_t logn) { return logn > 15 ? logn - 15 : 0; }
+// outer passes of the plain path: a single product has only k tiles of 2^15,
+// so from N = 2^14 on it is split into 2-4 smaller tiles per prime (more
+// blocks) at the cost of the outer passes' HBM traffic
+uint32_t pm_default_outer(uint32_t logn) { return logn <= 13 ? 0u : (logn == 14 ? 1u : 2u); }
+
+// input chunks of the host-buffer path (the output goes in half as many):
+// the set value, else pieces of at most 2^13 coefficients (~0.9 MB), 2 to 16
+// of them
+uint32_t pm_io_chunks(const pm_context* ctx, uint32_t logn) {
+    if (ctx->io_chunks[logn]) return ctx->io_chunks[logn];
+    uint32_t chunks = 2;
+    while (chunks < pm_context::kMaxIoChunks && (2u << logn) / chunks > 8192u) chunks *= 2;
+    return chunks;
+}
+
+constexpr int kCollectTimeoutSeconds = 60;
 
 extern "C" {
 
@@ -154,8 +170,8 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     // plain-path workspace for N up to max_n
     const size_t coef_bytes = size_t(max_n) * kLimbs * sizeof(uint32_t);
     const size_t res_bytes = size_t(k) * max_n * sizeof(uint32_t);
-    ctx->d_a = static_cast<uint32_t*>(ctx->device_alloc(coef_bytes));
-    ctx->d_b = static_cast<uint32_t*>(ctx->device_alloc(coef_bytes));
+    ctx->d_a = static_cast<uint32_t*>(ctx->device_alloc(2 * coef_bytes));  // a then b: one transfer target
+    ctx->d_b = ctx->d_a ? ctx->d_a + size_t(max_n) * kLimbs : nullptr;
     ctx->d_c = static_cast<uint32_t*>(ctx->device_alloc(coef_bytes));
     ctx->d_ra = static_cast<uint32_t*>(ctx->device_alloc(res_bytes));
     ctx->d_rb = static_cast<uint32_t*>(ctx->device_alloc(res_bytes));
@@ -168,6 +184,22 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
         pm_destroy(ctx);
         return create_failed(PM_EDEVICE, "stream creation failed");
     }
+    // page-locked staging of the host-buffer path, mapped into the device
+    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->h_in), 2 * coef_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_out), coef_bytes, hipHostMallocDefault) != hipSuccess) {
+        pm_destroy(ctx);
+        return create_failed(PM_ENOMEM, "page-locked staging allocation failed");
+    }
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->hd_in), ctx->h_in, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->hd_out), ctx->h_out, 0) != hipSuccess) {
+        pm_destroy(ctx);
+        return create_failed(PM_EDEVICE, "mapping page-locked staging into the device failed");
+    }
+    for (hipEvent_t& ev : ctx->io_done)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            pm_destroy(ctx);
+            return create_failed(PM_EDEVICE, "event creation failed");
+        }
     for (uint32_t logn = 0; logn <= pm::kMaxLogN; ++logn) ctx->outer[logn] = pm_default_outer(logn);
     *out = ctx;
     return PM_OK;
@@ -198,7 +230,12 @@ void pm_destroy(pm_context* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
+
     for (void* p : ctx->allocations) (void)hipFree(p);
+    if (ctx->h_in) (void)hipHostFree(ctx->h_in);
+    if (ctx->h_out) (void)hipHostFree(ctx->h_out);
+    for (hipEvent_t ev : ctx->io_done)
+        if (ev) (void)hipEventDestroy(ev);
     delete ctx;
 }
 
@@ -258,24 +295,94 @@ int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const u
 
 int pm_multiply(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t* c) {
     if (!ctx || !a || !b || !c) return PM_EINVAL;
+    if (c == a || c == b) return pm_fail(ctx, PM_EINVAL, "c may not alias a or b");
+    const int st = pm_multiply_submit(ctx, n, a, b);
+    if (st != PM_OK) return st;
+    return pm_multiply_collect(ctx, c);
+}
+
+// The host-buffer path.  The inputs a|b, as one array of 2n coefficients,
+// go to the device in `chunks` equal pieces: piece j is copied into
+// page-locked staging and a copy kernel that pulls it over PCIe is launched
+// before piece j+1 is copied, so host copies and transfers overlap.  The
+// product then runs from device memory, and the result goes back in
+// chunks/2 pieces (copy kernels writing into page-locked staging, one event
+// each) that collect copies out one by one as they land.  Copy kernels rather
+// than hipMemcpyAsync: they reach the PCIe rate at a few hundred KB and cost
+// no more than a kernel launch each.
+int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b) {
+    if (!ctx || !a || !b) return PM_EINVAL;
+    if (ctx->pending) return pm_fail(ctx, PM_EINVAL, "a submitted product has not been collected");
     uint32_t logn = 0;
     int st = pm_check_n(ctx, n, &logn);
     if (st != PM_OK) return st;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return pm_hip(ctx, e, "set device");
-    const size_t bytes = size_t(n) * kLimbs * sizeof(uint32_t);
-    e = hipMemcpyAsync(ctx->d_a, a, bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_b, b, bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return pm_hip(ctx, e, "copy inputs to device");
-    pmd::Job job{logn, 0, ctx->d_a, ctx->d_b, ctx->d_c, ctx->d_ra, ctx->d_rb};
-    st = pm_run_plain(ctx, job, ctx->outer[logn], ctx->stream);
-    if (st != PM_OK) {
-        (void)hipStreamSynchronize(ctx->stream);
-        return st;
+    const uint32_t chunks = pm_io_chunks(ctx, logn);
+    const uint32_t in_chunks = chunks < 2 ? 2 : chunks;  // a and b are separate arrays
+    const uint32_t per_poly = in_chunks / 2;
+    const size_t words = size_t(n) * kLimbs, span_words = words / per_poly;
+    uint32_t* d_in = ctx->d_a;  // room for 2 * max_n coefficients: b at d_in + n * 28
+    for (uint32_t j = 0; j < in_chunks && e == hipSuccess; ++j) {
+        const uint32_t poly = j / per_poly, part = j % per_poly;
+        const size_t off = poly * words + part * span_words;
+        std::memcpy(ctx->h_in + off, (poly ? b : a) + part * span_words, span_words * sizeof(uint32_t));
+        e = pmd::launch_copy(ctx->hd_in + off, d_in + off, span_words, ctx->stream);
     }
-    e = hipMemcpyAsync(c, ctx->d_c, bytes, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    return pm_hip(ctx, e, "product");
+    pmd::Job job{logn, 0, d_in, d_in + words, ctx->d_c, ctx->d_ra, ctx->d_rb};
+    if (e == hipSuccess) st = pm_run_plain(ctx, job, ctx->outer[logn], ctx->stream);
+    const uint32_t out_chunks = chunks < 2 ? 1 : chunks / 2;
+    const size_t out_words = words / out_chunks;
+    for (uint32_t j = 0; j < out_chunks && e == hipSuccess && st == PM_OK; ++j) {
+        e = pmd::launch_copy(ctx->d_c + j * out_words, ctx->hd_out + j * out_words, out_words, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->io_done[j], ctx->stream);
+    }
+    if (e != hipSuccess || st != PM_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return st != PM_OK ? st : pm_hip(ctx, e, "launch product");
+    }
+    ctx->pending = true;
+    ctx->pending_n = n;
+    ctx->pending_out_chunks = out_chunks;
+    return PM_OK;
+}
+
+int pm_multiply_collect(pm_context* ctx, uint32_t* c) {
+    if (!ctx || !c) return PM_EINVAL;
+    if (!ctx->pending) return pm_fail(ctx, PM_EINVAL, "no submitted product to collect");
+    ctx->pending = false;
+    const uint32_t chunks = ctx->pending_out_chunks;
+    const size_t per_words = size_t(ctx->pending_n / chunks) * kLimbs;
+    const auto start = std::chrono::steady_clock::now();
+    for (uint32_t j = 0; j < chunks; ++j) {
+        // poll rather than block: the chunk is usually done within tens of us
+        for (;;) {
+            const hipError_t q = hipEventQuery(ctx->io_done[j]);
+            if (q == hipSuccess) break;
+            if (q != hipErrorNotReady) {
+                (void)hipStreamSynchronize(ctx->stream);
+                return pm_hip(ctx, q, "product");
+            }
+            if (std::chrono::steady_clock::now() - start > std::chrono::seconds(kCollectTimeoutSeconds)) {
+                (void)hipStreamSynchronize(ctx->stream);
+                return pm_fail(ctx, PM_ETIMEOUT, "product did not complete within the time limit");
+            }
+        }
+        std::memcpy(c + j * per_words, ctx->h_out + j * per_words, per_words * sizeof(uint32_t));
+    }
+    return PM_OK;
+}
+
+int pm_set_io_chunks(pm_context* ctx, uint32_t n, uint32_t chunks) {
+    if (!ctx) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    if (chunks != 0 && (chunks > pm_context::kMaxIoChunks || (chunks & (chunks - 1)) || n / chunks < 64 ||
+                        n % chunks))
+        return pm_fail(ctx, PM_EINVAL, "chunks must be 0 or a power of two <= 16 leaving >= 64 coefficients each");
+    ctx->io_chunks[logn] = chunks;
+    return PM_OK;
 }
 
 int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,

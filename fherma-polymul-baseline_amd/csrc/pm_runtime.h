@@ -21,6 +21,18 @@ struct pm_context {
     uint32_t *d_a = nullptr, *d_b = nullptr, *d_c = nullptr, *d_ra = nullptr, *d_rb = nullptr;
     // outer radix-2 stages run as separate passes, per log2 N (plain path)
     uint32_t outer[pm::kMaxLogN + 1] = {};
+    // host-buffer path: page-locked staging (inputs a|b and the output) with
+    // its device views, output chunk events, and the product between submit
+    // and collect
+    static constexpr uint32_t kMaxIoChunks = 16;
+    uint32_t* h_in = nullptr;    // [2][max_n][28]
+    uint32_t* h_out = nullptr;   // [max_n][28]
+    uint32_t* hd_in = nullptr;   // device views of the same memory
+    uint32_t* hd_out = nullptr;
+    hipEvent_t io_done[kMaxIoChunks] = {};
+    uint32_t io_chunks[pm::kMaxLogN + 1] = {};  // chunks per log2 N (0: default)
+    bool pending = false;
+    uint32_t pending_n = 0, pending_out_chunks = 0;
     std::string last_error;
 
     void* device_alloc(size_t bytes);
@@ -31,3 +43,4 @@ int pm_hip(pm_context* ctx, hipError_t e, const char* what);
 int pm_check_n(pm_context* ctx, uint32_t n, uint32_t* logn);
 int pm_run_plain(pm_context* ctx, const pmd::Job& job, uint32_t outer, hipStream_t stream);
 uint32_t pm_default_outer(uint32_t logn);
+uint32_t pm_io_chunks(const pm_context* ctx, uint32_t logn);

@@ -63,8 +63,26 @@ int pm_get_info(const pm_context* ctx, pm_info_t* info);
 int pm_get_prime(const pm_context* ctx, uint32_t i, uint32_t* p, uint32_t* psi_max);
 
 /* Replaces the timed body of fherma_run (rns/solve.cu:721-853): host buffers
- * a, b, c of n*28 uint32 each (c may not alias a or b). */
+ * a, b, c of n*28 uint32 each (c may not alias a or b).  Equal to
+ * pm_multiply_submit followed by pm_multiply_collect. */
 int pm_multiply(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t* c);
+/* The two halves of pm_multiply, so a caller can do host work (such as
+ * allocating the output) while the device computes -- the role of the
+ * reference's chunked input pipeline and output completion
+ * (rns/solve.cu:492-503,721-853; input_pipeline.h:12-54; output_completion.h:63-91).
+ * submit copies a and b chunk by chunk into page-locked staging, each chunk's
+ * transfer starting while the next is copied, enqueues the product and
+ * returns once a and b are no longer needed; collect waits for the result's
+ * chunks and copies each into c as it lands.  One product may be pending per
+ * context; PM_EINVAL otherwise. */
+int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b);
+int pm_multiply_collect(pm_context* ctx, uint32_t* c);
+/* Transfer chunks of the host-buffer path for products of size n (no
+ * counterpart in the reference): the inputs a|b go in `chunks` pieces and
+ * the result in chunks/2.  0 restores the default (pieces of at most 2^13
+ * coefficients, 2..16 of them); otherwise a power of two <= 16 with
+ * n / chunks >= 64. */
+int pm_set_io_chunks(pm_context* ctx, uint32_t n, uint32_t chunks);
 /* Same product with device-resident a, b, c on a HIP stream (0 = the
  * context's stream); returns once the work is enqueued. */
 int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
