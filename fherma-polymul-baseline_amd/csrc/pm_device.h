@@ -93,4 +93,23 @@ struct BatchArgs {
 
 constexpr uint32_t kErrTimeout = 2;
 
+// Host-buffer path: flags in page-locked host memory (fine-grained) that the
+// host sets as input chunks land in staging (ready) and the device sets as
+// result chunks land in staging (done); counters (device memory) count the
+// finished pieces of each result chunk.
+constexpr uint32_t kMaxIoChunks = 16;
+constexpr uint32_t kMirrorStride = 64;  // words between the device copies of the ready flags
+struct IoArgs {
+    const uint32_t* ready;      // [kMaxIoChunks] host-written
+    uint32_t* done;             // [kMaxIoChunks] device-written
+    uint32_t* error;            // host-mapped: kErrTimeout when a wait gave up
+    uint32_t* counters;         // [kMaxIoChunks] device memory
+    uint32_t* mirror;           // [kMaxIoChunks + 1][kMirrorStride] device memory: ready flags as seen, then an error word
+    uint32_t seq;               // this product's sequence number (never 0)
+    uint32_t chunks;            // input chunks
+    uint32_t groups_per_chunk;  // ingest: 64-coefficient groups per input chunk
+    uint32_t pad;
+    uint64_t wait_ticks;        // bound of one wait (wall-clock ticks)
+};
+
 }  // namespace pmd

@@ -12,6 +12,13 @@ namespace pmd {
 // words 32-bit words from src to dst (16-byte aligned; either may be mapped
 // page-locked host memory)
 hipError_t launch_copy(const uint32_t* src, uint32_t* dst, size_t words, hipStream_t stream);
+// host-buffer path: conversion of a|b from page-locked host memory, group by
+// group as the host flags input chunks ready (also clears io.counters)
+hipError_t launch_ingest_convert(const Tables& tb, const Job& job, const IoArgs& io, hipStream_t stream);
+// device src -> page-locked host dst in `chunks` chunks, io.done[j] = io.seq
+// once chunk j is visible to the host
+hipError_t launch_egress(const uint32_t* src, uint32_t* dst_host, size_t chunk_words, uint32_t chunks,
+                         const IoArgs& io, hipStream_t stream);
 // residues of a and b for all primes (MFMA conversion)
 hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream);
 // the same for coefficients [c_first, c_end) of polynomials poly_first ..

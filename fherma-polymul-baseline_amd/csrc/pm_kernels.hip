@@ -1143,6 +1143,140 @@ __global__ void __launch_bounds__(256) k_copy(const uint32_t* __restrict__ src, 
     for (size_t i = vecs * 4 + tid; i < words; i += stride) dst[i] = src[i];
 }
 
+// ---------------------------------------------------------------------------
+// host-buffer path: conversion straight from page-locked host memory as the
+// host fills it, and the result's return in chunks with ready flags
+// ---------------------------------------------------------------------------
+// Block 0 is a watcher: one lane polls the host-written ready flags in chunk
+// order (a handful of PCIe reads per microsecond) and republishes each in
+// device memory (io.mirror, one flag per 256 bytes so they sit in different
+// L2 channels), where one lane per converting block polls it with growing
+// sleeps -- thousands of waves polling one host word, or one L2 line, in a
+// tight loop would crowd out the transfer itself.  The watcher publishes in
+// chunk order, so a block whose waves straddle two chunks waits for the later.
+// Every wait is bounded; a timeout latches kErrTimeout in the host-visible
+// io.error and this product's seq in the device error word, and every later
+// wait gives up at once.  Wave w of blocks 1.. converts group w of the
+// concatenated a|b array (64 coefficients) once its chunk is published;
+// job.a / job.b point at page-locked host memory.
+__global__ void __launch_bounds__(kConvSmallBlock) k_ingest_convert(Tables tb, Job job, IoArgs io) {
+    uint32_t* dev_error = io.mirror + kMaxIoChunks * kMirrorStride;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < kMaxIoChunks) io.counters[threadIdx.x] = 0u;  // egress chunk counters
+        if (threadIdx.x == 0) {
+            for (uint32_t j = 0; j < io.chunks; ++j) {
+                const uint64_t start = wall_clock64();
+                bool ok = true;
+                // relaxed polls (an acquire per poll would invalidate caches
+                // each time), one acquire once the flag is up
+                while (__hip_atomic_load(io.ready + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != io.seq) {
+                    if (wall_clock64() - start > io.wait_ticks) {
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                if (ok) (void)__hip_atomic_load(io.ready + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (!ok) {
+                    __hip_atomic_store(io.error, kErrTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __hip_atomic_store(dev_error, io.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __hip_atomic_store(io.mirror + j * kMirrorStride, io.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        return;
+    }
+    __shared__ uint4 lds_w[16 * 2 * 64];
+    __shared__ uint32_t s_ok;
+    for (uint32_t i = threadIdx.x; i < tb.quads * 128u; i += blockDim.x) lds_w[i] = tb.conv_w[i];
+    const uint32_t n = 1u << job.logn, per_poly = (n + kConvPerWave - 1) / kConvPerWave;
+    const uint32_t groups = 2u * per_poly, waves = blockDim.x >> 6;
+    const uint32_t g_first = (blockIdx.x - 1) * waves;
+    const uint32_t g_last = min(g_first + waves, groups) - 1u;  // launch: g_first < groups
+    const uint32_t* flag = io.mirror + (g_last / io.groups_per_chunk) * kMirrorStride;
+    if (threadIdx.x == 0) {
+        // the watcher's own waits are bounded by wait_ticks per chunk
+        const uint64_t start = wall_clock64(), limit = io.wait_ticks * (io.chunks + 1);
+        uint32_t ok = 1, nap = 1;
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != io.seq) {
+            if (__hip_atomic_load(dev_error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == io.seq ||
+                wall_clock64() - start > limit) {
+                ok = 0;
+                break;
+            }
+            // s_sleep takes an immediate: 64 x nap clocks, nap = 1, 2, 4 .. 32
+            switch (nap) {
+                case 1: __builtin_amdgcn_s_sleep(1); break;
+                case 2: __builtin_amdgcn_s_sleep(2); break;
+                case 4: __builtin_amdgcn_s_sleep(4); break;
+                case 8: __builtin_amdgcn_s_sleep(8); break;
+                case 16: __builtin_amdgcn_s_sleep(16); break;
+                default: __builtin_amdgcn_s_sleep(32); break;
+            }
+            nap = nap < 32 ? 2 * nap : 32;
+        }
+        if (ok) (void)__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = ok;
+    }
+    __syncthreads();  // the block's loads come after thread 0's acquire
+    const uint32_t g = __builtin_amdgcn_readfirstlane(g_first + (threadIdx.x >> 6));
+    if (!s_ok || g >= groups) return;
+    const uint32_t poly = g / per_poly, c0 = (g % per_poly) * kConvPerWave;
+    convert_wave(tb, poly ? job.b : job.a, n, n, c0, poly ? job.rb : job.ra, lds_w, threadIdx.x & 63u);
+}
+
+// Copies chunk `chunk` (chunk_words words, a multiple of 4) of src (device)
+// to dst (page-locked host); the last block to finish sets done[chunk] = seq
+// once every store of the chunk is visible to the host.  One launch per chunk,
+// so chunks land in order.
+__global__ void __launch_bounds__(256) k_egress(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                size_t chunk_words, IoArgs io, uint32_t chunk) {
+    const size_t first = size_t(chunk) * chunk_words / 4, count = chunk_words / 4;
+    const uint4* s4 = reinterpret_cast<const uint4*>(src) + first;
+    uint4* d4 = reinterpret_cast<uint4*>(dst) + first;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+        d4[i] = s4[i];
+    __threadfence_system();  // this wave's stores reach the host before the count
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t before = __hip_atomic_fetch_add(io.counters + chunk, 1u, __ATOMIC_ACQ_REL,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+        if (before + 1u == gridDim.x) {
+            __threadfence_system();
+            __hip_atomic_store(io.done + chunk, io.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
+
+hipError_t launch_ingest_convert(const Tables& tb, const Job& job, const IoArgs& io, hipStream_t stream) {
+    const uint32_t n = 1u << job.logn;
+    const uint32_t groups = 2u * ((n + kConvPerWave - 1) / kConvPerWave);
+    const uint32_t waves_per_block = kConvSmallBlock / kWaveSize;
+    const uint32_t blocks = 1 + (groups + waves_per_block - 1) / waves_per_block;  // + the watcher
+    if (!io.groups_per_chunk || io.chunks > kMaxIoChunks ||
+        (groups + io.groups_per_chunk - 1) / io.groups_per_chunk != io.chunks)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ingest_convert, dim3(blocks), dim3(kConvSmallBlock), 0, stream, tb, job, io);
+    return hipGetLastError();
+}
+
+hipError_t launch_egress(const uint32_t* src, uint32_t* dst_host, size_t chunk_words, uint32_t chunks,
+                         const IoArgs& io, hipStream_t stream) {
+    if (!chunks || chunks > kMaxIoChunks || chunk_words % 4 ||
+        ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst_host)) & 15u))
+        return hipErrorInvalidValue;
+    // ~64 blocks keep a PCIe transfer at its rate; fewer for small chunks
+    size_t blocks = (chunk_words / 4 + 1023) / 1024;
+    blocks = blocks < 1 ? 1 : (blocks > 64 ? 64 : blocks);
+    for (uint32_t j = 0; j < chunks; ++j) {
+        hipLaunchKernelGGL(k_egress, dim3(uint32_t(blocks)), dim3(256), 0, stream, src, dst_host, chunk_words, io, j);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_copy(const uint32_t* src, uint32_t* dst, size_t words, hipStream_t stream) {
     if (!words) return hipSuccess;
     if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) return hipErrorInvalidValue;

@@ -1,6 +1,8 @@
 // Context, resident tables and the plain product path behind include/polymul.h.
 #include <hip/hip_runtime.h>
 
+#include <immintrin.h>
+
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -185,21 +187,38 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
         return create_failed(PM_EDEVICE, "stream creation failed");
     }
     // page-locked staging of the host-buffer path, mapped into the device
-    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->h_in), 2 * coef_bytes, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_out), coef_bytes, hipHostMallocDefault) != hipSuccess) {
+    // (the device reads the inputs and writes the result there while the host
+    // copies other chunks), and a fine-grained (coherent) page of flags
+    // through which each side tells the other that a chunk has landed
+    const unsigned mapped = hipHostMallocMapped;
+    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->h_in), 2 * coef_bytes, mapped) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_out), coef_bytes, mapped) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_flags), 4096, mapped | hipHostMallocCoherent) != hipSuccess) {
         pm_destroy(ctx);
         return create_failed(PM_ENOMEM, "page-locked staging allocation failed");
     }
+    std::memset(ctx->h_flags, 0, 4096);
     if (hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->hd_in), ctx->h_in, 0) != hipSuccess ||
-        hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->hd_out), ctx->h_out, 0) != hipSuccess) {
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->hd_out), ctx->h_out, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->hd_flags), ctx->h_flags, 0) != hipSuccess) {
         pm_destroy(ctx);
         return create_failed(PM_EDEVICE, "mapping page-locked staging into the device failed");
     }
-    for (hipEvent_t& ev : ctx->io_done)
-        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-            pm_destroy(ctx);
-            return create_failed(PM_EDEVICE, "event creation failed");
-        }
+    // egress chunk counters, then the ingest's mirror of the ready flags and its error word
+    const size_t io_words = pmd::kMaxIoChunks + (pmd::kMaxIoChunks + 1) * pmd::kMirrorStride;
+    ctx->d_io_counters = static_cast<uint32_t*>(ctx->device_alloc(io_words * sizeof(uint32_t)));
+    if (ctx->d_io_counters && hipMemset(ctx->d_io_counters, 0, io_words * sizeof(uint32_t)) != hipSuccess) {
+        pm_destroy(ctx);
+        return create_failed(PM_EDEVICE, "clearing the transfer counters failed");
+    }
+    if (!ctx->d_io_counters) {
+        pm_destroy(ctx);
+        return create_failed(PM_ENOMEM, "device workspace allocation failed");
+    }
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    ctx->wait_ticks = uint64_t(rate_khz) * 1000u * 5u;  // 5 s per wait on a host flag
     for (uint32_t logn = 0; logn <= pm::kMaxLogN; ++logn) ctx->outer[logn] = pm_default_outer(logn);
     *out = ctx;
     return PM_OK;
@@ -234,8 +253,7 @@ void pm_destroy(pm_context* ctx) {
     for (void* p : ctx->allocations) (void)hipFree(p);
     if (ctx->h_in) (void)hipHostFree(ctx->h_in);
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
-    for (hipEvent_t ev : ctx->io_done)
-        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     delete ctx;
 }
 
@@ -301,15 +319,21 @@ int pm_multiply(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* 
     return pm_multiply_collect(ctx, c);
 }
 
-// The host-buffer path.  The inputs a|b, as one array of 2n coefficients,
-// go to the device in `chunks` equal pieces: piece j is copied into
-// page-locked staging and a copy kernel that pulls it over PCIe is launched
-// before piece j+1 is copied, so host copies and transfers overlap.  The
-// product then runs from device memory, and the result goes back in
-// chunks/2 pieces (copy kernels writing into page-locked staging, one event
-// each) that collect copies out one by one as they land.  Copy kernels rather
-// than hipMemcpyAsync: they reach the PCIe rate at a few hundred KB and cost
-// no more than a kernel launch each.
+// The host-buffer path.  Every kernel of the product is enqueued first; the
+// first one converts a|b (one array of 2n coefficients) straight from
+// page-locked staging, each 64-coefficient group as soon as the host flags
+// its chunk ready, so host copies, PCIe transfers and conversion overlap.
+// The NTTs and CRT run from device memory, and an egress kernel returns the
+// result into page-locked staging chunk by chunk, flagging each chunk as it
+// lands; collect copies each chunk out as soon as its flag is up.
+namespace {
+
+volatile uint32_t* ready_flags(pm_context* ctx) { return ctx->h_flags; }
+volatile uint32_t* done_flags(pm_context* ctx) { return ctx->h_flags + pmd::kMaxIoChunks; }
+volatile uint32_t* error_flag(pm_context* ctx) { return ctx->h_flags + 2 * pmd::kMaxIoChunks; }
+
+}  // namespace
+
 int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b) {
     if (!ctx || !a || !b) return PM_EINVAL;
     if (ctx->pending) return pm_fail(ctx, PM_EINVAL, "a submitted product has not been collected");
@@ -320,26 +344,35 @@ int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uin
     if (e != hipSuccess) return pm_hip(ctx, e, "set device");
     const uint32_t chunks = pm_io_chunks(ctx, logn);
     const uint32_t in_chunks = chunks < 2 ? 2 : chunks;  // a and b are separate arrays
-    const uint32_t per_poly = in_chunks / 2;
-    const size_t words = size_t(n) * kLimbs, span_words = words / per_poly;
-    uint32_t* d_in = ctx->d_a;  // room for 2 * max_n coefficients: b at d_in + n * 28
-    for (uint32_t j = 0; j < in_chunks && e == hipSuccess; ++j) {
-        const uint32_t poly = j / per_poly, part = j % per_poly;
-        const size_t off = poly * words + part * span_words;
-        std::memcpy(ctx->h_in + off, (poly ? b : a) + part * span_words, span_words * sizeof(uint32_t));
-        e = pmd::launch_copy(ctx->hd_in + off, d_in + off, span_words, ctx->stream);
-    }
-    pmd::Job job{logn, 0, d_in, d_in + words, ctx->d_c, ctx->d_ra, ctx->d_rb};
-    if (e == hipSuccess) st = pm_run_plain(ctx, job, ctx->outer[logn], ctx->stream);
     const uint32_t out_chunks = chunks < 2 ? 1 : chunks / 2;
-    const size_t out_words = words / out_chunks;
-    for (uint32_t j = 0; j < out_chunks && e == hipSuccess && st == PM_OK; ++j) {
-        e = pmd::launch_copy(ctx->d_c + j * out_words, ctx->hd_out + j * out_words, out_words, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(ctx->io_done[j], ctx->stream);
-    }
-    if (e != hipSuccess || st != PM_OK) {
+    const uint32_t groups_per_poly = (n + pmd::kConvPerWave - 1) / pmd::kConvPerWave;
+    const uint32_t groups_per_chunk = 2 * groups_per_poly / in_chunks;
+    const size_t words = size_t(n) * kLimbs;
+    if (++ctx->io_seq == 0) ctx->io_seq = 1;
+    *error_flag(ctx) = 0;
+    pmd::IoArgs io{ctx->hd_flags, ctx->hd_flags + pmd::kMaxIoChunks, ctx->hd_flags + 2 * pmd::kMaxIoChunks,
+                   ctx->d_io_counters, ctx->d_io_counters + pmd::kMaxIoChunks, ctx->io_seq, in_chunks,
+                   groups_per_chunk, 0, ctx->wait_ticks};
+    pmd::Job job{logn, 0, ctx->hd_in, ctx->hd_in + words, ctx->d_c, ctx->d_ra, ctx->d_rb};
+    e = pmd::launch_ingest_convert(ctx->tables, job, io, ctx->stream);
+    // input chunk j: coefficients [j, j+1) * span of the concatenated a|b.  The
+    // other launches come after the copies: a launch costs several us of host
+    // time, and the copies are what the device is waiting for
+    const size_t span_words = words * 2 / in_chunks;
+    if (e == hipSuccess)
+        for (uint32_t j = 0; j < in_chunks; ++j) {
+            const size_t off = j * span_words;
+            const uint32_t* src = off < words ? a + off : b + (off - words);
+            std::memcpy(ctx->h_in + off, src, span_words * sizeof(uint32_t));
+            _mm_sfence();  // the copy (possibly streaming stores) lands before the flag
+            __atomic_store_n(&ready_flags(ctx)[j], ctx->io_seq, __ATOMIC_RELEASE);
+        }
+    if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, ctx->outer[logn], ctx->stream);
+    if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, ctx->stream);
+    if (e == hipSuccess) e = pmd::launch_egress(ctx->d_c, ctx->hd_out, words / out_chunks, out_chunks, io, ctx->stream);
+    if (e != hipSuccess) {
         (void)hipStreamSynchronize(ctx->stream);
-        return st != PM_OK ? st : pm_hip(ctx, e, "launch product");
+        return pm_hip(ctx, e, "launch product");
     }
     ctx->pending = true;
     ctx->pending_n = n;
@@ -351,24 +384,33 @@ int pm_multiply_collect(pm_context* ctx, uint32_t* c) {
     if (!ctx || !c) return PM_EINVAL;
     if (!ctx->pending) return pm_fail(ctx, PM_EINVAL, "no submitted product to collect");
     ctx->pending = false;
-    const uint32_t chunks = ctx->pending_out_chunks;
-    const size_t per_words = size_t(ctx->pending_n / chunks) * kLimbs;
+    const uint32_t chunks = ctx->pending_out_chunks, seq = ctx->io_seq;
+    const size_t chunk_words = size_t(ctx->pending_n) * kLimbs / chunks;
     const auto start = std::chrono::steady_clock::now();
     for (uint32_t j = 0; j < chunks; ++j) {
-        // poll rather than block: the chunk is usually done within tens of us
-        for (;;) {
-            const hipError_t q = hipEventQuery(ctx->io_done[j]);
-            if (q == hipSuccess) break;
-            if (q != hipErrorNotReady) {
+        // spin on the device-written flag; now and then look at the stream so
+        // a device failure ends the wait instead of the time limit
+        for (uint32_t spins = 0; __atomic_load_n(&done_flags(ctx)[j], __ATOMIC_ACQUIRE) != seq; ++spins) {
+            _mm_pause();
+            if ((spins & 4095u) != 4095u) continue;
+            const hipError_t q = hipStreamQuery(ctx->stream);
+            if (q != hipSuccess && q != hipErrorNotReady) {
                 (void)hipStreamSynchronize(ctx->stream);
                 return pm_hip(ctx, q, "product");
+            }
+            if (q == hipSuccess && __atomic_load_n(&done_flags(ctx)[j], __ATOMIC_ACQUIRE) != seq) {
+                return pm_fail(ctx, PM_EDEVICE, "product finished without flagging its result");
             }
             if (std::chrono::steady_clock::now() - start > std::chrono::seconds(kCollectTimeoutSeconds)) {
                 (void)hipStreamSynchronize(ctx->stream);
                 return pm_fail(ctx, PM_ETIMEOUT, "product did not complete within the time limit");
             }
         }
-        std::memcpy(c + j * per_words, ctx->h_out + j * per_words, per_words * sizeof(uint32_t));
+        std::memcpy(c + j * chunk_words, ctx->h_out + j * chunk_words, chunk_words * sizeof(uint32_t));
+    }
+    if (*error_flag(ctx) != 0) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return pm_fail(ctx, PM_ETIMEOUT, "a wait for an input chunk gave up");
     }
     return PM_OK;
 }

@@ -24,12 +24,16 @@ struct pm_context {
     // host-buffer path: page-locked staging (inputs a|b and the output) with
     // its device views, output chunk events, and the product between submit
     // and collect
-    static constexpr uint32_t kMaxIoChunks = 16;
+    static constexpr uint32_t kMaxIoChunks = pmd::kMaxIoChunks;
     uint32_t* h_in = nullptr;    // [2][max_n][28]
     uint32_t* h_out = nullptr;   // [max_n][28]
+    uint32_t* h_flags = nullptr; // ready[16] | done[16] | error
     uint32_t* hd_in = nullptr;   // device views of the same memory
     uint32_t* hd_out = nullptr;
-    hipEvent_t io_done[kMaxIoChunks] = {};
+    uint32_t* hd_flags = nullptr;
+    uint32_t* d_io_counters = nullptr;
+    uint32_t io_seq = 0;
+    uint64_t wait_ticks = 0;
     uint32_t io_chunks[pm::kMaxLogN + 1] = {};  // chunks per log2 N (0: default)
     bool pending = false;
     uint32_t pending_n = 0, pending_out_chunks = 0;
