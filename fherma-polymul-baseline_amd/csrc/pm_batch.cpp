@@ -1,11 +1,14 @@
-// Batched products of uneven N in one persistent launch captured in a hipGraph.
+// Batched products of uneven N, one launch per phase, captured once in a
+// hipGraph and replayed.
 //
 // The reference has no batch path (one product per fherma_run call,
 // rns/solve.cu:721-853); this builds a work list over many products -- residue
-// conversion blocks, optional outer NTT stages, fused convolution tiles, CRT
-// blocks -- that resident blocks claim in order and run as soon as the items
-// they depend on have finished (per-product counters), so conversion, NTT and
-// CRT of different products overlap inside a single kernel.
+// conversion blocks, optional outer NTT stages, fused convolution tiles
+// grouped by tile size and ordered prime-major, CRT blocks -- in which every
+// phase is one contiguous run of items, and launches one kernel per phase over
+// its run.  (A persistent single-kernel mode whose blocks claimed items and
+// waited on per-product counters was removed after it hung the GPU; the flag
+// is refused.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -116,15 +119,13 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     if (b->blocks > b->n_items) b->blocks = b->n_items;
     if (b->blocks == 0) b->blocks = 1;
 
-    // capture reset + persistent kernel once; runs replay the graph
+    // capture the phase launches once; runs replay the graph
     hipStream_t cap = nullptr;
     e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
     if (e != hipSuccess) return pm_hip(ctx, e, "capture stream");
     e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipError_t le = b->persistent
-                            ? pmd::launch_persistent(ctx->tables, ba, b->d_words, b->reset_count, b->blocks, cap)
-                            : pmd::launch_phased(ctx->tables, ba, b->ranges, b->scalar_conversion, b->scalar_crt, cap);
+        hipError_t le = pmd::launch_phased(ctx->tables, ba, b->ranges, b->scalar_conversion, b->scalar_crt, cap);
         hipError_t ce = hipStreamEndCapture(cap, &b->graph);
         e = le != hipSuccess ? le : ce;
     }
@@ -221,7 +222,7 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
         return PM_EINVAL;
     *out = nullptr;
     // The persistent kernel hung the GPU on a uniform batch after a change to
-    // the tile code it shares; no cause was found, so the mode is switched off.
+    // the tile code it shares; no cause was found, so the mode was removed.
     if (flags & PM_BATCH_PERSISTENT)
         return pm_fail(ctx, PM_EUNSUPPORTED, "persistent batch mode is disabled (it hung the GPU in testing); "
                                              "use the phased mode");
@@ -231,7 +232,6 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
     if (!b) return pm_fail(ctx, PM_ENOMEM, "host allocation failed");
     b->ctx = ctx;
     b->count = count;
-    b->persistent = (flags & PM_BATCH_PERSISTENT) != 0;
     b->scalar_conversion = (flags & PM_BATCH_SCALAR_CONVERSION) != 0;
     b->scalar_crt = (flags & PM_BATCH_SCALAR_CRT) != 0;
     const int st = pm_batch_build(ctx, b, ns, d_a, d_b, d_c);
@@ -344,11 +344,7 @@ int pm_batch_phase_times(pm_batch* b, void* hip_stream, uint32_t max, uint32_t* 
 int pm_batch_info(const pm_batch* b, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes) {
     if (!b) return PM_EINVAL;
     if (items) *items = b->n_items;
-    if (blocks && !b->persistent) {
-        *blocks = 0;  // phased: one launch per phase, sized to its items
-        blocks = nullptr;
-    }
-    if (blocks) *blocks = b->blocks;
+    if (blocks) *blocks = 0;  // one launch per phase, each sized to its items
     if (device_bytes) *device_bytes = b->device_bytes;
     return PM_OK;
 }

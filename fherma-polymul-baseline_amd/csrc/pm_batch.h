@@ -23,7 +23,6 @@ struct pm_batch {
     std::vector<pmd::Job> jobs;     // host copy (device pointers)
     pmd::BatchArgs args{};
     pmd::PhaseRanges ranges{};
-    bool persistent = false;        // one persistent launch, else one launch per phase
     bool scalar_conversion = false; // phased only: scalar limb reduction instead of MFMA
     bool scalar_crt = false;        // phased only: vector-ALU CRT sums instead of MFMA
     uint32_t* d_words = nullptr;    // claim counter, error flag, per-job counters

@@ -9,9 +9,6 @@
 
 namespace pmd {
 
-// words 32-bit words from src to dst (16-byte aligned; either may be mapped
-// page-locked host memory)
-hipError_t launch_copy(const uint32_t* src, uint32_t* dst, size_t words, hipStream_t stream);
 // host-buffer path: conversion of a|b from page-locked host memory, group by
 // group as the host flags input chunks ready (also clears io.counters)
 hipError_t launch_ingest_convert(const Tables& tb, const Job& job, const IoArgs& io, hipStream_t stream);
@@ -38,9 +35,6 @@ hipError_t launch_crt_range(const Tables& tb, const Job& job, uint32_t c_first, 
                             hipStream_t stream);
 // the same by 64-bit multiply-accumulate on the vector ALUs (comparison path)
 hipError_t launch_crt_scalar(const Tables& tb, const Job& job, hipStream_t stream);
-// reset counters, then one persistent launch over the batch work list
-hipError_t launch_persistent(const Tables& tb, const BatchArgs& ba, uint32_t* reset_words, uint32_t reset_count,
-                             uint32_t blocks, hipStream_t stream);
 // Contiguous item ranges of a work list by phase (items are ordered by type).
 struct PhaseRanges {
     uint32_t conv_first = 0, conv_count = 0;

@@ -62,9 +62,18 @@ __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p,
 // butterfly of the phase is register-local.  Phases are joined by an exchange
 // through padded LDS (address x + x/32).
 // ---------------------------------------------------------------------------
+// log2 of the residues per thread of a tile of 2^tlog: 32 per thread, in
+// 1024-thread blocks of 128 VGPRs.  (64 per thread in 512-thread blocks of
+// 256 VGPRs measured 26 % slower on the 2^15 tile: half the waves per CU.)
+__host__ __device__ constexpr int tile_elog(int tlog) { return tlog < 5 ? tlog : 5; }
+// threads of a block that holds one 2^15-residue tile (or several smaller ones)
+__host__ __device__ constexpr uint32_t tile_block_threads(int tlog) {
+    return tlog < 5 ? kBlock : (kTileElems >> tile_elog(tlog));
+}
+
 template <int TLOG>
 struct Geo {
-    static constexpr int ELOG = TLOG < 5 ? TLOG : 5;
+    static constexpr int ELOG = tile_elog(TLOG);
     static constexpr int E = 1 << ELOG;
     static constexpr int TPTLOG = TLOG - ELOG;
     static constexpr int TPT = 1 << TPTLOG;
@@ -643,7 +652,8 @@ __global__ void __launch_bounds__(256) k_convert_scalar(Tables tb, Job job) {
 
 // units = k << u sub-blocks; unit -> (prime = unit >> u, s = unit & (2^u-1))
 template <int TLOG>
-__global__ void __launch_bounds__(kBlock) k_conv_tile(Tables tb, Job job, uint32_t u, uint32_t units) {
+__global__ void __launch_bounds__(tile_block_threads(TLOG)) k_conv_tile(Tables tb, Job job, uint32_t u,
+                                                                        uint32_t units) {
     extern __shared__ uint32_t lds[];
     using G = Geo<TLOG>;
     const uint32_t tiles_per_block = blockDim.x >> G::TPTLOG;
@@ -709,61 +719,8 @@ __device__ __forceinline__ void outer_inverse_columns(const Tables& tb, uint32_t
 }
 
 // ---------------------------------------------------------------------------
-// persistent batch kernel
+// batch work-list items
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t conv_items_of(const Job& j) {
-    const uint32_t n = 1u << j.logn;
-    return 2u * ((n + kConvPerBlock - 1) / kConvPerBlock);
-}
-
-// Spin (thread 0 only) until *counter >= target; false on error or timeout.
-__device__ bool wait_for(uint32_t* counter, uint32_t target, const BatchArgs& ba) {
-    const uint64_t start = wall_clock64();
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (__hip_atomic_load(ba.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
-        if (wall_clock64() - start > ba.wait_ticks) {
-            atomicMax(ba.error, kErrTimeout);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-    return true;
-}
-
-__device__ bool wait_item_deps(const Tables& tb, const BatchArgs& ba, const Item& it) {
-    const uint32_t k = tb.k;
-    switch (it.type) {
-        case kItemConv:
-            return true;
-        case kItemOuterFwd:
-            return wait_for(&ba.counters[it.job].conv, conv_items_of(ba.jobs[it.job]), ba);
-        case kItemTile: {
-            const uint32_t count = it.a0 >> 8;
-            uint32_t last = 0xFFFFFFFFu;
-            for (uint32_t i = 0; i < count; ++i) {
-                const uint32_t j = ba.units[it.a1 + i].job;
-                if (j == last) continue;
-                last = j;
-                const Job job = ba.jobs[j];
-                const bool ok = job.outer == 0 ? wait_for(&ba.counters[j].conv, conv_items_of(job), ba)
-                                               : wait_for(&ba.counters[j].fwd, 2u * k, ba);
-                if (!ok) return false;
-            }
-            return true;
-        }
-        case kItemOuterInv: {
-            const Job job = ba.jobs[it.job];
-            return wait_for(&ba.counters[it.job].tile, k << job.outer, ba);
-        }
-        case kItemCrt: {
-            const Job job = ba.jobs[it.job];
-            return job.outer == 0 ? wait_for(&ba.counters[it.job].tile, k, ba)
-                                  : wait_for(&ba.counters[it.job].inv, k, ba);
-        }
-    }
-    return false;
-}
-
 template <int TLOG>
 __device__ __forceinline__ void tile_item(const Tables& tb, const BatchArgs& ba, const Item& it, uint32_t* lds) {
     using G = Geo<TLOG>;
@@ -776,118 +733,6 @@ __device__ __forceinline__ void tile_item(const Tables& tb, const BatchArgs& ba,
     const size_t off = (size_t(prime) << job.logn) + (size_t(s) << TLOG);
     conv_tile<TLOG>(tb, job.ra + off, job.rb + off, job.ra + off, prime, (1u << job.outer) + s, job.logn,
                     lds + tile * G::STRIDE, tau, valid);
-}
-
-// Item handlers are separate (not inlined) functions so that each gets the
-// full register budget instead of the union of all paths.
-__device__ __noinline__ void handle_tile(const Tables& tb, const BatchArgs& ba, const Item& it, uint32_t* lds) {
-    switch (it.a0 & 0xFFu) {
-        case 5: tile_item<5>(tb, ba, it, lds); break;
-        case 6: tile_item<6>(tb, ba, it, lds); break;
-        case 7: tile_item<7>(tb, ba, it, lds); break;
-        case 8: tile_item<8>(tb, ba, it, lds); break;
-        case 9: tile_item<9>(tb, ba, it, lds); break;
-        case 10: tile_item<10>(tb, ba, it, lds); break;
-        case 11: tile_item<11>(tb, ba, it, lds); break;
-        case 12: tile_item<12>(tb, ba, it, lds); break;
-        case 13: tile_item<13>(tb, ba, it, lds); break;
-        case 14: tile_item<14>(tb, ba, it, lds); break;
-        case 15: tile_item<15>(tb, ba, it, lds); break;
-        default: break;
-    }
-}
-
-__device__ __noinline__ void handle_conv(const Tables& tb, const Job& job, const Item& it, uint32_t* lds,
-                                         bool load_weights) {
-    uint4* lds_w = reinterpret_cast<uint4*>(lds);
-    const uint32_t tid = threadIdx.x;
-    if (load_weights) {
-        for (uint32_t i = tid; i < tb.quads * 128u; i += kBlock) lds_w[i] = tb.conv_w[i];
-        __syncthreads();
-    }
-    const uint32_t n = 1u << job.logn;
-    const uint32_t c0 = it.a1 * kConvPerBlock + (tid >> 6) * kConvPerWave;
-    if (c0 < n) convert_wave(tb, it.a0 ? job.b : job.a, n, n, c0, it.a0 ? job.rb : job.ra, lds_w, tid & 63u);
-}
-
-__device__ __noinline__ void handle_outer(const Tables& tb, const Job& job, const Item& it) {
-    const uint32_t tid = threadIdx.x, tlog = job.logn - job.outer;
-    if (it.type == kItemOuterFwd) {
-        const uint32_t poly = it.a0 / tb.k, prime = it.a0 % tb.k;
-        uint32_t* base = (poly ? job.rb : job.ra) + (size_t(prime) << job.logn);
-        if (job.outer == 1) outer_forward_columns<1>(tb, base, tlog, prime, tid, kBlock);
-        else outer_forward_columns<2>(tb, base, tlog, prime, tid, kBlock);
-    } else {
-        uint32_t* base = job.ra + (size_t(it.a0) << job.logn);
-        if (job.outer == 1) outer_inverse_columns<1>(tb, base, tlog, it.a0, tid, kBlock);
-        else outer_inverse_columns<2>(tb, base, tlog, it.a0, tid, kBlock);
-    }
-}
-
-__device__ __noinline__ void handle_crt(const Tables& tb, const Job& job, const Item& it) {
-    const uint32_t n = 1u << job.logn;
-    const uint32_t c = it.a0 * kBlock + threadIdx.x;
-    if (c < n) crt_one(tb, job.ra, n, c, job.c);
-}
-
-__global__ void __launch_bounds__(kBlock) k_persistent(Tables tb, BatchArgs ba) {
-    extern __shared__ uint32_t lds[];
-    __shared__ uint32_t s_item, s_ok;
-    bool weights_in_lds = false;
-    const uint32_t tid = threadIdx.x;
-    for (;;) {
-        if (tid == 0) {
-            uint32_t i = atomicAdd(ba.next, 1u);
-            if (__hip_atomic_load(ba.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) i = ba.n_items;
-            s_item = i;
-        }
-        __syncthreads();
-        const uint32_t idx = s_item;
-        if (idx >= ba.n_items) break;
-        const Item it = ba.items[idx];
-        if (tid == 0) s_ok = wait_item_deps(tb, ba, it) ? 1u : 0u;
-        __syncthreads();
-        if (!s_ok) continue;  // error latched: drain the queue
-        const Job job = ba.jobs[it.type == kItemTile ? 0u : it.job];
-        switch (it.type) {
-            case kItemConv:
-                handle_conv(tb, job, it, lds, !weights_in_lds);
-                weights_in_lds = true;
-                break;
-            case kItemOuterFwd:
-            case kItemOuterInv:
-                handle_outer(tb, job, it);
-                break;
-            case kItemTile:
-                weights_in_lds = false;
-                handle_tile(tb, ba, it, lds);
-                break;
-            case kItemCrt:
-                handle_crt(tb, job, it);
-                break;
-        }
-        __syncthreads();  // all results of the item stored (and LDS free)
-        if (tid == 0) {
-            __threadfence();  // release the item's results at device scope
-            switch (it.type) {
-                case kItemConv: atomicAdd(&ba.counters[it.job].conv, 1u); break;
-                case kItemOuterFwd: atomicAdd(&ba.counters[it.job].fwd, 1u); break;
-                case kItemTile: {
-                    const uint32_t count = it.a0 >> 8;
-                    for (uint32_t i = 0; i < count; ++i) atomicAdd(&ba.counters[ba.units[it.a1 + i].job].tile, 1u);
-                    break;
-                }
-                case kItemOuterInv: atomicAdd(&ba.counters[it.job].inv, 1u); break;
-                default: break;
-            }
-        }
-    }
-}
-
-// clears the claim counter, error flag and per-job counters (a kernel, so it
-// can sit in a captured graph)
-__global__ void k_reset(uint32_t* words, uint32_t count) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) words[i] = 0u;
 }
 
 // Outer forward CT stages (bits logn-1 .. logn-u) for a and b: one thread per
@@ -995,7 +840,7 @@ __global__ void __launch_bounds__(kBlock) k_items_outer(Tables tb, BatchArgs ba,
 }
 
 template <int TLOG>
-__global__ void __launch_bounds__(kBlock) k_items_tile(Tables tb, BatchArgs ba, uint32_t first) {
+__global__ void __launch_bounds__(tile_block_threads(TLOG)) k_items_tile(Tables tb, BatchArgs ba, uint32_t first) {
     extern __shared__ uint32_t lds[];
     const Item it = ba.items[first + blockIdx.x];
     tile_item<TLOG>(tb, ba, it, lds);
@@ -1093,7 +938,7 @@ template <int TLOG>
 static hipError_t launch_tile(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream) {
     using G = Geo<TLOG>;
     const uint32_t units = tb.k << u;
-    uint32_t tiles = kBlock >> G::TPTLOG;
+    uint32_t tiles = tile_block_threads(TLOG) >> G::TPTLOG;
     if (tiles > units) tiles = units;
     // fewer tiles per block while that leaves CUs idle: one product has only
     // k << u tiles
@@ -1129,18 +974,6 @@ static hipError_t launch_tile_dispatch(uint32_t tlog, const Tables& tb, const Jo
         case 15: return launch_tile<15>(tb, job, u, stream);
         default: return hipErrorInvalidValue;
     }
-}
-
-// words / 4 16-byte pieces, then the remaining words; either side may be
-// page-locked host memory (the loads and stores then cross PCIe)
-__global__ void __launch_bounds__(256) k_copy(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                              size_t words) {
-    const size_t vecs = words / 4, stride = size_t(gridDim.x) * blockDim.x;
-    const uint4* s4 = reinterpret_cast<const uint4*>(src);
-    uint4* d4 = reinterpret_cast<uint4*>(dst);
-    const size_t tid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (size_t i = tid; i < vecs; i += stride) d4[i] = s4[i];
-    for (size_t i = vecs * 4 + tid; i < words; i += stride) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1277,16 +1110,6 @@ hipError_t launch_egress(const uint32_t* src, uint32_t* dst_host, size_t chunk_w
     return hipSuccess;
 }
 
-hipError_t launch_copy(const uint32_t* src, uint32_t* dst, size_t words, hipStream_t stream) {
-    if (!words) return hipSuccess;
-    if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) return hipErrorInvalidValue;
-    // ~128 blocks keep a PCIe transfer at its rate; fewer for small pieces
-    size_t blocks = (words / 4 + 1023) / 1024;
-    blocks = blocks < 1 ? 1 : (blocks > 128 ? 128 : blocks);
-    hipLaunchKernelGGL(k_copy, dim3(uint32_t(blocks)), dim3(256), 0, stream, src, dst, words);
-    return hipGetLastError();
-}
-
 hipError_t launch_convert_range(const Tables& tb, const Job& job, uint32_t c_first, uint32_t c_end,
                                 uint32_t poly_first, uint32_t polys, hipStream_t stream) {
     const uint32_t n = 1u << job.logn;
@@ -1333,20 +1156,11 @@ hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t 
     return e;
 }
 
-hipError_t launch_persistent(const Tables& tb, const BatchArgs& ba, uint32_t* reset_words, uint32_t reset_count,
-                             uint32_t blocks, hipStream_t stream) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, reset_words, reset_count);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_persistent, dim3(blocks), dim3(kBlock), size_t(kLdsWords) * sizeof(uint32_t), stream, tb,
-                       ba);
-    return hipGetLastError();
-}
-
 template <int TLOG>
 static void launch_items_tile(const Tables& tb, const BatchArgs& ba, uint32_t first, uint32_t count,
                               hipStream_t stream) {
-    hipLaunchKernelGGL(k_items_tile<TLOG>, dim3(count), dim3(kBlock), size_t(kLdsWords) * sizeof(uint32_t), stream,
+    hipLaunchKernelGGL(k_items_tile<TLOG>, dim3(count), dim3(tile_block_threads(TLOG)),
+                       size_t(kLdsWords) * sizeof(uint32_t), stream,
                        tb, ba, first);
 }
 
@@ -1499,8 +1313,6 @@ hipError_t configure_kernels() {
     if ((e = allow_items_lds<13>()) != hipSuccess) return e;
     if ((e = allow_items_lds<14>()) != hipSuccess) return e;
     if ((e = allow_items_lds<15>()) != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_persistent),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(kLdsWords * sizeof(uint32_t)));
+    return hipSuccess;
 }
 }  // namespace pmd

@@ -220,6 +220,17 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
         rate_khz = 100000;
     ctx->wait_ticks = uint64_t(rate_khz) * 1000u * 5u;  // 5 s per wait on a host flag
     for (uint32_t logn = 0; logn <= pm::kMaxLogN; ++logn) ctx->outer[logn] = pm_default_outer(logn);
+    // one product of size max_n on zero inputs: the runtime loads each kernel
+    // at its first launch, which would otherwise land in the first timed call
+    {
+        std::vector<uint32_t> zeros(size_t(max_n) * kLimbs, 0u), sink(zeros.size());
+        const int st = pm_multiply(ctx, max_n, zeros.data(), zeros.data(), sink.data());
+        if (st != PM_OK) {
+            const std::string reason = ctx->last_error;
+            pm_destroy(ctx);
+            return create_failed(st, "warm-up product: " + reason);
+        }
+    }
     *out = ctx;
     return PM_OK;
 }

@@ -22,8 +22,8 @@ def say(*a):
 
 
 @pytest.mark.skip(reason="a modified persistent kernel faulted the GPU on this uneven mix and the one run "
-                         "testing a fix faulted again; the kernel was reverted to the last version this test "
-                         "passed with, and the case stays off the GPU until a cause is shown")
+                         "testing a fix faulted again; no cause was shown, the persistent mode was removed "
+                         "(the library refuses it) and the case stays off the GPU")
 def test_batch_uneven_sizes(pm, orc, workload):
     q = workload.modulus(1)
     ctx = pm.Context(q, 1 << 17)
@@ -79,7 +79,7 @@ def test_batch_uneven_sizes_phased(pm, orc, workload):
 
 
 PERSISTENT_OFF = ("the persistent kernel hung the GPU on this batch after a change to the tile code it shares; "
-                  "no cause was found, the mode is disabled and the case stays off the GPU")
+                  "no cause was found, the mode was removed (the library refuses it) and the case stays off the GPU")
 
 
 @pytest.mark.parametrize("mode", [pytest.param("persistent", marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
@@ -111,5 +111,5 @@ def test_batch_rejects_small_n(pm, workload):
     with pytest.raises(pm.PolymulError):
         pm.Batch(ctx, [2048])
     with pytest.raises(pm.PolymulError):
-        pm.Batch(ctx, [1024], mode="persistent")  # disabled mode is refused, never launched
+        pm.Batch(ctx, [1024], mode="persistent")  # removed mode is refused, never launched
     ctx.close()
