@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -69,12 +70,12 @@ static const T* upload(pm_context* ctx, const std::vector<T>& v, int* status) {
 uint32_t pm_default_outer(uint32_t logn) { return logn <= 13 ? 0u : (logn == 14 ? 1u : 2u); }
 
 // input chunks of the host-buffer path (the output goes in half as many):
-// the set value, else pieces of at most 2^13 coefficients (~0.9 MB), 2 to 16
-// of them
+// the set value, else pieces of at most 2^12 coefficients (~0.46 MB), 2 to
+// 16 of them
 uint32_t pm_io_chunks(const pm_context* ctx, uint32_t logn) {
     if (ctx->io_chunks[logn]) return ctx->io_chunks[logn];
     uint32_t chunks = 2;
-    while (chunks < pm_context::kMaxIoChunks && (2u << logn) / chunks > 8192u) chunks *= 2;
+    while (chunks < pm_context::kMaxIoChunks && (2u << logn) / chunks > 4096u) chunks *= 2;
     return chunks;
 }
 
@@ -219,6 +220,15 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
         rate_khz = 100000;
     ctx->wait_ticks = uint64_t(rate_khz) * 1000u * 5u;  // 5 s per wait on a host flag
+    {
+        const unsigned cores = std::thread::hardware_concurrency();
+        const unsigned workers = cores > 4 ? 3u : (cores > 1 ? cores - 1 : 0u);
+        ctx->copy_pool.reset(new (std::nothrow) PmCopyPool(workers));
+        if (!ctx->copy_pool) {
+            pm_destroy(ctx);
+            return create_failed(PM_ENOMEM, "host allocation failed");
+        }
+    }
     for (uint32_t logn = 0; logn <= pm::kMaxLogN; ++logn) ctx->outer[logn] = pm_default_outer(logn);
     // one product of size max_n on zero inputs: the runtime loads each kernel
     // at its first launch, which would otherwise land in the first timed call
@@ -262,6 +272,7 @@ void pm_destroy(pm_context* ctx) {
     }
 
     for (void* p : ctx->allocations) (void)hipFree(p);
+    ctx->copy_pool.reset();  // joins the copy threads
     if (ctx->h_in) (void)hipHostFree(ctx->h_in);
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
@@ -370,14 +381,15 @@ int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uin
     // other launches come after the copies: a launch costs several us of host
     // time, and the copies are what the device is waiting for
     const size_t span_words = words * 2 / in_chunks;
-    if (e == hipSuccess)
+    if (e == hipSuccess) {
+        PmCopyPool::Chunk chunks_of_input[pmd::kMaxIoChunks];
         for (uint32_t j = 0; j < in_chunks; ++j) {
             const size_t off = j * span_words;
-            const uint32_t* src = off < words ? a + off : b + (off - words);
-            std::memcpy(ctx->h_in + off, src, span_words * sizeof(uint32_t));
-            _mm_sfence();  // the copy (possibly streaming stores) lands before the flag
-            __atomic_store_n(&ready_flags(ctx)[j], ctx->io_seq, __ATOMIC_RELEASE);
+            chunks_of_input[j] = {ctx->h_in + off, off < words ? a + off : b + (off - words),
+                                  span_words * sizeof(uint32_t), &ready_flags(ctx)[j], ctx->io_seq};
         }
+        ctx->copy_pool->run(chunks_of_input, in_chunks);  // returns once a and b are no longer needed
+    }
     if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, ctx->outer[logn], ctx->stream);
     if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, ctx->stream);
     if (e == hipSuccess) e = pmd::launch_egress(ctx->d_c, ctx->hd_out, words / out_chunks, out_chunks, io, ctx->stream);

@@ -3,9 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
+#include "pm_copy_pool.h"
 #include "pm_device.h"
 #include "pm_params.h"
 
@@ -34,6 +36,7 @@ struct pm_context {
     uint32_t* d_io_counters = nullptr;
     uint32_t io_seq = 0;
     uint64_t wait_ticks = 0;
+    std::unique_ptr<PmCopyPool> copy_pool;  // host copies of the input chunks
     uint32_t io_chunks[pm::kMaxLogN + 1] = {};  // chunks per log2 N (0: default)
     bool pending = false;
     uint32_t pending_n = 0, pending_out_chunks = 0;

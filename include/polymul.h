@@ -79,7 +79,7 @@ int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uin
 int pm_multiply_collect(pm_context* ctx, uint32_t* c);
 /* Transfer chunks of the host-buffer path for products of size n (no
  * counterpart in the reference): the inputs a|b go in `chunks` pieces and
- * the result in chunks/2.  0 restores the default (pieces of at most 2^13
+ * the result in chunks/2.  0 restores the default (pieces of at most 2^12
  * coefficients, 2..16 of them); otherwise a power of two <= 16 with
  * n / chunks >= 64. */
 int pm_set_io_chunks(pm_context* ctx, uint32_t n, uint32_t chunks);
