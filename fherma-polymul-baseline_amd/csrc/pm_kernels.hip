@@ -306,42 +306,103 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
 // coefficient c0 + 16*tile + l%16; the weight operand of quad Q holds rows
 // (prime 4Q + i/4, digit i%4).  Output lane l gets the 4 digit sums of
 // (prime 4Q + l/16, coefficient c0 + 16*tile + l%16).
-// Coefficients at or past c_end are neither read nor stored; n is the row
-// stride of the residue array.
-__device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* __restrict__ coef, uint32_t n,
-                                             uint32_t c_end, uint32_t c0, uint32_t* __restrict__ res,
-                                             const uint4* __restrict__ lds_w, uint32_t lane,
-                                             uint32_t quad_begin = 0, uint32_t quad_end = 0xFFFFFFFFu) {
+// Per-block conversion tables in LDS: the MFMA weight operands and, per
+// (quad, lane group), the prime, its Shoup constant of 2^16 and the digit-sum
+// corrections -- read with LDS instructions so that no global load is issued
+// between a group's prefetch and its use (the vector memory counter drains in
+// order).
+struct ConvLds {
+    uint4 w[16 * 2 * 64];  // [quad][step][lane]
+    uint32_t p[64];
+    uint32_t sh16[64];
+    int4 corr[64];
+};
+
+// whole block; ends with a barrier.  Slice [quad_begin, quad_end) of the quads.
+__device__ __forceinline__ void conv_tables_to_lds(const Tables& tb, ConvLds& L, uint32_t quad_begin = 0,
+                                                   uint32_t quad_end = 0xFFFFFFFFu) {
+    quad_end = quad_end < tb.quads ? quad_end : tb.quads;
+    for (uint32_t i = threadIdx.x + quad_begin * 128u; i < quad_end * 128u; i += blockDim.x) L.w[i] = tb.conv_w[i];
+    for (uint32_t i = threadIdx.x; i < 64u; i += blockDim.x) {
+        const uint32_t pi = i < tb.k ? i : 0u;
+        L.p[i] = tb.primes[pi];
+        L.sh16[i] = tb.conv_sh16[pi];
+        L.corr[i] = i < tb.quads * 4u ? reinterpret_cast<const int4*>(tb.conv_corr)[i] : make_int4(0, 0, 0, 0);
+    }
+    __syncthreads();
+}
+
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) uint32_t* global_store_u32;
+
+// Operand bytes of one 64-coefficient group as loaded (raw, unconditional
+// buffer loads, so a prefetch can stay in flight): lane l holds 16 bytes at
+// 16*(l/16) and at 64 + 16*(l/16) of coefficient c0 + 16*tile + l%16.  The
+// buffer's range is the c_end coefficients: bytes past the 112-byte
+// coefficient (l >= 48, second load) and coefficients past c_end are read as
+// zero by the hardware range check, without a memory access.
+constexpr uint32_t kRawBufferWord3 = 0x00020000u;  // 32-bit data format, no swizzle (gfx9 V# word 3)
+constexpr uint32_t kOutOfRange = 0x80000000u;       // an offset past any buffer range used here
+
+__device__ __forceinline__ void conv_load(const uint32_t* __restrict__ coef, uint32_t c_end, uint32_t c0,
+                                          uint32_t lane, v4u (&v0)[4], v4u (&v1)[4]) {
+    const uint32_t col = lane & 15u, kg = lane >> 4;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(coef), short(0), int(c_end * (kLimbs * 4u)), int(kRawBufferWord3));
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const uint32_t byte0 = (c0 + 16u * ct + col) * (kLimbs * 4u) + 16u * kg;
+        v0[ct] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, byte0, 0, 0);
+        v1[ct] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, kg < 3 ? byte0 + 64u : kOutOfRange, 0, 0);
+    }
+}
+
+// The same operand bytes read from page-locked host memory: plain loads,
+// issued only by the lanes that need them (an out-of-range buffer load would
+// still take a slot in the PCIe read path).
+__device__ __forceinline__ void conv_load_host(const uint32_t* __restrict__ coef, uint32_t c_end, uint32_t c0,
+                                               uint32_t lane, v4u (&v0)[4], v4u (&v1)[4]) {
+    const uint32_t col = lane & 15u, kg = lane >> 4;
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const uint32_t c = c0 + 16u * ct + col;
+        v0[ct] = v4u{0, 0, 0, 0};
+        v1[ct] = v4u{0, 0, 0, 0};
+        if (c < c_end) {
+            const v4u* src = reinterpret_cast<const v4u*>(coef + size_t(c) * kLimbs);
+            v0[ct] = src[kg];
+            if (kg < 3) v1[ct] = src[4 + kg];
+        }
+    }
+}
+
+// Residues of the group for quads [quad_begin, quad_end) of primes; n is the
+// row stride of the residue array, coefficients at or past c_end are not
+// stored.
+__device__ __forceinline__ void conv_compute(const Tables& tb, const v4u (&v0)[4], const v4u (&v1)[4], uint32_t n,
+                                             uint32_t c_end, uint32_t c0, uint32_t* __restrict__ res_any,
+                                             const ConvLds& L, uint32_t lane, uint32_t quad_begin = 0,
+                                             uint32_t quad_end = 0xFFFFFFFFu) {
+    const global_store_u32 res = (global_store_u32)res_any;
     const uint32_t col = lane & 15u, kg = lane >> 4;
     v4i b0[4], b1[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
-        const uint32_t c = c0 + 16u * ct + col;
-        uint4 v0 = make_uint4(0, 0, 0, 0), v1 = make_uint4(0, 0, 0, 0);
-        if (c < c_end) {
-            const uint4* src = reinterpret_cast<const uint4*>(coef + size_t(c) * kLimbs);
-            v0 = src[kg];
-            if (kg < 3) v1 = src[4 + kg];
-        }
-        // bytes a in [0,255] -> a - 128 in [-128,127]
-        b0[ct] = v4i{int(v0.x ^ 0x80808080u), int(v0.y ^ 0x80808080u), int(v0.z ^ 0x80808080u),
-                     int(v0.w ^ 0x80808080u)};
-        b1[ct] = kg < 3 ? v4i{int(v1.x ^ 0x80808080u), int(v1.y ^ 0x80808080u), int(v1.z ^ 0x80808080u),
-                              int(v1.w ^ 0x80808080u)}
-                        : v4i{0, 0, 0, 0};
+        // bytes a in [0,255] -> a - 128 in [-128,127] (the zero bytes past the
+        // coefficient become -128 but meet zero weight rows)
+        const v4u m = v4u{0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u};
+        b0[ct] = __builtin_bit_cast(v4i, v0[ct] ^ m);
+        b1[ct] = __builtin_bit_cast(v4i, v1[ct] ^ m);
     }
     quad_end = quad_end < tb.quads ? quad_end : tb.quads;
     for (uint32_t quad = quad_begin; quad < quad_end; ++quad) {
-        const uint4 w0u = lds_w[(quad * 2 + 0) * 64 + lane];
-        const uint4 w1u = lds_w[(quad * 2 + 1) * 64 + lane];
-        const v4i w0 = v4i{int(w0u.x), int(w0u.y), int(w0u.z), int(w0u.w)};
-        const v4i w1 = v4i{int(w1u.x), int(w1u.y), int(w1u.z), int(w1u.w)};
+        const v4i w0 = __builtin_bit_cast(v4i, L.w[(quad * 2 + 0) * 64 + lane]);
+        const v4i w1 = __builtin_bit_cast(v4i, L.w[(quad * 2 + 1) * 64 + lane]);
         const uint32_t prime = quad * 4u + kg;
         const bool live = prime < tb.k;
-        const uint32_t pi = live ? prime : 0u;
-        const uint32_t p = tb.primes[pi], p2 = 2u * p;
-        const int4 corr = reinterpret_cast<const int4*>(tb.conv_corr)[quad * 4u + kg];
-        const Tw c16{65536u, tb.conv_sh16[pi]};
+        const uint32_t p = L.p[prime], p2 = 2u * p;
+        const int4 corr = L.corr[prime];
+        const Tw c16{65536u, L.sh16[prime]};
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
             v4i acc = v4i{0, 0, 0, 0};
@@ -359,6 +420,16 @@ __device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* _
             if (live && c < c_end) res[size_t(prime) * n + c] = r;
         }
     }
+}
+
+// One group, loaded then converted (no prefetch).
+__device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* __restrict__ coef, uint32_t n,
+                                             uint32_t c_end, uint32_t c0, uint32_t* __restrict__ res,
+                                             const ConvLds& L, uint32_t lane, uint32_t quad_begin = 0,
+                                             uint32_t quad_end = 0xFFFFFFFFu) {
+    v4u v0[4], v1[4];
+    conv_load(coef, c_end, c0, lane, v0, v1);
+    conv_compute(tb, v0, v1, n, c_end, c0, res, L, lane, quad_begin, quad_end);
 }
 
 // Scalar limb reduction, the comparison path of the MFMA conversion (role of
@@ -628,18 +699,16 @@ constexpr size_t crt_block_lds() {
 // so a single product still spreads over many waves.
 __global__ void __launch_bounds__(kBlock) k_convert(Tables tb, Job job, uint32_t c_first, uint32_t c_end,
                                                     uint32_t poly_first) {
-    __shared__ uint4 lds_w[16 * 2 * 64];
+    __shared__ ConvLds L;
     const uint32_t per = (tb.quads + gridDim.z - 1) / gridDim.z;
     const uint32_t quad_begin = blockIdx.z * per, quad_end = quad_begin + per;
-    for (uint32_t i = threadIdx.x + quad_begin * 128u; i < tb.quads * 128u && i < quad_end * 128u; i += blockDim.x)
-        lds_w[i] = tb.conv_w[i];
-    __syncthreads();
+    conv_tables_to_lds(tb, L, quad_begin, quad_end);
     const uint32_t n = 1u << job.logn;
     const uint32_t poly = poly_first + blockIdx.y;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t c0 = c_first + blockIdx.x * blockDim.x + wave * kConvPerWave;  // one coefficient per thread
     if (c0 >= c_end) return;
-    convert_wave(tb, poly ? job.b : job.a, n, c_end, c0, poly ? job.rb : job.ra, lds_w, lane, quad_begin, quad_end);
+    convert_wave(tb, poly ? job.b : job.a, n, c_end, c0, poly ? job.rb : job.ra, L, lane, quad_begin, quad_end);
 }
 
 __global__ void __launch_bounds__(256) k_convert_scalar(Tables tb, Job job) {
@@ -802,16 +871,51 @@ __global__ void __launch_bounds__(kCrtPerBlock) k_crt(Tables tb, Job job) {
 // phased batch: one launch per phase over a contiguous range of the work list
 // (the launches order the phases; block b runs item first + b)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, uint32_t first) {
-    __shared__ uint4 lds_w[16 * 2 * 64];
-    for (uint32_t i = threadIdx.x; i < tb.quads * 128u; i += blockDim.x) lds_w[i] = tb.conv_w[i];
-    __syncthreads();
-    const Item it = ba.items[first + blockIdx.x];
-    const Job job = ba.jobs[it.job];
-    const uint32_t n = 1u << job.logn;
-    const uint32_t c0 = it.a1 * kConvPerBlock + (threadIdx.x >> 6) * kConvPerWave;
-    if (c0 >= n) return;
-    convert_wave(tb, it.a0 ? job.b : job.a, n, n, c0, it.a0 ? job.rb : job.ra, lds_w, threadIdx.x & 63u);
+// Conversion items (kConvPerBlock coefficients of one polynomial each, 16
+// groups of 64) run by resident blocks: wave w of block b takes groups
+// b*W + w, + grid*W, ..., and loads the next group's coefficients while it
+// converts the current one; the weights go to LDS once per block.
+__global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, uint32_t first, uint32_t count) {
+    __shared__ ConvLds L;
+    conv_tables_to_lds(tb, L);
+    constexpr uint32_t per_item = kConvPerBlock / kConvPerWave;
+    const uint32_t waves = blockDim.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t total = count * per_item, stride = gridDim.x * waves;
+    uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (threadIdx.x >> 6));
+    if (g >= total) return;
+    struct Where {
+        const uint32_t* coef;
+        uint32_t* res;
+        uint32_t n, c0;
+    };
+    auto where = [&](uint32_t gi) {
+        const Item it = ba.items[first + gi / per_item];
+        const Job job = ba.jobs[it.job];
+        return Where{it.a0 ? job.b : job.a, it.a0 ? job.rb : job.ra, 1u << job.logn,
+                     it.a1 * kConvPerBlock + (gi % per_item) * kConvPerWave};
+    };
+    Where cur = where(g);
+    v4u v0[4], v1[4];
+    conv_load(cur.coef, cur.n, cur.c0, lane, v0, v1);
+    for (;;) {
+        const uint32_t gn = g + stride;
+        const bool more = gn < total;
+        Where nxt = cur;
+        v4u n0[4], n1[4];
+        if (more) {
+            nxt = where(gn);
+            conv_load(nxt.coef, nxt.n, nxt.c0, lane, n0, n1);
+        }
+        if (cur.c0 < cur.n) conv_compute(tb, v0, v1, cur.n, cur.n, cur.c0, cur.res, L, lane);
+        if (!more) break;
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            v0[ct] = n0[ct];
+            v1[ct] = n1[ct];
+        }
+        g = gn;
+        cur = nxt;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_items_conv_scalar(Tables tb, BatchArgs ba, uint32_t first) {
@@ -1020,9 +1124,9 @@ __global__ void __launch_bounds__(kConvSmallBlock) k_ingest_convert(Tables tb, J
         }
         return;
     }
-    __shared__ uint4 lds_w[16 * 2 * 64];
+    __shared__ ConvLds L;
     __shared__ uint32_t s_ok;
-    for (uint32_t i = threadIdx.x; i < tb.quads * 128u; i += blockDim.x) lds_w[i] = tb.conv_w[i];
+    conv_tables_to_lds(tb, L);
     const uint32_t n = 1u << job.logn, per_poly = (n + kConvPerWave - 1) / kConvPerWave;
     const uint32_t groups = 2u * per_poly, waves = blockDim.x >> 6;
     const uint32_t g_first = (blockIdx.x - 1) * waves;
@@ -1056,7 +1160,9 @@ __global__ void __launch_bounds__(kConvSmallBlock) k_ingest_convert(Tables tb, J
     const uint32_t g = __builtin_amdgcn_readfirstlane(g_first + (threadIdx.x >> 6));
     if (!s_ok || g >= groups) return;
     const uint32_t poly = g / per_poly, c0 = (g % per_poly) * kConvPerWave;
-    convert_wave(tb, poly ? job.b : job.a, n, n, c0, poly ? job.rb : job.ra, lds_w, threadIdx.x & 63u);
+    v4u v0[4], v1[4];
+    conv_load_host(poly ? job.b : job.a, n, c0, threadIdx.x & 63u, v0, v1);
+    conv_compute(tb, v0, v1, n, n, c0, poly ? job.rb : job.ra, L, threadIdx.x & 63u);
 }
 
 // Copies chunk `chunk` (chunk_words words, a multiple of 4) of src (device)
@@ -1168,9 +1274,11 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
                         hipStream_t stream) {
     if (!count) return hipSuccess;
     switch (kind & 0xFFu) {
-        case kPhaseConv:
-            hipLaunchKernelGGL(k_items_conv, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
+        case kPhaseConv: {
+            const uint32_t grid = count < 2 * tb.cus ? count : 2 * tb.cus;  // two resident blocks per CU
+            hipLaunchKernelGGL(k_items_conv, dim3(grid), dim3(kBlock), 0, stream, tb, ba, first, count);
             break;
+        }
         case kPhaseConvScalar:
             hipLaunchKernelGGL(k_items_conv_scalar, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
             break;
