@@ -20,6 +20,11 @@ namespace pmd {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+// Buffer resources (V#) for raw buffer loads and stores: 32-bit data format,
+// no swizzle, no stride (gfx9 V# word 3); loads past the range read zero.
+constexpr uint32_t kRawBufferWord3 = 0x00020000u;
+constexpr uint32_t kOutOfRange = 0x80000000u;  // an offset past any buffer range used here
+
 // ---------------------------------------------------------------------------
 // small-modulus primitives (p < 2^30)
 // ---------------------------------------------------------------------------
@@ -276,25 +281,36 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
     const size_t table = size_t(pi) << tb.max_logn;
     uint32_t a[G::E], b[G::E];
     const uint32_t x0 = thr_scatter<TLOG, 0>(tau);
-    const uint32_t* a_base = a_src + x0;
-    const uint32_t* b_base = b_src + x0;
-    // unconditional loads (a padding tile points at a real tile of the same
-    // size and only its stores are skipped): per-element conditions would
-    // turn the loads into a chain of branches
+    // Buffer loads: one descriptor per operand (the base stays in scalar
+    // registers), the thread's offset in one VGPR and each register's offset
+    // as a scalar, so no per-load 64-bit address occupies VGPRs.  They are
+    // issued in the order the first forward stage pairs them (register r with
+    // r + E/2, a then b), so that stage starts as the first loads land.
+    // Unconditional (a padding tile points at a real tile of the same size and
+    // only its stores are skipped).
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(a_src), short(0),
+                                                                        int(G::T * 4u), int(kRawBufferWord3));
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(b_src), short(0),
+                                                                        int(G::T * 4u), int(kRawBufferWord3));
 #pragma unroll
-    for (int r = 0; r < G::E; ++r) {
-        a[r] = a_base[G::reg_scatter(0, r)];
-        b[r] = b_base[G::reg_scatter(0, r)];
+    for (int h = 0; h < G::E / 2; ++h) {
+        const int r0 = h, r1 = h + G::E / 2;
+        a[r0] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r0) * 4u, 0);
+        a[r1] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r1) * 4u, 0);
+        b[r0] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r0) * 4u, 0);
+        b[r1] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r1) * 4u, 0);
     }
     forward_from<TLOG, 0>(a, b, lds, tau, tb.fwd + table, bm, p, p2);
 #pragma unroll
     for (int r = 0; r < G::E; ++r) a[r] = mont_mul(csub(a[r], p2), csub(b[r], p2), p, neg_inv);
     inverse_from<TLOG, G::NPH - 1>(a, lds, tau, tb.inv + table, bm, p, p2);
     const Tw s = tb.scale[logn_full * tb.k + pi];
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, short(0), int(G::T * 4u),
+                                                                        int(kRawBufferWord3));
 #pragma unroll
     for (int r = 0; r < G::E; ++r) {
         const uint32_t t = csub(shoup(a[r], s, p), p);  // [0,2p) -> [0,p)
-        if (valid) dst[x0 + G::reg_scatter(0, r)] = t;
+        if (valid) __builtin_amdgcn_raw_buffer_store_b32(t, rd, x0 * 4u, G::reg_scatter(0, r) * 4u, 0);
     }
 }
 
@@ -341,8 +357,6 @@ typedef __attribute__((address_space(1))) uint32_t* global_store_u32;
 // buffer's range is the c_end coefficients: bytes past the 112-byte
 // coefficient (l >= 48, second load) and coefficients past c_end are read as
 // zero by the hardware range check, without a memory access.
-constexpr uint32_t kRawBufferWord3 = 0x00020000u;  // 32-bit data format, no swizzle (gfx9 V# word 3)
-constexpr uint32_t kOutOfRange = 0x80000000u;       // an offset past any buffer range used here
 
 __device__ __forceinline__ void conv_load(const uint32_t* __restrict__ coef, uint32_t c_end, uint32_t c0,
                                           uint32_t lane, v4u (&v0)[4], v4u (&v1)[4]) {
