@@ -20,6 +20,15 @@ namespace pmd {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+// A pointer the caller knows is the same in every lane of the wave, moved to
+// scalar registers (a buffer descriptor must be wave-uniform).
+template <class T>
+__device__ __forceinline__ T* wave_uniform(T* ptr) {
+    const uint64_t v = reinterpret_cast<uint64_t>(ptr);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<T*>(uint64_t(lo) | (uint64_t(hi) << 32));
+}
+
 // Buffer resources (V#) for raw buffer loads and stores: 32-bit data format,
 // no swizzle, no stride (gfx9 V# word 3); loads past the range read zero.
 constexpr uint32_t kRawBufferWord3 = 0x00020000u;
@@ -281,36 +290,48 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
     const size_t table = size_t(pi) << tb.max_logn;
     uint32_t a[G::E], b[G::E];
     const uint32_t x0 = thr_scatter<TLOG, 0>(tau);
-    // Buffer loads: one descriptor per operand (the base stays in scalar
-    // registers), the thread's offset in one VGPR and each register's offset
-    // as a scalar, so no per-load 64-bit address occupies VGPRs.  They are
-    // issued in the order the first forward stage pairs them (register r with
-    // r + E/2, a then b), so that stage starts as the first loads land.
+    // Loads issued in the order the first forward stage pairs them (register r
+    // with r + E/2, a then b), so that stage starts as the first loads land.
     // Unconditional (a padding tile points at a real tile of the same size and
-    // only its stores are skipped).
-    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(a_src), short(0),
-                                                                        int(G::T * 4u), int(kRawBufferWord3));
-    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(b_src), short(0),
-                                                                        int(G::T * 4u), int(kRawBufferWord3));
+    // only its stores are skipped).  When a tile spans whole waves, buffer
+    // loads: one descriptor per operand in scalar registers, the thread's
+    // offset in one VGPR and each register's offset as a scalar, so no per-load
+    // 64-bit address occupies VGPRs.
+    constexpr bool kWaveTile = G::TPTLOG >= 6;  // else the descriptors below go unused
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(wave_uniform(a_src)), short(0), int(G::T * 4u), int(kRawBufferWord3));
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(wave_uniform(b_src)), short(0), int(G::T * 4u), int(kRawBufferWord3));
 #pragma unroll
     for (int h = 0; h < G::E / 2; ++h) {
         const int r0 = h, r1 = h + G::E / 2;
-        a[r0] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r0) * 4u, 0);
-        a[r1] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r1) * 4u, 0);
-        b[r0] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r0) * 4u, 0);
-        b[r1] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r1) * 4u, 0);
+        if constexpr (kWaveTile) {
+            a[r0] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r0) * 4u, 0);
+            a[r1] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r1) * 4u, 0);
+            b[r0] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r0) * 4u, 0);
+            b[r1] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r1) * 4u, 0);
+        } else {
+            a[r0] = a_src[x0 + G::reg_scatter(0, r0)];
+            a[r1] = a_src[x0 + G::reg_scatter(0, r1)];
+            b[r0] = b_src[x0 + G::reg_scatter(0, r0)];
+            b[r1] = b_src[x0 + G::reg_scatter(0, r1)];
+        }
     }
     forward_from<TLOG, 0>(a, b, lds, tau, tb.fwd + table, bm, p, p2);
 #pragma unroll
     for (int r = 0; r < G::E; ++r) a[r] = mont_mul(csub(a[r], p2), csub(b[r], p2), p, neg_inv);
     inverse_from<TLOG, G::NPH - 1>(a, lds, tau, tb.inv + table, bm, p, p2);
     const Tw s = tb.scale[logn_full * tb.k + pi];
-    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dst, short(0), int(G::T * 4u),
-                                                                        int(kRawBufferWord3));
+    const __amdgpu_buffer_rsrc_t rd =
+        __builtin_amdgcn_make_buffer_rsrc(wave_uniform(dst), short(0), int(G::T * 4u), int(kRawBufferWord3));
 #pragma unroll
     for (int r = 0; r < G::E; ++r) {
         const uint32_t t = csub(shoup(a[r], s, p), p);  // [0,2p) -> [0,p)
-        if (valid) __builtin_amdgcn_raw_buffer_store_b32(t, rd, x0 * 4u, G::reg_scatter(0, r) * 4u, 0);
+        if (!valid) continue;
+        if constexpr (kWaveTile)
+            __builtin_amdgcn_raw_buffer_store_b32(t, rd, x0 * 4u, G::reg_scatter(0, r) * 4u, 0);
+        else
+            dst[x0 + G::reg_scatter(0, r)] = t;
     }
 }
 
