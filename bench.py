@@ -290,6 +290,10 @@ def run_uneven(args):
     sptr = stream.cuda_stream
     inputs = make_inputs(pm, q, ns, args.seed)
     print(f"[bench] uneven inputs ready: {len(ns)} products", file=sys.stderr, flush=True)
+    # per-N batch schedule (outer passes, hence tile size and tiles per block)
+    # from the autotuner, on small batches of each size
+    tuned = pm.autotune_batch(ctx, sorted(set(ns)), count=8, reps=3, stream=stream)
+    print(f"[bench] batch schedule: {tuned}", file=sys.stderr, flush=True)
     results = {}
     outputs = {}
     for mode in ("phased",):
@@ -337,6 +341,9 @@ def run_uneven(args):
     plain_total_ms = sum(counts[n] * float(np.mean(per_n[n])) for n in counts if n in per_n)
     line = {"workload": "batched uneven N (2^12..2^16 log-uniform), one graph launch",
             "products": len(ns), "sizes": {str(n): c for n, c in sorted(counts.items())}, "k_primes": k,
+            "autotuned_batch_outer": {str(n): r["outer"] for n, r in tuned.items()},
+            "autotune_ms_per_launch_of_8": {str(n): {str(u): round(t, 4) for u, t in r["ms"].items()}
+                                            for n, r in tuned.items()},
             "batched": results,
             "plain_one_at_a_time_products_per_s": round(len(ns) / (plain_total_ms * 1e-3), 1),
             "bit_exact_vs_plain": True}

@@ -76,6 +76,8 @@ def library():
     lib.pm_synchronize.argtypes = [P]
     lib.pm_set_outer.argtypes = [P, u32, u32]
     lib.pm_get_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
+    lib.pm_set_batch_outer.argtypes = [P, u32, u32]
+    lib.pm_get_batch_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_stage_convert.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_convert_scalar.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
@@ -106,7 +108,9 @@ def library():
 
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
-            "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_multiply_device", "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_stage_convert",
+            "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_multiply_device",
+            "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
+            "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
             "pm_batch_run", "pm_batch_wait", "pm_batch_info", "pm_plan_worklist", "pm_batch_phase_times",
@@ -260,22 +264,34 @@ class Context:
         self._check("pm_get_outer", self._lib.pm_get_outer(self._h, int(n), ctypes.byref(v)))
         return v.value
 
+    def set_batch_outer(self, n, outer):
+        """Batch schedule: outer radix-2 passes for products of size n in
+        batches and served launches created afterwards."""
+        self._check("pm_set_batch_outer", self._lib.pm_set_batch_outer(self._h, int(n), int(outer)))
+
+    def get_batch_outer(self, n):
+        v = ctypes.c_uint32()
+        self._check("pm_get_batch_outer", self._lib.pm_get_batch_outer(self._h, int(n), ctypes.byref(v)))
+        return v.value
+
     def schedule(self):
         """{n: outer} for every power of two up to max_n."""
         return {1 << lg: self.get_outer(1 << lg) for lg in range(1, self.info()["max_n"].bit_length())}
 
     def load_schedule(self, path):
-        """Apply a schedule written by `autotune_plain` (sizes this context
-        cannot hold are skipped)."""
+        """Apply a schedule written by `autotune_plain` / `autotune_batch`
+        (sizes this context cannot hold are skipped); returns
+        {"outer": {n: u}, "batch_outer": {n: u}} as applied."""
         import json
         with open(path) as f:
-            table = json.load(f)["outer"]
-        applied = {}
-        for n_str, outer in table.items():
-            n = int(n_str)
-            if n <= self.info()["max_n"]:
-                self.set_outer(n, outer)
-                applied[n] = outer
+            doc = json.load(f)
+        applied = {"outer": {}, "batch_outer": {}}
+        for key, setter in (("outer", self.set_outer), ("batch_outer", self.set_batch_outer)):
+            for n_str, outer in doc.get(key, {}).items():
+                n = int(n_str)
+                if n <= self.info()["max_n"]:
+                    setter(n, outer)
+                    applied[key][n] = outer
         return applied
 
     def stage_convert(self, n, a, b, ra, rb, stream=None):
@@ -462,9 +478,68 @@ def autotune_plain(ctx, ns, reps=20, path=None, stream=None):
         ctx.set_outer(n, best)
         result[n] = {"outer": best, "ms": times}
     if path is not None:
-        with open(path, "w") as f:
-            json.dump({"device": torch.cuda.get_device_name(0),
-                       "outer": {str(n): r["outer"] for n, r in result.items()}}, f, indent=1)
+        _save_schedule(path, "outer", {n: r["outer"] for n, r in result.items()})
+    return result
+
+
+def _save_schedule(path, key, table):
+    """Write table {n: outer} under `key` of the schedule file, keeping what
+    the file already holds for the other key."""
+    import json
+    import torch
+    doc = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            doc = json.load(f)
+    doc["device"] = torch.cuda.get_device_name(0)
+    doc.setdefault(key, {}).update({str(n): int(u) for n, u in table.items()})
+    with open(path, "w") as f:
+        json.dump(doc, f, indent=1)
+
+
+def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
+    """Per-N batch schedule: times a graph launch of `count` products of each
+    size with every valid number of outer passes (0..2, tile 2^5..2^15), keeps
+    the fastest in the context (for batches and served launches created
+    afterwards) and returns {n: {"outer": u, "ms": {u: median ms per launch}}};
+    with `path` the choice is also written under "batch_outer" of the
+    schedule file `load_schedule` reads."""
+    import torch
+    from . import workload as wl
+    s = stream if stream is not None else torch.cuda.Stream()
+    result = {}
+    for n in sorted(set(int(x) for x in ns)):
+        logn = n.bit_length() - 1
+        a = wl.poly(n, 0, n, ctx.q)
+        b = wl.poly(n, 1, n, ctx.q)
+        times = {}
+        for outer in (0, 1, 2):
+            if outer > logn or logn - outer > 15 or logn - outer < 5:
+                continue
+            ctx.set_batch_outer(n, outer)
+            batch = Batch(ctx, [n] * count)
+            for i in range(count):
+                batch.upload(i, a, b)
+            with torch.cuda.stream(s):
+                for _ in range(2):
+                    batch.run(s.cuda_stream)
+                batch.wait(s.cuda_stream)
+                ms = []
+                for _ in range(reps):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(s)
+                    batch.run(s.cuda_stream)
+                    e1.record(s)
+                    e1.synchronize()
+                    ms.append(e0.elapsed_time(e1))
+                batch.wait(s.cuda_stream)
+            batch.close()
+            times[outer] = float(np.median(ms))
+        best = min(times, key=times.get)
+        ctx.set_batch_outer(n, best)
+        result[n] = {"outer": best, "ms": times}
+    if path is not None:
+        _save_schedule(path, "batch_outer", {n: r["outer"] for n, r in result.items()})
     return result
 
 

@@ -62,7 +62,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
         const size_t n = size_t(1) << b->logn[i];
         pmd::Job& j = jobs[i];
         j.logn = b->logn[i];
-        j.outer = b->logn[i] > 15 ? b->logn[i] - 15 : 0;
+        j.outer = ctx->batch_outer[b->logn[i]];
         const bool caller_io = d_a && d_b && d_c;
         if (caller_io) {
             j.a = d_a[i];
@@ -297,7 +297,7 @@ int pm_plan_worklist(uint32_t k, uint32_t count, const uint32_t* ns, uint32_t* n
     for (uint32_t i = 0; i < count; ++i) {
         if (ns[i] < 32 || (ns[i] & (ns[i] - 1)) || ns[i] > (1u << 17)) return PM_EUNSUPPORTED;
         jobs[i].logn = uint32_t(__builtin_ctz(ns[i]));
-        jobs[i].outer = jobs[i].logn > 15 ? jobs[i].logn - 15 : 0;
+        jobs[i].outer = pm_default_batch_outer(jobs[i].logn);
     }
     std::vector<pmd::Item> it;
     std::vector<pmd::Unit> un;

@@ -69,6 +69,13 @@ static const T* upload(pm_context* ctx, const std::vector<T>& v, int* status) {
 // blocks) at the cost of the outer passes' HBM traffic
 uint32_t pm_default_outer(uint32_t logn) { return logn <= 13 ? 0u : (logn == 14 ? 1u : 2u); }
 
+// batches have tiles enough to fill the device: the largest tile, 2^15
+uint32_t pm_default_batch_outer(uint32_t logn) { return logn > 15 ? logn - 15 : 0u; }
+
+bool pm_outer_valid(uint32_t logn, uint32_t outer, uint32_t lo) {
+    return outer <= 2 && outer <= logn && logn - outer <= 15 && logn - outer >= lo;
+}
+
 // input chunks of the host-buffer path (the output goes in half as many):
 // the set value, else pieces of at most 2^12 coefficients (~0.46 MB), 2 to
 // 16 of them
@@ -229,7 +236,10 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
             return create_failed(PM_ENOMEM, "host allocation failed");
         }
     }
-    for (uint32_t logn = 0; logn <= pm::kMaxLogN; ++logn) ctx->outer[logn] = pm_default_outer(logn);
+    for (uint32_t logn = 0; logn <= pm::kMaxLogN; ++logn) {
+        ctx->outer[logn] = pm_default_outer(logn);
+        ctx->batch_outer[logn] = pm_default_batch_outer(logn);
+    }
     // one product of size max_n on zero inputs: the runtime loads each kernel
     // at its first launch, which would otherwise land in the first timed call
     {
@@ -496,6 +506,26 @@ int pm_set_outer(pm_context* ctx, uint32_t n, uint32_t outer) {
     if (outer > 2 || outer > logn || logn - outer > 15 || logn - outer < 1)
         return pm_fail(ctx, PM_EUNSUPPORTED, "outer stages must leave a tile of 2..2^15 (outer <= 2)");
     ctx->outer[logn] = outer;
+    return PM_OK;
+}
+
+int pm_set_batch_outer(pm_context* ctx, uint32_t n, uint32_t outer) {
+    if (!ctx) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    if (!pm_outer_valid(logn, outer, 5))
+        return pm_fail(ctx, PM_EUNSUPPORTED, "batch outer stages must leave a tile of 2^5..2^15 (outer <= 2)");
+    ctx->batch_outer[logn] = outer;
+    return PM_OK;
+}
+
+int pm_get_batch_outer(pm_context* ctx, uint32_t n, uint32_t* outer) {
+    if (!ctx || !outer) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    *outer = ctx->batch_outer[logn];
     return PM_OK;
 }
 

@@ -21,8 +21,10 @@ struct pm_context {
     hipStream_t stream = nullptr;
     // plain-path workspace
     uint32_t *d_a = nullptr, *d_b = nullptr, *d_c = nullptr, *d_ra = nullptr, *d_rb = nullptr;
-    // outer radix-2 stages run as separate passes, per log2 N (plain path)
+    // outer radix-2 stages run as separate passes, per log2 N: plain path,
+    // and batches / served launches
     uint32_t outer[pm::kMaxLogN + 1] = {};
+    uint32_t batch_outer[pm::kMaxLogN + 1] = {};
     // host-buffer path: page-locked staging (inputs a|b and the output) with
     // its device views, output chunk events, and the product between submit
     // and collect
@@ -50,4 +52,7 @@ int pm_hip(pm_context* ctx, hipError_t e, const char* what);
 int pm_check_n(pm_context* ctx, uint32_t n, uint32_t* logn);
 int pm_run_plain(pm_context* ctx, const pmd::Job& job, uint32_t outer, hipStream_t stream);
 uint32_t pm_default_outer(uint32_t logn);
+uint32_t pm_default_batch_outer(uint32_t logn);
+// whether `outer` passes leave a tile of size 2^lo..2^15 for products of 2^logn
+bool pm_outer_valid(uint32_t logn, uint32_t outer, uint32_t lo);
 uint32_t pm_io_chunks(const pm_context* ctx, uint32_t logn);

@@ -158,7 +158,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
         uint8_t* base = arena_base + r->offset;
         pmd::Job& job = L.h_jobs[j];
         job.logn = r->logn;
-        job.outer = r->logn > 15 ? r->logn - 15 : 0;
+        job.outer = ctx->batch_outer[r->logn];
         job.a = reinterpret_cast<const uint32_t*>(base);
         job.b = job.a + n * kLimbs;
         job.c = reinterpret_cast<uint32_t*>(base) + 2 * n * kLimbs;

@@ -96,6 +96,12 @@ int pm_synchronize(pm_context* ctx);
  * the autotuner in the Python mirror (`autotune_plain`). */
 int pm_set_outer(pm_context* ctx, uint32_t n, uint32_t outer);
 int pm_get_outer(pm_context* ctx, uint32_t n, uint32_t* outer);
+/* Batch schedule (no counterpart in the reference): outer passes (0..2,
+ * leaving a tile of 2^5..2^15) for products of size n in batches and served
+ * launches created afterwards; default max(0, log2 n - 15) (the largest
+ * tile).  Set by `autotune_batch` in the Python mirror. */
+int pm_set_batch_outer(pm_context* ctx, uint32_t n, uint32_t outer);
+int pm_get_batch_outer(pm_context* ctx, uint32_t n, uint32_t* outer);
 
 /* ---- stage entry points (device buffers, context stream) ------------------
  * The three kernels of the plain path, exposed so each can be checked alone.

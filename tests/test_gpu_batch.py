@@ -113,3 +113,42 @@ def test_batch_rejects_small_n(pm, workload):
     with pytest.raises(pm.PolymulError):
         pm.Batch(ctx, [1024], mode="persistent")  # removed mode is refused, never launched
     ctx.close()
+
+
+def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
+    """Every batch schedule (outer passes leaving a tile of 2^5..2^15) gives
+    the golden product; the batch autotuner keeps the fastest per size and its
+    choice round-trips through the schedule file."""
+    for n, outers in ((32768, (0, 1, 2)), (65536, (1, 2))):
+        case = [c for c in GOLDEN["cases"] if c["n"] == n and c["modulus"].startswith("seed:")][0]
+        q = workload.modulus(int(case["modulus"].split(":")[1]))
+        ctx = pm.Context(q, 1 << 16)
+        a, b = workload.family(case["family"], n, q, case["seed"])
+        for outer in outers:
+            say("batch schedule", n, outer)
+            ctx.set_batch_outer(n, outer)
+            assert ctx.get_batch_outer(n) == outer
+            batch = pm.Batch(ctx, [n] * 3)
+            for i in range(3):
+                batch.upload(i, a, b)
+            batch.run()
+            batch.wait()
+            for i in range(3):
+                assert sha(batch.download(i)) == case["sha256"]["c"], (n, outer, i)
+            batch.close()
+        ctx.close()
+    ctx = pm.Context(workload.modulus(1), 1 << 16)
+    with pytest.raises(pm.PolymulError):
+        ctx.set_batch_outer(32, 1)  # would leave a tile of 2^4
+    with pytest.raises(pm.PolymulError):
+        ctx.set_batch_outer(65536, 0)  # tile of 2^16
+    path = tmp_path / "schedule.json"
+    say("batch autotune")
+    res = pm.autotune_batch(ctx, [4096, 32768], count=4, reps=2, path=str(path))
+    assert set(res) == {4096, 32768}
+    for n, r in res.items():
+        assert r["outer"] == ctx.get_batch_outer(n) and r["outer"] in r["ms"]
+    ctx.set_batch_outer(32768, 2 if res[32768]["outer"] != 2 else 0)
+    applied = ctx.load_schedule(str(path))
+    assert applied["batch_outer"][32768] == res[32768]["outer"] == ctx.get_batch_outer(32768)
+    ctx.close()

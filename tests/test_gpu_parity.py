@@ -292,7 +292,7 @@ def test_plain_schedule_and_autotune(pm, workload, tmp_path):
         assert r["outer"] == ctx.get_outer(n) and r["outer"] in r["ms"]
     ctx.set_outer(32768, 0 if res[32768]["outer"] else 1)
     applied = ctx.load_schedule(str(path))
-    assert applied[32768] == res[32768]["outer"] == ctx.get_outer(32768)
+    assert applied["outer"][32768] == res[32768]["outer"] == ctx.get_outer(32768)
     ctx.close()
 
 
