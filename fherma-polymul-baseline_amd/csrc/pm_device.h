@@ -20,7 +20,10 @@ constexpr uint32_t kCrtPerBlock = 256;
 constexpr uint32_t kCrtWaveCoefs = 64;
 constexpr uint32_t kCrtMfmaWaves = 8;
 constexpr uint32_t kCrtWords = 31;
-constexpr uint32_t kCrtLdsStride = 68;  // words per row (64 + 4: spreads the banks)
+// words per row: a partial-sum store writes words 4r..4r+3 of 16 coefficients,
+// lanes (g, col) at g*stride + col; stride = 16 mod 64 puts the 64 lanes on
+// 64 distinct banks (and each 32-lane half on distinct banks of 32)
+constexpr uint32_t kCrtLdsStride = 80;
 constexpr uint32_t kCrtWaveLds = kCrtWords * kCrtLdsStride * 6;  // bytes: u32 low + i16 high
 
 struct Tw {
