@@ -130,6 +130,17 @@ def run_headline(args):
     # and as the reference's harness times it: the `solution` program calls
     # fherma_init once and times each fherma_run (fresh output vector) alone
     harness_ms, harness_exact = dropin_harness(pm, q, n, inputs[:args.harness_cases], plain_c)
+    # the reference's own headline configuration: q = 2^868 - 11993087
+    # (EXPERIMENTS_RU.md:94), same harness, its result checked against the
+    # device path of a context for that q
+    q_ref = wl.special_modulus(11993087)
+    ref_inputs = [(wl.poly(1000 + i, 0, n, q_ref), wl.poly(1000 + i, 1, n, q_ref))
+                  for i in range(min(args.harness_cases, 8))]
+    ref_ctx = pm.Context(q_ref, n)
+    ref_first = ref_ctx.multiply(*ref_inputs[0])
+    ref_ctx.close()
+    ref_ms, ref_exact = dropin_harness(pm, q_ref, n, ref_inputs * (args.harness_cases // len(ref_inputs) or 1),
+                                       ref_first)
     print(f"[bench] plain: {plain_rate:.0f} products/s, p50 {percentile(plain_ms, 50):.3f} ms, "
           f"host drop-in p50 {percentile(host_ms, 50):.3f} ms, fherma_run p50 {percentile(harness_ms, 50):.3f} ms",
           file=sys.stderr, flush=True)
@@ -197,7 +208,10 @@ def run_headline(args):
             "dropin_fherma_run_p50_ms": round(percentile(harness_ms, 50), 4),
             "dropin_fherma_run_p99_ms": round(percentile(harness_ms, 99), 4),
             "dropin_fherma_run_cases": len(harness_ms),
-            "gpu_paths_agree": gpu_consistent and harness_exact,
+            "dropin_fherma_run_special_q_p50_ms": round(percentile(ref_ms, 50), 4),
+            "dropin_fherma_run_special_q_p99_ms": round(percentile(ref_ms, 99), 4),
+            "special_q": "2^868 - 11993087 (the reference's published configuration)",
+            "gpu_paths_agree": gpu_consistent and harness_exact and ref_exact,
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
