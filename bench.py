@@ -150,7 +150,11 @@ def run_headline(args):
 
     # ---- roofline of the dominant kernel -------------------------------------
     # each phase timed with HIP events on the launch stream (direct launches of
-    # the same kernels the graph replays), median of several runs
+    # the same kernels the graph replays), median of several runs, after a few
+    # untimed launches so the clocks are where the timed steps ran
+    for _ in range(max(3, args.warmup)):
+        batch.run(sptr)
+    batch.wait(sptr)
     runs = [dict(batch.phase_times(sptr)) for _ in range(5)]
     phase_ms = {name: percentile([r[name] for r in runs], 50) for name in runs[0]}
     per_phase = phase_bytes(n, k)

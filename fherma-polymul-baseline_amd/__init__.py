@@ -23,6 +23,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 PM_OK, PM_EINVAL, PM_EUNSUPPORTED, PM_EDEVICE, PM_ENOMEM, PM_EBUDGET, PM_ETIMEOUT = range(7)
+MAX_IO_CHUNKS = 16  # input/output chunks of the host-buffer path at most
 
 
 class PolymulError(RuntimeError):
@@ -72,6 +73,8 @@ def library():
     lib.pm_multiply_submit.argtypes = [P, u32, P, P]
     lib.pm_multiply_collect.argtypes = [P, P]
     lib.pm_set_io_chunks.argtypes = [P, u32, u32]
+    lib.pm_set_io_wait_limit.argtypes = [P, u32]
+    lib.pm_inject_io_stall.argtypes = [P, ctypes.c_int32]
     lib.pm_multiply_device.argtypes = [P, u32, P, P, P, P]
     lib.pm_synchronize.argtypes = [P]
     lib.pm_set_outer.argtypes = [P, u32, u32]
@@ -108,7 +111,8 @@ def library():
 
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
-            "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_multiply_device",
+            "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_set_io_wait_limit",
+            "pm_inject_io_stall", "pm_multiply_device",
             "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
             "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
@@ -243,6 +247,14 @@ class Context:
         self._pending_n = None
         self._check("pm_multiply_collect", self._lib.pm_multiply_collect(self._h, c.ctypes.data))
         return c
+
+    def set_io_wait_limit(self, microseconds):
+        """Bound of one device wait for a host-buffer input chunk (0 = 5 s)."""
+        self._check("pm_set_io_wait_limit", self._lib.pm_set_io_wait_limit(self._h, int(microseconds)))
+
+    def inject_io_stall(self, chunk):
+        """Next submit never raises input chunk `chunk`'s ready flag (failure injection; -1 cancels)."""
+        self._check("pm_inject_io_stall", self._lib.pm_inject_io_stall(self._h, int(chunk)))
 
     def set_io_chunks(self, n, chunks):
         """Chunks of the host-buffer path for products of size n (0 = default)."""

@@ -34,7 +34,7 @@ struct Tw {
 struct Tables {
     const uint32_t* primes;       // [k]
     const uint32_t* mont;         // [k] -p^-1 mod 2^32
-    const Tw* fwd;                // [k][max_n] psi^brv(i)
+    const Tw* fwd;                // [k][max_n] psi^brv(i), value stored negated mod 2^32
     const Tw* inv;                // [k][max_n] psi^-brv(i)
     const Tw* scale;              // [max_logn+1][k]
     const uint4* conv_w;          // [quads][2][64] int8x16 weight operands

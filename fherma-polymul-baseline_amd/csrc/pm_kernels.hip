@@ -50,24 +50,33 @@ __device__ __forceinline__ uint32_t csub(uint32_t x, uint32_t m) {
 // Cooley-Tukey: X in [0,4p), Y any -> both in [0,4p).  The Shoup product
 // t = Y*w mod p in [0,2p) is carried negated, nt = q*p - Y*w (mod 2^32), so
 // the outputs are x - nt and x + 2p + nt: one three-input add instead of an
-// add and a subtract, 8 VALU instructions per butterfly.  The three-input add
+// add and a subtract.  Forward twiddles are stored as w.value = -w mod 2^32,
+// so nt = q*p + Y*w.value is one multiply and one multiply-add (the low half
+// of v_mad_u64_u32): 7 VALU instructions per butterfly.  The three-input add
 // is written out because the compiler, left to itself, reassociates it into
 // code that needs more registers (and spills at 128 VGPRs).
 __device__ __forceinline__ void ct_bfly(uint32_t& X, uint32_t& Y, Tw w, uint32_t p, uint32_t p2) {
     const uint32_t x = csub(X, p2);
     const uint32_t quotient = __umulhi(Y, w.shoup);
-    const uint32_t nt = quotient * p - Y * w.value;
+    const uint32_t nt = quotient * p + Y * w.value;
     uint32_t y;
     asm("v_add3_u32 %0, %1, %2, %3" : "=v"(y) : "v"(x), "v"(p2), "v"(nt));
     X = x - nt;
     Y = y;
 }
-// Gentleman-Sande: X, Y in [0,2p) -> both in [0,2p)
+// Gentleman-Sande: X, Y in [0,2p) -> both in [0,2p).  The Shoup product is
+// formed as d*w + q*(2^32 - p): one multiply and one multiply-add (the low
+// half of v_mad_u64_u32), 8 VALU instructions per butterfly.  2^32 - p is
+// passed through an empty asm so that the optimiser cannot see it is -p and
+// fold the sum back into two multiplies and a subtract.
 __device__ __forceinline__ void gs_bfly(uint32_t& X, uint32_t& Y, Tw w, uint32_t p, uint32_t p2) {
     const uint32_t s = X + Y;
     const uint32_t d = X - Y + p2;
     X = csub(s, p2);
-    Y = shoup(d, w, p);
+    uint32_t np = 0u - p;
+    asm("" : "+v"(np));
+    const uint32_t quotient = __umulhi(d, w.shoup);
+    Y = d * w.value + quotient * np;
 }
 // a*b*2^-32 mod p in [0,2p) for a, b in [0,2p) (Montgomery REDC).
 __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t neg_inv) {
@@ -151,12 +160,15 @@ __device__ __forceinline__ uint32_t thr_scatter(uint32_t tau) {
 }
 
 __device__ __forceinline__ uint32_t padded(uint32_t x) { return x + (x >> 5); }
+constexpr uint32_t kLdsHalf = 16384;  // words: 64 KB, the reach of an LDS immediate offset
 
 // Butterflies between scheduling barriers, which bound the twiddles in flight
 // (VGPR budget of 1024-thread blocks): measured best at 16 for the 2^15 tile,
 // while the smaller tiles spill with more than 8.
 template <int TLOG>
 constexpr int twiddle_chunk() { return TLOG == 15 ? 16 : 8; }
+template <int TLOG>
+constexpr int inverse_twiddle_chunk() { return 2 * twiddle_chunk<TLOG>(); }
 
 // An exchange keeps every element inside its wave when the wave bits of the
 // thread index (thread bits 6 and up) sit on the same index bits in both
@@ -177,12 +189,20 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* 
     using G = Geo<TLOG>;
     constexpr bool kWave = wave_local<TLOG>(PFROM, PTO);
     // thread and register bits are disjoint, so padded(x|c) = padded(x)+padded(c):
-    // one base address per thread plus immediate offsets
-    uint32_t* wbase = lds + padded(thr_scatter<TLOG, PFROM>(tau));
-    uint32_t* rbase = lds + padded(thr_scatter<TLOG, PTO>(tau));
+    // one base address per thread plus immediate offsets.  An LDS immediate
+    // offset is below 64 KB, so slots above that use a second base 64 KB up,
+    // formed in this exchange (opaque to the optimiser): left to itself the
+    // compiler gives each such slot an address register of its own, shared
+    // by every exchange of the same layout, and spills them across the tile.
+    const uint32_t wx = padded(thr_scatter<TLOG, PFROM>(tau)), rx = padded(thr_scatter<TLOG, PTO>(tau));
+    uint32_t wx_hi = wx + kLdsHalf, rx_hi = rx + kLdsHalf;
+    if constexpr (G::STRIDE > kLdsHalf) asm volatile("" : "+v"(wx_hi), "+v"(rx_hi));
+    const auto slot = [&](uint32_t x, uint32_t x_hi, uint32_t c) -> uint32_t& {
+        return c < kLdsHalf ? lds[x + c] : lds[x_hi + (c - kLdsHalf)];
+    };
     if constexpr (LEAD && !kWave) __syncthreads();
 #pragma unroll
-    for (int r = 0; r < G::E; ++r) wbase[padded(G::reg_scatter(PFROM, r))] = v[r];
+    for (int r = 0; r < G::E; ++r) slot(wx, wx_hi, padded(G::reg_scatter(PFROM, r))) = v[r];
     if constexpr (kWave) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -191,7 +211,7 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* 
         __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < G::E; ++r) v[r] = rbase[padded(G::reg_scatter(PTO, r))];
+    for (int r = 0; r < G::E; ++r) v[r] = slot(rx, rx_hi, padded(G::reg_scatter(PTO, r)));
     if constexpr (kWave) {
         __builtin_amdgcn_wave_barrier();
     } else {
@@ -241,6 +261,13 @@ template <int TLOG, int P>
 __device__ __forceinline__ void inv_phase(uint32_t (&a)[Geo<TLOG>::E], const Tw* __restrict__ tw,
                                           uint32_t bm, uint32_t tau_x, uint32_t p, uint32_t p2) {
     using G = Geo<TLOG>;
+    // the inverse table is indexed like the forward one: without this the
+    // compiler keeps the forward phase's twiddle offsets live across the tile
+    // (and spills them) instead of recomputing them here
+    if constexpr (G::TPTLOG >= 6)
+        asm volatile("" : "+v"(tau_x));  // bm is scalar here (conv_tile)
+    else
+        asm volatile("" : "+v"(tau_x), "+v"(bm));
 #pragma unroll
     for (int bit = 0; bit < G::width(P); ++bit) {
         const int beta = G::lo(P) + bit;
@@ -251,7 +278,7 @@ __device__ __forceinline__ void inv_phase(uint32_t (&a)[Geo<TLOG>::E], const Tw*
             if (r & (1 << bit)) continue;
             const Tw w = tw[bm * m + group_of<TLOG, P>(tau_x, r, beta)];
             gs_bfly(a[r], a[r | (1 << bit)], w, p, p2);
-            if (++done % (2 * twiddle_chunk<TLOG>()) == 0) __builtin_amdgcn_sched_barrier(0);
+            if (++done % inverse_twiddle_chunk<TLOG>() == 0) __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -293,7 +320,13 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
                                           uint32_t prime, uint32_t bm, uint32_t logn_full,
                                           uint32_t* lds, uint32_t tau, bool valid) {
     using G = Geo<TLOG>;
-    const uint32_t pi = valid ? prime : 0u;
+    uint32_t pi = valid ? prime : 0u;
+    if constexpr (G::TPTLOG >= 6) {
+        // a tile that spans whole waves has one prime and sub-block per wave:
+        // scalar registers for them and all that follows (moduli, table bases)
+        pi = __builtin_amdgcn_readfirstlane(pi);
+        bm = __builtin_amdgcn_readfirstlane(bm);
+    }
     const uint32_t p = tb.primes[pi], p2 = 2u * p, neg_inv = tb.mont[pi];
     const size_t table = size_t(pi) << tb.max_logn;
     uint32_t a[G::E], b[G::E];

@@ -87,6 +87,7 @@ uint32_t pm_io_chunks(const pm_context* ctx, uint32_t logn) {
 }
 
 constexpr int kCollectTimeoutSeconds = 60;
+constexpr uint64_t kDefaultIoWaitMs = 5000;  // bound of one device wait on a host flag
 
 extern "C" {
 
@@ -141,7 +142,8 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     std::vector<pmd::Tw> fwd(s.forward.size()), inv(s.inverse.size()), scale(s.scale.size()),
         limb_pow(s.limb_pow.size());
     for (size_t i = 0; i < limb_pow.size(); ++i) limb_pow[i] = {s.limb_pow[i].value, s.limb_pow[i].shoup};
-    for (size_t i = 0; i < fwd.size(); ++i) fwd[i] = {s.forward[i].value, s.forward[i].shoup};
+    // forward twiddles carry -w mod 2^32: the butterfly forms q*p - Y*w as q*p + Y*(-w)
+    for (size_t i = 0; i < fwd.size(); ++i) fwd[i] = {0u - s.forward[i].value, s.forward[i].shoup};
     for (size_t i = 0; i < inv.size(); ++i) inv[i] = {s.inverse[i].value, s.inverse[i].shoup};
     for (size_t i = 0; i < scale.size(); ++i) scale[i] = {s.scale[i].value, s.scale[i].shoup};
 
@@ -226,7 +228,8 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     int rate_khz = 0;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
         rate_khz = 100000;
-    ctx->wait_ticks = uint64_t(rate_khz) * 1000u * 5u;  // 5 s per wait on a host flag
+    ctx->clock_khz = uint64_t(rate_khz);
+    ctx->wait_ticks = ctx->clock_khz * kDefaultIoWaitMs;
     {
         const unsigned cores = std::thread::hardware_concurrency();
         const unsigned workers = cores > 4 ? 3u : (cores > 1 ? cores - 1 : 0u);
@@ -395,9 +398,11 @@ int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uin
         PmCopyPool::Chunk chunks_of_input[pmd::kMaxIoChunks];
         for (uint32_t j = 0; j < in_chunks; ++j) {
             const size_t off = j * span_words;
+            volatile uint32_t* flag = int32_t(j) == ctx->stall_chunk ? &ctx->stall_sink : &ready_flags(ctx)[j];
             chunks_of_input[j] = {ctx->h_in + off, off < words ? a + off : b + (off - words),
-                                  span_words * sizeof(uint32_t), &ready_flags(ctx)[j], ctx->io_seq};
+                                  span_words * sizeof(uint32_t), flag, ctx->io_seq};
         }
+        ctx->stall_chunk = -1;
         ctx->copy_pool->run(chunks_of_input, in_chunks);  // returns once a and b are no longer needed
     }
     if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, ctx->outer[logn], ctx->stream);
@@ -445,6 +450,18 @@ int pm_multiply_collect(pm_context* ctx, uint32_t* c) {
         (void)hipStreamSynchronize(ctx->stream);
         return pm_fail(ctx, PM_ETIMEOUT, "a wait for an input chunk gave up");
     }
+    return PM_OK;
+}
+
+int pm_set_io_wait_limit(pm_context* ctx, uint32_t microseconds) {
+    if (!ctx) return PM_EINVAL;
+    ctx->wait_ticks = microseconds ? ctx->clock_khz * microseconds / 1000u : ctx->clock_khz * kDefaultIoWaitMs;
+    return PM_OK;
+}
+
+int pm_inject_io_stall(pm_context* ctx, int32_t chunk) {
+    if (!ctx || chunk < -1 || chunk >= int32_t(pmd::kMaxIoChunks)) return PM_EINVAL;
+    ctx->stall_chunk = chunk;
     return PM_OK;
 }
 

@@ -83,6 +83,16 @@ int pm_multiply_collect(pm_context* ctx, uint32_t* c);
  * coefficients, 2..16 of them); otherwise a power of two <= 16 with
  * n / chunks >= 64. */
 int pm_set_io_chunks(pm_context* ctx, uint32_t n, uint32_t chunks);
+/* Bound of one device-side wait for a host-buffer input chunk (no
+ * counterpart in the reference): 0 restores the default of 5 s.  A wait that
+ * runs out latches a timeout; the product still drains, and collect returns
+ * PM_ETIMEOUT. */
+int pm_set_io_wait_limit(pm_context* ctx, uint32_t microseconds);
+/* Failure injection for tests (no counterpart in the reference): the next
+ * pm_multiply_submit copies input chunk `chunk` but never raises its ready
+ * flag, as a host that stalls would; -1 cancels.  The device wait for it runs
+ * out after the wait limit and collect returns PM_ETIMEOUT.  One-shot. */
+int pm_inject_io_stall(pm_context* ctx, int32_t chunk);
 /* Same product with device-resident a, b, c on a HIP stream (0 = the
  * context's stream); returns once the work is enqueued. */
 int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,

@@ -339,3 +339,34 @@ def test_host_path_chunks_and_split(pm, workload, torch_cuda):
     assert lib.pm_multiply_submit(ctx.handle, a.shape[0], a.ctypes.data, b.ctypes.data) == 1  # one pending
     assert (ctx.collect() == want[1][2]).all()
     ctx.close()
+
+
+def test_host_path_stall_times_out_and_recovers(pm, workload, torch_cuda):
+    """A host that never raises one input chunk's ready flag (injected from the
+    host, so every kernel still ends normally): the device wait for it runs out
+    after the wait limit, collect reports PM_ETIMEOUT, and the next products on
+    the same context are exact again."""
+    q = workload.modulus(29)
+    ctx = pm.Context(q, 1 << 15)
+    n = 1 << 15
+    a, b = workload.family("dense-random-1", n, q, 3)
+    ctx.set_io_chunks(n, 8)  # a|b in 8 input chunks
+    want = ctx.multiply(a, b)
+    with pytest.raises(pm.PolymulError):
+        ctx.inject_io_stall(pm.MAX_IO_CHUNKS)
+    for stall in (0, 5, 7):
+        say("stall input chunk", stall)
+        ctx.set_io_wait_limit(2000)  # 2 ms
+        ctx.inject_io_stall(stall)
+        ctx.submit(a, b)
+        with pytest.raises(pm.PolymulError) as err:
+            ctx.collect()
+        assert err.value.status == pm.PM_ETIMEOUT, err.value
+        ctx.set_io_wait_limit(0)  # the default bound again
+        assert (ctx.multiply(a, b) == want).all(), stall
+        a2, b2 = workload.family("dense-random-1", n, q, 4)
+        c2 = torch_cuda.zeros(n * 28, dtype=torch_cuda.int32, device="cuda")
+        ctx.multiply_device(n, to_dev(torch_cuda, a2), to_dev(torch_cuda, b2), c2)
+        ctx.synchronize()
+        assert (ctx.multiply(a2, b2) == to_host(c2).reshape(n, 28)).all(), stall
+    ctx.close()
