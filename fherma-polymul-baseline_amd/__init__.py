@@ -75,6 +75,7 @@ def library():
     lib.pm_set_io_chunks.argtypes = [P, u32, u32]
     lib.pm_set_io_wait_limit.argtypes = [P, u32]
     lib.pm_inject_io_stall.argtypes = [P, ctypes.c_int32]
+    lib.pm_place_near_device.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.pm_multiply_device.argtypes = [P, u32, P, P, P, P]
     lib.pm_synchronize.argtypes = [P]
     lib.pm_set_outer.argtypes = [P, u32, u32]
@@ -112,7 +113,7 @@ def library():
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
             "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_set_io_wait_limit",
-            "pm_inject_io_stall", "pm_multiply_device",
+            "pm_inject_io_stall", "pm_place_near_device", "pm_multiply_device",
             "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
             "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
@@ -130,6 +131,15 @@ def _ptr(x):
     if hasattr(x, "data_ptr"):
         return x.data_ptr()
     return x.ctypes.data
+
+
+def place_near_device(device=0):
+    """Restricts the calling thread to the CPUs of the device's NUMA node
+    (pm_place_near_device); returns (status, node, cpus the thread may use)."""
+    lib = library()
+    node, cpus = ctypes.c_int(-1), ctypes.c_int(0)
+    status = lib.pm_place_near_device(int(device), ctypes.byref(node), ctypes.byref(cpus))
+    return status, node.value, cpus.value
 
 
 def plan_rns(q, max_n):

@@ -12,6 +12,7 @@
 #include <string>
 
 #include "fherma.h"
+#include "pm_placement.h"
 #include "polymul.h"
 
 namespace {
@@ -27,6 +28,9 @@ void* fherma_init(const fherma::Point& p) {
     if (p.N < 2 || (p.N & (p.N - 1)) || p.N > 131072 || p.W != PM_WIDTH || p.L != PM_LIMBS ||
         p.q.data.size() != PM_LIMBS)
         throw std::runtime_error("coverage: power-of-two 2<=N<=131072, W=868, L=28");
+    // the caller on the GPU's NUMA node before staging is allocated, as the
+    // reference's init does (host_affinity.h:14-46); unknown topology: as is
+    if (pm_placement_enabled()) (void)pm_place_near_device(0, nullptr, nullptr);
     pm_context* ctx = nullptr;
     const int status = pm_create(p.N, p.W, p.L, p.q.data.data(), 0, &ctx);
     if (status != PM_OK) raise("fherma_init", nullptr, status);

@@ -1,6 +1,8 @@
 // Host copy pool of the host-buffer path (see pm_copy_pool.h).
 #include "pm_copy_pool.h"
 
+#include "pm_placement.h"
+
 #include <immintrin.h>
 
 #include <chrono>
@@ -14,10 +16,11 @@ constexpr auto kSpin = std::chrono::microseconds(200);
 
 }  // namespace
 
-PmCopyPool::PmCopyPool(unsigned workers) {
+PmCopyPool::PmCopyPool(unsigned workers, const std::vector<int>& cpus) {
     for (unsigned i = 0; i < workers; ++i) {
+        const int cpu = cpus.empty() ? -1 : cpus[i % cpus.size()];
         try {
-            threads_.emplace_back([this] { worker_loop(); });
+            threads_.emplace_back([this, cpu] { worker_loop(cpu); });
         } catch (...) {
             break;  // fewer workers: the caller copies the rest
         }
@@ -46,7 +49,8 @@ void PmCopyPool::drain() {
     }
 }
 
-void PmCopyPool::worker_loop() {
+void PmCopyPool::worker_loop(int cpu) {
+    if (cpu >= 0) (void)pm_pin_thread({cpu});  // best effort: unpinned if refused
     uint64_t seen = 0;
     for (;;) {
         // spin a little, then sleep until the next generation (or stop)

@@ -25,7 +25,8 @@ public:
     };
     static constexpr uint32_t kMaxChunks = 64;
 
-    explicit PmCopyPool(unsigned workers);
+    // Worker i is pinned to cpus[i % cpus.size()] (no pinning when empty).
+    explicit PmCopyPool(unsigned workers, const std::vector<int>& cpus = {});
     ~PmCopyPool();
     PmCopyPool(const PmCopyPool&) = delete;
     PmCopyPool& operator=(const PmCopyPool&) = delete;
@@ -36,7 +37,7 @@ public:
     unsigned workers() const { return unsigned(threads_.size()); }
 
 private:
-    void worker_loop();
+    void worker_loop(int cpu);
     void drain();
 
     std::vector<std::thread> threads_;

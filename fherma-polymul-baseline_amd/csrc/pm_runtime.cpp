@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <immintrin.h>
+#include <sched.h>
 
 #include <chrono>
 #include <cstring>
@@ -14,6 +15,7 @@
 #include "pm_device.h"
 #include "pm_kernels.h"
 #include "pm_params.h"
+#include "pm_placement.h"
 #include "pm_runtime.h"
 #include "polymul.h"
 
@@ -231,9 +233,19 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     ctx->clock_khz = uint64_t(rate_khz);
     ctx->wait_ticks = ctx->clock_khz * kDefaultIoWaitMs;
     {
-        const unsigned cores = std::thread::hardware_concurrency();
+        // copy workers on the device's NUMA node, one per physical core, off
+        // the creating thread's core (the thread that will call submit)
+        std::vector<int> cpus;
+        int node = -1;
+        if (pm_placement_enabled()) {
+            char bus[64] = {};
+            if (hipDeviceGetPCIBusId(bus, sizeof(bus), device) == hipSuccess) node = pm_pci_numa_node(bus);
+            cpus = pm_worker_order(pm_allowed_cpus(node), sched_getcpu());
+        }
+        ctx->numa_node = node;
+        const unsigned cores = cpus.empty() ? std::thread::hardware_concurrency() : unsigned(cpus.size());
         const unsigned workers = cores > 4 ? 3u : (cores > 1 ? cores - 1 : 0u);
-        ctx->copy_pool.reset(new (std::nothrow) PmCopyPool(workers));
+        ctx->copy_pool.reset(new (std::nothrow) PmCopyPool(workers, cpus));
         if (!ctx->copy_pool) {
             pm_destroy(ctx);
             return create_failed(PM_ENOMEM, "host allocation failed");
@@ -457,6 +469,20 @@ int pm_set_io_wait_limit(pm_context* ctx, uint32_t microseconds) {
     if (!ctx) return PM_EINVAL;
     ctx->wait_ticks = microseconds ? ctx->clock_khz * microseconds / 1000u : ctx->clock_khz * kDefaultIoWaitMs;
     return PM_OK;
+}
+
+int pm_place_near_device(int device, int* node, int* cpus_on_node) {
+    if (node) *node = -1;
+    if (cpus_on_node) *cpus_on_node = 0;
+    char bus[64] = {};
+    if (hipDeviceGetPCIBusId(bus, sizeof(bus), device) != hipSuccess) return PM_EDEVICE;
+    const int numa = pm_pci_numa_node(bus);
+    if (node) *node = numa;
+    if (numa < 0) return PM_EUNSUPPORTED;
+    std::vector<int> cpus;
+    for (const PmCpu& c : pm_allowed_cpus(numa)) cpus.push_back(c.cpu);
+    if (cpus_on_node) *cpus_on_node = int(cpus.size());
+    return pm_pin_thread(cpus) ? PM_OK : PM_EUNSUPPORTED;
 }
 
 int pm_inject_io_stall(pm_context* ctx, int32_t chunk) {

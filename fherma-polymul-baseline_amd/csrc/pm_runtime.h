@@ -39,6 +39,7 @@ struct pm_context {
     uint32_t io_seq = 0;
     uint64_t wait_ticks = 0;        // bound of one device wait on a host flag
     uint64_t clock_khz = 100000;    // device wall clock (ticks per ms)
+    int numa_node = -1;             // NUMA node of the device (-1: unknown)
     int32_t stall_chunk = -1;       // input chunk whose ready flag the next submit withholds
     uint32_t stall_sink = 0;        // where that flag goes instead
     std::unique_ptr<PmCopyPool> copy_pool;  // host copies of the input chunks

@@ -88,6 +88,15 @@ int pm_set_io_chunks(pm_context* ctx, uint32_t n, uint32_t chunks);
  * runs out latches a timeout; the product still drains, and collect returns
  * PM_ETIMEOUT. */
 int pm_set_io_wait_limit(pm_context* ctx, uint32_t microseconds);
+/* Host placement (reference host_affinity.h:14-46, pin_near_gpu): restricts
+ * the calling thread to the CPUs of `device`'s NUMA node that the process may
+ * use, so that page-locked staging allocated afterwards (pm_create) and the
+ * caller's copies stay on the GPU's node.  *node and *cpus_on_node report the
+ * node and how many CPUs the thread may now use (either may be NULL).
+ * PM_EUNSUPPORTED, placement unchanged, when the topology is unknown.
+ * pm_create pins its copy workers on that node by itself;
+ * PM_HOST_PLACEMENT=0 in the environment turns both off. */
+int pm_place_near_device(int device, int* node, int* cpus_on_node);
 /* Failure injection for tests (no counterpart in the reference): the next
  * pm_multiply_submit copies input chunk `chunk` but never raises its ready
  * flag, as a host that stalls would; -1 cancels.  The device wait for it runs
