@@ -254,28 +254,13 @@ def dropin_harness(pm, q, n, cases, first_c):
     """Runs the `solution` harness (fherma_init once, each fherma_run timed
     alone, as main.cpp:181-186 does) over a point directory of these cases;
     returns the per-case milliseconds and whether case 0 equals first_c."""
-    import subprocess
     import tempfile
     with tempfile.TemporaryDirectory(prefix="polymul_point_") as d:
-        os.makedirs(os.path.join(d, "point"))
-        with open(os.path.join(d, "point", "q.bin"), "wb") as f:
-            f.write(np.asarray(q, dtype="<u4").tobytes())
-        for i, (a, b) in enumerate(cases):
-            cd = os.path.join(d, "cases", f"{i:06d}")
-            os.makedirs(cd)
-            with open(os.path.join(cd, "a.bin"), "wb") as f:
-                f.write(a.astype("<u4").tobytes())
-            with open(os.path.join(cd, "b.bin"), "wb") as f:
-                f.write(b.astype("<u4").tobytes())
-        with open(os.path.join(d, "manifest.json"), "w") as f:
-            json.dump({"point": {"N": n, "W": 868, "L": 28}, "cases": len(cases)}, f)
-        subprocess.run([os.path.join(PKG_DIR, "solution"), d], check=True, timeout=300,
-                       stdout=subprocess.DEVNULL)
-        report = json.load(open(os.path.join(d, "out", "results.json")))
+        pm.pointdir.write_point(d, q, n, cases)
+        report = pm.pointdir.run_solution(d, timeout=300)
         ms = [c["seconds"] * 1e3 for c in report["cases"] if c["status"] == "ok"]
-        with open(os.path.join(d, "out", "000000", "c.bin"), "rb") as f:
-            c0 = np.frombuffer(f.read(), dtype="<u4").reshape(n, 28)
-    return ms, bool(len(ms) == len(cases) and np.array_equal(c0, first_c))
+        c0 = pm.pointdir.read_output(d, 0, n)
+    return ms, bool(len(ms) == len(cases) and c0 is not None and np.array_equal(c0, first_c))
 
 
 def make_inputs(pm, q, ns, seed, pinned=False):

@@ -15,6 +15,7 @@ import os
 
 import numpy as np
 
+from . import pointdir  # noqa: F401  (the solution harness's point directories)
 from . import workload  # noqa: F401  (seeded synthetic inputs)
 
 LIMBS = 28
