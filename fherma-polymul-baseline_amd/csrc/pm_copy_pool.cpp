@@ -43,8 +43,10 @@ void PmCopyPool::drain() {
         if (j >= kClosed || j >= count_.load(std::memory_order_relaxed)) return;
         const Chunk& c = chunks_[j];
         std::memcpy(c.dst, c.src, c.bytes);
-        _mm_sfence();  // the copy (possibly streaming stores) lands before the flag
-        __atomic_store_n(c.flag, c.value, __ATOMIC_RELEASE);
+        if (c.flag) {
+            _mm_sfence();  // the copy (possibly streaming stores) lands before the flag
+            __atomic_store_n(c.flag, c.value, __ATOMIC_RELEASE);
+        }
         finished_.fetch_add(1, std::memory_order_acq_rel);
     }
 }

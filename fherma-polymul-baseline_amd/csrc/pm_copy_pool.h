@@ -20,7 +20,7 @@ public:
         void* dst;
         const void* src;
         size_t bytes;
-        volatile uint32_t* flag;  // set to `value` once the chunk is in place
+        volatile uint32_t* flag;  // set to `value` once the chunk is in place (none if null)
         uint32_t value;
     };
     static constexpr uint32_t kMaxChunks = 64;

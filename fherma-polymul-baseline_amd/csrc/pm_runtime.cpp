@@ -379,6 +379,29 @@ volatile uint32_t* ready_flags(pm_context* ctx) { return ctx->h_flags; }
 volatile uint32_t* done_flags(pm_context* ctx) { return ctx->h_flags + pmd::kMaxIoChunks; }
 volatile uint32_t* error_flag(pm_context* ctx) { return ctx->h_flags + 2 * pmd::kMaxIoChunks; }
 
+// One result chunk out of staging into the caller's array, split over the copy
+// workers and this thread: the next chunk is still crossing PCIe meanwhile, so
+// only the last chunk's copy is exposed, and that at several cores' rate.
+void copy_out(pm_context* ctx, uint32_t* dst, const uint32_t* src, size_t bytes) {
+    constexpr size_t kMinPiece = size_t(64) << 10;
+    const size_t parts_max = size_t(ctx->copy_pool->workers()) + 1;
+    size_t parts = bytes / kMinPiece;
+    parts = parts < 1 ? 1 : (parts > parts_max ? parts_max : parts);
+    if (parts == 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    PmCopyPool::Chunk pieces[PmCopyPool::kMaxChunks];
+    const size_t piece = (bytes / parts + 63) & ~size_t(63);
+    uint32_t count = 0;
+    for (size_t off = 0; off < bytes && count < PmCopyPool::kMaxChunks; off += piece, ++count) {
+        const size_t len = bytes - off < piece ? bytes - off : piece;
+        pieces[count] = {reinterpret_cast<char*>(dst) + off, reinterpret_cast<const char*>(src) + off, len, nullptr,
+                         0};
+    }
+    ctx->copy_pool->run(pieces, count);
+}
+
 }  // namespace
 
 int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b) {
@@ -456,7 +479,7 @@ int pm_multiply_collect(pm_context* ctx, uint32_t* c) {
                 return pm_fail(ctx, PM_ETIMEOUT, "product did not complete within the time limit");
             }
         }
-        std::memcpy(c + j * chunk_words, ctx->h_out + j * chunk_words, chunk_words * sizeof(uint32_t));
+        copy_out(ctx, c + j * chunk_words, ctx->h_out + j * chunk_words, chunk_words * sizeof(uint32_t));
     }
     if (*error_flag(ctx) != 0) {
         (void)hipStreamSynchronize(ctx->stream);
