@@ -18,7 +18,10 @@ constexpr uint32_t kCrtPerBlock = 256;
 // MFMA CRT: groups of 64 coefficients per wave, 8 waves per block; per wave
 // the LDS holds 31 words (low 32 bits + signed high 16 bits) per coefficient
 constexpr uint32_t kCrtWaveCoefs = 64;
-constexpr uint32_t kCrtMfmaWaves = 8;
+constexpr uint32_t kCrtMfmaWaves = 8;   // waves per block of the batched CRT
+// waves per block of the single-product CRT: one product has few groups (512
+// at N = 2^15), and blocks of 4 spread them over twice as many CUs
+constexpr uint32_t kCrtPlainWaves = 4;
 constexpr uint32_t kCrtWords = 31;
 // words per row: a partial-sum store writes words 4r..4r+3 of 16 coefficients,
 // lanes (g, col) at g*stride + col; stride = 16 mod 64 puts the 64 lanes on

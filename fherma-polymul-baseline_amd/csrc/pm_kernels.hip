@@ -1075,10 +1075,10 @@ __device__ __forceinline__ void crt_group_loop(const Tables& tb, uint32_t groups
 // coefficients [c_first, c_end) with c_first a multiple of 64 and c_end a
 // multiple of 64 or n; job.c may be device memory or mapped page-locked host
 // memory
-__global__ void __launch_bounds__(kCrtMfmaWaves * 64) k_crt_mfma(Tables tb, Job job, uint32_t c_first,
-                                                                 uint32_t c_end) {
+__global__ void __launch_bounds__(kCrtPlainWaves * 64) k_crt_mfma(Tables tb, Job job, uint32_t c_first,
+                                                                  uint32_t c_end) {
     const uint32_t n = 1u << job.logn;
-    crt_group_loop<kCrtMfmaWaves>(tb, (c_end - c_first + kCrtWaveCoefs - 1) / kCrtWaveCoefs,
+    crt_group_loop<kCrtPlainWaves>(tb, (c_end - c_first + kCrtWaveCoefs - 1) / kCrtWaveCoefs,
                                   [&](uint32_t gi, const uint32_t*& t, uint32_t*& out, uint32_t& nn, uint32_t& c0) {
                                       t = job.ra;
                                       out = job.c;
@@ -1434,10 +1434,10 @@ hipError_t launch_crt_range(const Tables& tb, const Job& job, uint32_t c_first, 
     if (c_first >= c_end || c_end > n || c_first % kCrtWaveCoefs || (c_end != n && c_end % kCrtWaveCoefs))
         return hipErrorInvalidValue;
     const uint32_t groups = (c_end - c_first + kCrtWaveCoefs - 1) / kCrtWaveCoefs;
-    uint32_t grid = (groups + kCrtMfmaWaves - 1) / kCrtMfmaWaves;
+    uint32_t grid = (groups + kCrtPlainWaves - 1) / kCrtPlainWaves;
     grid = grid < tb.cus ? grid : tb.cus;
-    hipLaunchKernelGGL(k_crt_mfma, dim3(grid), dim3(kCrtMfmaWaves * 64), crt_block_lds<kCrtMfmaWaves>(), stream, tb,
-                       job, c_first, c_end);
+    hipLaunchKernelGGL(k_crt_mfma, dim3(grid), dim3(kCrtPlainWaves * 64), crt_block_lds<kCrtPlainWaves>(), stream,
+                       tb, job, c_first, c_end);
     return hipGetLastError();
 }
 
@@ -1465,7 +1465,7 @@ hipError_t configure_kernels() {
     hipError_t e = hipSuccess;
     if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_crt_mfma),
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 int(crt_block_lds<kCrtMfmaWaves>()))) != hipSuccess)
+                                 int(crt_block_lds<kCrtPlainWaves>()))) != hipSuccess)
         return e;
     if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_crt_mfma),
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
