@@ -219,6 +219,16 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* 
     }
 }
 
+// Twiddle tables are read-only while any kernel runs: read through the
+// constant address space, so that uniform indices become scalar loads (the
+// compiler cannot otherwise rule out that a table aliases the tile's own
+// stores) and hold no vector registers.
+typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+__device__ __forceinline__ Tw const_twiddle(const Tw* tw, uint32_t i) {
+    const ConstWords e = (ConstWords)(tw + i);
+    return Tw{e[0], e[1]};
+}
+
 // Group index (index bits above beta) of register r in phase P.  When every
 // thread bit of the phase lies below beta+1 the group -- hence the twiddle --
 // is the same for all threads, which lets the compiler use scalar loads.
@@ -249,7 +259,7 @@ __device__ __forceinline__ void fwd_phase2(uint32_t (&a)[Geo<TLOG>::E], uint32_t
 #pragma unroll
         for (int r = 0; r < G::E; ++r) {
             if (r & (1 << bit)) continue;
-            const Tw w = tw[bm * m + group_of<TLOG, P>(tau_x, r, beta)];
+            const Tw w = const_twiddle(tw, bm * m + group_of<TLOG, P>(tau_x, r, beta));
             ct_bfly(a[r], a[r | (1 << bit)], w, p, p2);
             ct_bfly(b[r], b[r | (1 << bit)], w, p, p2);
             if (++done % twiddle_chunk<TLOG>() == 0) __builtin_amdgcn_sched_barrier(0);
@@ -276,7 +286,7 @@ __device__ __forceinline__ void inv_phase(uint32_t (&a)[Geo<TLOG>::E], const Tw*
 #pragma unroll
         for (int r = 0; r < G::E; ++r) {
             if (r & (1 << bit)) continue;
-            const Tw w = tw[bm * m + group_of<TLOG, P>(tau_x, r, beta)];
+            const Tw w = const_twiddle(tw, bm * m + group_of<TLOG, P>(tau_x, r, beta));
             gs_bfly(a[r], a[r | (1 << bit)], w, p, p2);
             if (++done % inverse_twiddle_chunk<TLOG>() == 0) __builtin_amdgcn_sched_barrier(0);
         }
