@@ -336,7 +336,7 @@ class Context:
 
 
 class Batch:
-    """Products of uneven N in one persistent, graph-captured launch."""
+    """Products of uneven N in one captured hipGraph (one launch per phase)."""
 
     PERSISTENT, PHASED, SCALAR_CONVERSION, SCALAR_CRT = 1, 2, 4, 8
 

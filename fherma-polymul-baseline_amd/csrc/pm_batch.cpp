@@ -90,13 +90,10 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     auto* d_jobs = static_cast<pmd::Job*>(b->alloc(jobs.size() * sizeof(pmd::Job)));
     auto* d_items = static_cast<pmd::Item*>(b->alloc(items.size() * sizeof(pmd::Item)));
     auto* d_units = static_cast<pmd::Unit*>(b->alloc(units.size() * sizeof(pmd::Unit)));
-    b->reset_count = 2 + 4 * count;
-    b->d_words = static_cast<uint32_t*>(b->alloc(size_t(b->reset_count) * sizeof(uint32_t)));
-    if (!d_jobs || !d_items || !d_units || !b->d_words) return pm_fail(ctx, PM_ENOMEM, "batch table allocation failed");
+    if (!d_jobs || !d_items || !d_units) return pm_fail(ctx, PM_ENOMEM, "batch table allocation failed");
     hipError_t e = hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(pmd::Job), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_items, items.data(), items.size() * sizeof(pmd::Item), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_units, units.data(), units.size() * sizeof(pmd::Unit), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(b->d_words, 0, size_t(b->reset_count) * sizeof(uint32_t));
     if (e != hipSuccess) return pm_hip(ctx, e, "upload batch tables");
     b->jobs = jobs;
 
@@ -104,20 +101,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     ba.jobs = d_jobs;
     ba.items = d_items;
     ba.units = d_units;
-    ba.next = b->d_words;
-    ba.error = b->d_words + 1;
-    ba.counters = reinterpret_cast<pmd::JobCounters*>(b->d_words + 2);
     ba.n_items = b->n_items;
-    int rate_khz = 0;
-    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || rate_khz <= 0)
-        rate_khz = 100000;
-    ba.wait_ticks = uint64_t(rate_khz) * 1000u * 10u;  // 10 s per dependency wait
-    hipDeviceProp_t prop;
-    e = hipGetDeviceProperties(&prop, ctx->device);
-    if (e != hipSuccess) return pm_hip(ctx, e, "device properties");
-    b->blocks = uint32_t(prop.multiProcessorCount);  // one 1024-thread block per CU (LDS bound)
-    if (b->blocks > b->n_items) b->blocks = b->n_items;
-    if (b->blocks == 0) b->blocks = 1;
 
     // capture the phase launches once; runs replay the graph
     hipStream_t cap = nullptr;
@@ -281,12 +265,9 @@ int pm_batch_run(pm_batch* b, void* hip_stream) {
 int pm_batch_wait(pm_batch* b, void* hip_stream) {
     if (!b) return PM_EINVAL;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->ctx->stream;
-    hipError_t e = hipStreamSynchronize(stream);
+    // phased launches have no in-kernel waits: a device error is all there is to report
+    const hipError_t e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return pm_hip(b->ctx, e, "batch wait");
-    uint32_t err = 0;
-    e = hipMemcpy(&err, b->d_words + 1, sizeof(err), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return pm_hip(b->ctx, e, "read batch status");
-    if (err == pmd::kErrTimeout) return pm_fail(b->ctx, PM_ETIMEOUT, "a batch dependency wait timed out");
     return PM_OK;
 }
 

@@ -18,14 +18,13 @@ void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool la
 
 struct pm_batch {
     pm_context* ctx = nullptr;
-    uint32_t count = 0, n_items = 0, blocks = 0, reset_count = 0;
+    uint32_t count = 0, n_items = 0;
     std::vector<uint32_t> logn;
     std::vector<pmd::Job> jobs;     // host copy (device pointers)
     pmd::BatchArgs args{};
     pmd::PhaseRanges ranges{};
     bool scalar_conversion = false; // phased only: scalar limb reduction instead of MFMA
     bool scalar_crt = false;        // phased only: vector-ALU CRT sums instead of MFMA
-    uint32_t* d_words = nullptr;    // claim counter, error flag, per-job counters
     std::vector<void*> allocations;
     uint64_t device_bytes = 0;
     hipGraph_t graph = nullptr;

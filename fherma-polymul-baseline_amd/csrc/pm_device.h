@@ -8,7 +8,7 @@ namespace pmd {
 constexpr uint32_t kLimbs = 28;
 constexpr uint32_t kCrtLimbs = 29;
 constexpr uint32_t kWaveSize = 64;
-constexpr uint32_t kBlock = 1024;          // threads of NTT / persistent blocks
+constexpr uint32_t kBlock = 1024;          // threads of an NTT tile block
 constexpr uint32_t kTileElems = 32768;     // residues held by one NTT block (32 per thread)
 constexpr uint32_t kLdsWords = kTileElems + kTileElems / 32;  // padded exchange buffer
 constexpr uint32_t kConvPerWave = 64;      // coefficients per wave in the conversion
@@ -65,7 +65,8 @@ struct Job {
     uint32_t* rb;        // [k][n] residues of b
 };
 
-// Persistent batch: a work list of items claimed in order by resident blocks.
+// Batch work list: the items of each phase (conversion groups, outer-stage
+// columns, tiles, CRT groups); a tile item covers one or more units.
 enum ItemType : uint32_t { kItemConv = 0, kItemOuterFwd = 1, kItemTile = 2, kItemOuterInv = 3, kItemCrt = 4 };
 
 struct Item {
@@ -80,21 +81,11 @@ struct Unit {
     uint32_t prime_s;  // prime << 2 | sub-block
 };
 
-// per-job completion counters, zeroed before every run
-struct JobCounters {
-    uint32_t conv, fwd, tile, inv;
-};
-
 struct BatchArgs {
     const Job* jobs;
     const Item* items;
     const Unit* units;
-    uint32_t* next;             // item claim counter
-    uint32_t* error;            // 0 ok, 2 wait timed out
-    JobCounters* counters;      // [jobs]
     uint32_t n_items;
-    uint32_t pad;
-    uint64_t wait_ticks;        // bound on one dependency wait (s_memrealtime ticks)
 };
 
 constexpr uint32_t kErrTimeout = 2;

@@ -4,7 +4,7 @@
 // Requests of different N under one q are submitted at any time.  A native
 // scheduler thread admits them in arrival order while their device memory
 // fits an HBM budget (first-fit over one arena), groups whatever has been
-// admitted into the next persistent launch (the batch kernel of pm_batch.cpp
+// admitted into the next batch launch (the phased kernels of pm_batch.cpp
 // fed with a freshly written work list), and keeps up to `max_inflight`
 // launches running at once on separate streams, so requests that arrive
 // while others are on the GPU start as soon as memory and a launch slot free
@@ -105,7 +105,6 @@ struct Launch {
     pmd::Job* d_jobs = nullptr;
     pmd::Item* d_items = nullptr;
     pmd::Unit* d_units = nullptr;
-    uint32_t* d_words = nullptr;
     // pinned staging of the work list (async upload)
     pmd::Job* h_jobs = nullptr;
     pmd::Item* h_items = nullptr;
@@ -118,13 +117,12 @@ struct Launch {
 
 struct pm_server {
     pm_context* ctx = nullptr;
-    uint32_t max_batch = 0, max_inflight = 0, blocks = 0;
+    uint32_t max_batch = 0, max_inflight = 0;
     size_t max_items = 0, max_units = 0;
     uint64_t budget = 0;
     uint8_t* arena_base = nullptr;
     Arena arena;
     std::vector<Launch> launches;
-    uint64_t wait_ticks = 0;
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -191,11 +189,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
     ba.jobs = L.d_jobs;
     ba.items = L.d_items;
     ba.units = L.d_units;
-    ba.next = L.d_words;
-    ba.error = L.d_words + 1;
-    ba.counters = reinterpret_cast<pmd::JobCounters*>(L.d_words + 2);
     ba.n_items = uint32_t(items.size());
-    ba.wait_ticks = wait_ticks;
     pmd::PhaseRanges ranges;
     if (!pm_phase_ranges(items, ranges)) {
         error = "work list phases are not contiguous";
@@ -338,13 +332,6 @@ int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_ba
     const size_t per_req_items = 3 * ((n + pmd::kBlock - 1) / pmd::kBlock) + 3 * k + size_t(k) * 4 + 1;
     s->max_items = per_req_items * max_batch + 64;
     s->max_units = size_t(k) * 4 * max_batch + 1;
-    int rate_khz = 0;
-    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || rate_khz <= 0)
-        rate_khz = 100000;
-    s->wait_ticks = uint64_t(rate_khz) * 1000u * 10u;
-    hipDeviceProp_t prop;
-    e = hipGetDeviceProperties(&prop, ctx->device);
-    s->blocks = e == hipSuccess ? uint32_t(prop.multiProcessorCount) : 256u;
     s->launches.resize(max_inflight);
     for (Launch& L : s->launches) {
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
@@ -352,7 +339,6 @@ int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_ba
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L.d_jobs), max_batch * sizeof(pmd::Job));
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L.d_items), s->max_items * sizeof(pmd::Item));
         if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L.d_units), s->max_units * sizeof(pmd::Unit));
-        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L.d_words), (2 + 4 * size_t(max_batch)) * 4);
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_jobs), max_batch * sizeof(pmd::Job));
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_items), s->max_items * sizeof(pmd::Item));
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_units), s->max_units * sizeof(pmd::Unit));
@@ -439,7 +425,6 @@ void pm_server_destroy(pm_server* s) {
         (void)hipFree(L.d_jobs);
         (void)hipFree(L.d_items);
         (void)hipFree(L.d_units);
-        (void)hipFree(L.d_words);
         (void)hipHostFree(L.h_jobs);
         (void)hipHostFree(L.h_items);
         (void)hipHostFree(L.h_units);

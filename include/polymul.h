@@ -186,7 +186,7 @@ int pm_plan_worklist(uint32_t k, uint32_t count, const uint32_t* ns, uint32_t* n
  * Continuous batching: requests of any N (powers of two, 32 <= N <= max_n)
  * are submitted at any time; a native scheduler thread admits them in
  * arrival order while their device memory (3*N*112 + 2*k*N*4 bytes) fits
- * hbm_budget_bytes, groups what has been admitted into the next persistent
+ * hbm_budget_bytes, groups what has been admitted into the next batch
  * launch (at most max_batch products), and keeps up to max_inflight launches
  * running concurrently on separate streams (each launch runs the phased
  * kernels of the batch path on the launch's own stream).  a, b, c are host buffers of
