@@ -186,6 +186,21 @@ def run_headline(args):
         done += 1
     cpu_elapsed = time.perf_counter() - t0
     cpu_rate = done / cpu_elapsed
+    # and on the box's host cores: independent products on one thread per core
+    # (the oracle's C code runs outside the GIL), each checked the same way
+    from concurrent.futures import ThreadPoolExecutor
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    sample = min(len(batch_c), 4 * threads)
+
+    def one(i):
+        a, b = inputs[i]
+        return bool(np.array_equal(orc.rns_multiply(q, a, b, primes), batch_c[i]))
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as pool:
+        par_exact = all(pool.map(one, range(sample)))
+    par_rate = sample / (time.perf_counter() - t0)
+    exact = exact and par_exact
 
     line = {
         "metric": METRIC,
@@ -228,8 +243,10 @@ def run_headline(args):
             "whole_launch_gbs": round(per_product * B / (launch_ms * 1e-3) / 1e9, 1),
         },
         "cpu_baseline": {
-            "value": round(cpu_rate, 4), "unit": "products/s", "cores": 1, "kind": "port",
-            "sample": f"{done} products of the same workload (N={n}), oracle RNS port, single thread",
+            "value": round(par_rate, 4), "unit": "products/s", "cores": threads, "kind": "port",
+            "sample": f"{sample} products of the same workload (N={n}), oracle RNS port, one product per thread on "
+                      f"{threads} threads; single thread: {done} products",
+            "single_thread_value": round(cpu_rate, 4),
         },
         "bit_exact": bool(exact and gpu_consistent and done > 0),
     }
