@@ -152,3 +152,26 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
     applied = ctx.load_schedule(str(path))
     assert applied["batch_outer"][32768] == res[32768]["outer"] == ctx.get_batch_outer(32768)
     ctx.close()
+
+
+def test_batch_in_staggered_parts(pm, workload):
+    """A batch of 16 or more products runs as three staggered parts on three
+    graph branches: every product equals the same product on the plain path,
+    on two replays in a row (the parts' work lists and branches)."""
+    q = workload.modulus(6)
+    ctx = pm.Context(q, 1 << 16)
+    ns = [1024, 4096, 32768, 256, 8192, 65536, 2048, 16384] * 3
+    say("batch in parts", len(ns))
+    batch = pm.Batch(ctx, ns)
+    inputs = []
+    for i, n in enumerate(ns):
+        a, b = workload.family("dense-random-1", n, q, 300 + i)
+        batch.upload(i, a, b)
+        inputs.append((a, b))
+    for replay in range(2):
+        batch.run()
+        batch.wait()
+        for i, (n, (a, b)) in enumerate(zip(ns, inputs)):
+            assert (batch.download(i) == ctx.multiply(a, b)).all(), (replay, i, n)
+    batch.close()
+    ctx.close()
