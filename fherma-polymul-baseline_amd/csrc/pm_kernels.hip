@@ -33,6 +33,13 @@ __device__ __forceinline__ T* wave_uniform(T* ptr) {
 // no swizzle, no stride (gfx9 V# word 3); loads past the range read zero.
 constexpr uint32_t kRawBufferWord3 = 0x00020000u;
 constexpr uint32_t kOutOfRange = 0x80000000u;  // an offset past any buffer range used here
+// Cache policy of the fused tile's residue loads and result stores: nt
+// (streaming).  Each residue is read once and each result is read once more,
+// by the CRT; streaming them keeps the L2 for the twiddle tables that the
+// blocks of one prime share.  At 128 x N = 2^15 the tile phase drops from
+// 1.72 to 1.66 ms; nt on the conversion's 64-byte store runs or on the CRT's
+// loads and 16-byte output stores is slower.
+constexpr int kTileStreamPolicy = 2;
 
 // ---------------------------------------------------------------------------
 // small-modulus primitives (p < 2^30)
@@ -357,10 +364,14 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
     for (int h = 0; h < G::E / 2; ++h) {
         const int r0 = h, r1 = h + G::E / 2;
         if constexpr (kWaveTile) {
-            a[r0] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r0) * 4u, 0);
-            a[r1] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r1) * 4u, 0);
-            b[r0] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r0) * 4u, 0);
-            b[r1] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r1) * 4u, 0);
+            a[r0] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r0) * 4u,
+                                                         kTileStreamPolicy);
+            a[r1] = __builtin_amdgcn_raw_buffer_load_b32(ra, x0 * 4u, G::reg_scatter(0, r1) * 4u,
+                                                         kTileStreamPolicy);
+            b[r0] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r0) * 4u,
+                                                         kTileStreamPolicy);
+            b[r1] = __builtin_amdgcn_raw_buffer_load_b32(rb, x0 * 4u, G::reg_scatter(0, r1) * 4u,
+                                                         kTileStreamPolicy);
         } else {
             a[r0] = a_src[x0 + G::reg_scatter(0, r0)];
             a[r1] = a_src[x0 + G::reg_scatter(0, r1)];
@@ -380,7 +391,7 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
         const uint32_t t = csub(shoup(a[r], s, p), p);  // [0,2p) -> [0,p)
         if (!valid) continue;
         if constexpr (kWaveTile)
-            __builtin_amdgcn_raw_buffer_store_b32(t, rd, x0 * 4u, G::reg_scatter(0, r) * 4u, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(t, rd, x0 * 4u, G::reg_scatter(0, r) * 4u, kTileStreamPolicy);
         else
             dst[x0 + G::reg_scatter(0, r)] = t;
     }
