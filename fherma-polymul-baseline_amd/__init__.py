@@ -422,7 +422,8 @@ class Server:
         """Queue a*b; returns a request id.  The arrays stay referenced until
         wait().  `out` (optional, same shape as a, uint32, C-contiguous) receives
         the product; page-locked memory there lets the result come back
-        asynchronously behind the kernels."""
+        asynchronously behind the kernels.  Page-locked a and b are read by a
+        kernel of the launch (no runtime copy)."""
         a = np.ascontiguousarray(a, dtype=np.uint32)
         b = np.ascontiguousarray(b, dtype=np.uint32)
         if out is None:

@@ -96,6 +96,14 @@ constexpr uint32_t kErrTimeout = 2;
 // finished pieces of each result chunk.
 constexpr uint32_t kMaxIoChunks = 16;
 constexpr uint32_t kMirrorStride = 64;  // words between the device copies of the ready flags
+// One span of a gather copy: words 32-bit words from src to dst (either may
+// be page-locked host memory).
+struct CopySeg {
+    const uint32_t* src;
+    uint32_t* dst;
+    uint64_t words;
+};
+
 struct IoArgs {
     const uint32_t* ready;      // [kMaxIoChunks] host-written
     uint32_t* done;             // [kMaxIoChunks] device-written

@@ -16,6 +16,11 @@ hipError_t launch_ingest_convert(const Tables& tb, const Job& job, const IoArgs&
 // once chunk j is visible to the host
 hipError_t launch_egress(const uint32_t* src, uint32_t* dst_host, size_t chunk_words, uint32_t chunks,
                          const IoArgs& io, hipStream_t stream);
+// copies every span of segs[0 .. n_segs) (segs may be page-locked host
+// memory: the kernel reads it); max_words bounds the longest span.  With
+// to_host the stores are made visible to the host before the kernel ends.
+hipError_t launch_copy_segments(const CopySeg* segs, uint32_t n_segs, uint64_t max_words, bool to_host,
+                                hipStream_t stream);
 // residues of a and b for all primes (MFMA conversion)
 hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream);
 // the same for coefficients [c_first, c_end) of polynomials poly_first ..

@@ -1266,6 +1266,26 @@ __global__ void __launch_bounds__(kConvSmallBlock) k_ingest_convert(Tables tb, J
 // to dst (page-locked host); the last block to finish sets done[chunk] = seq
 // once every store of the chunk is visible to the host.  One launch per chunk,
 // so chunks land in order.
+// Gather copy of the serving path: block row y copies span segs[y], 16 bytes
+// per lane where both ends are 16-byte aligned, else 4.  The span table is
+// read where it lies (page-locked host memory is readable by kernels), so a
+// launch needs no copy engine: every step of a served batch is a kernel on
+// one queue.
+__global__ void __launch_bounds__(256) k_copy_segments(const CopySeg* __restrict__ segs, uint32_t to_host) {
+    const CopySeg sg = segs[blockIdx.y];
+    const size_t lane0 = size_t(blockIdx.x) * blockDim.x + threadIdx.x, step = size_t(gridDim.x) * blockDim.x;
+    if (((reinterpret_cast<uintptr_t>(sg.src) | reinterpret_cast<uintptr_t>(sg.dst)) & 15u) == 0) {
+        const size_t n4 = sg.words / 4;
+        const uint4* s4 = reinterpret_cast<const uint4*>(sg.src);
+        uint4* d4 = reinterpret_cast<uint4*>(sg.dst);
+        for (size_t i = lane0; i < n4; i += step) d4[i] = s4[i];
+        for (size_t i = 4 * n4 + lane0; i < sg.words; i += step) sg.dst[i] = sg.src[i];
+    } else {
+        for (size_t i = lane0; i < sg.words; i += step) sg.dst[i] = sg.src[i];
+    }
+    if (to_host) __threadfence_system();  // the stores reach the host before the kernel's end is signalled
+}
+
 __global__ void __launch_bounds__(256) k_egress(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                 size_t chunk_words, IoArgs io, uint32_t chunk) {
     const size_t first = size_t(chunk) * chunk_words / 4, count = chunk_words / 4;
@@ -1294,6 +1314,18 @@ hipError_t launch_ingest_convert(const Tables& tb, const Job& job, const IoArgs&
         (groups + io.groups_per_chunk - 1) / io.groups_per_chunk != io.chunks)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_ingest_convert, dim3(blocks), dim3(kConvSmallBlock), 0, stream, tb, job, io);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_segments(const CopySeg* segs, uint32_t n_segs, uint64_t max_words, bool to_host,
+                                hipStream_t stream) {
+    if (n_segs == 0) return hipSuccess;
+    if (!segs || n_segs > 65535u) return hipErrorInvalidValue;
+    // up to 64 blocks of 1024 words per pass for the longest span
+    uint64_t blocks = (max_words + 1023) / 1024;
+    blocks = blocks < 1 ? 1 : (blocks > 64 ? 64 : blocks);
+    hipLaunchKernelGGL(k_copy_segments, dim3(uint32_t(blocks), n_segs), dim3(256), 0, stream, segs,
+                       to_host ? 1u : 0u);
     return hipGetLastError();
 }
 

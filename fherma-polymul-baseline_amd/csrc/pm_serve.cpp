@@ -94,7 +94,14 @@ struct Request {
     uint32_t* c = nullptr;
     size_t offset = 0, bytes = 0;  // arena slice: a, b, c, ra, rb
     Clock::time_point submitted, finished;
-    bool c_pinned = false;  // c is page-locked host memory: copy back on the launch stream
+    // device addresses of a and b when both are page-locked host memory (the
+    // launch's gather kernel reads them), of c when it is (the scatter kernel
+    // writes it); null otherwise
+    const uint32_t* a_dev = nullptr;
+    const uint32_t* b_dev = nullptr;
+    uint32_t* c_dev = nullptr;
+    bool ab_pinned = false;
+    bool c_pinned = false;
     bool done = false;
     int status = PM_OK;
 };
@@ -109,6 +116,10 @@ struct Launch {
     pmd::Job* h_jobs = nullptr;
     pmd::Item* h_items = nullptr;
     pmd::Unit* h_units = nullptr;
+    // span tables of the gather (inputs and work list in) and scatter (page-
+    // locked results out) kernels, read by those kernels from host memory
+    pmd::CopySeg* h_in_segs = nullptr;
+    pmd::CopySeg* h_out_segs = nullptr;
     std::vector<Request*> requests;
     bool busy = false;
 };
@@ -170,21 +181,35 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
     }
     std::memcpy(L.h_items, items.data(), items.size() * sizeof(pmd::Item));
     std::memcpy(L.h_units, units.data(), std::max<size_t>(units.size(), 1) * sizeof(pmd::Unit));
+    // Inputs and the work list go in through one gather kernel, page-locked
+    // results come back through one scatter kernel: with the conversion, tile
+    // and CRT launches the whole batch is kernels on one queue, and this thread
+    // makes no call that waits on the device (a copy-engine transfer behind a
+    // kernel can hold the enqueueing thread in the runtime for ~10 ms).  Only
+    // pageable inputs take a runtime copy.
     hipError_t e = hipSuccess;
+    uint32_t n_in = 0;
+    uint64_t max_words = 0;
+    auto span = [&max_words](pmd::CopySeg* segs, uint32_t& n, const void* src, void* dst, size_t bytes) {
+        segs[n++] = pmd::CopySeg{static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), bytes / 4};
+        max_words = std::max<uint64_t>(max_words, bytes / 4);
+    };
     for (uint32_t j = 0; j < count && e == hipSuccess; ++j) {
         const size_t bytes = (size_t(1) << reqs[j]->logn) * kLimbs * 4;
-        e = hipMemcpyAsync(const_cast<uint32_t*>(L.h_jobs[j].a), reqs[j]->a, bytes, hipMemcpyHostToDevice, L.stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(const_cast<uint32_t*>(L.h_jobs[j].b), reqs[j]->b, bytes, hipMemcpyHostToDevice,
-                               L.stream);
+        uint32_t* da = const_cast<uint32_t*>(L.h_jobs[j].a);
+        uint32_t* db = const_cast<uint32_t*>(L.h_jobs[j].b);
+        if (reqs[j]->ab_pinned) {
+            span(L.h_in_segs, n_in, reqs[j]->a_dev, da, bytes);
+            span(L.h_in_segs, n_in, reqs[j]->b_dev, db, bytes);
+        } else {
+            e = hipMemcpyAsync(da, reqs[j]->a, bytes, hipMemcpyHostToDevice, L.stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(db, reqs[j]->b, bytes, hipMemcpyHostToDevice, L.stream);
+        }
     }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(L.d_jobs, L.h_jobs, count * sizeof(pmd::Job), hipMemcpyHostToDevice, L.stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(L.d_items, L.h_items, items.size() * sizeof(pmd::Item), hipMemcpyHostToDevice, L.stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(L.d_units, L.h_units, std::max<size_t>(units.size(), 1) * sizeof(pmd::Unit),
-                           hipMemcpyHostToDevice, L.stream);
+    span(L.h_in_segs, n_in, L.h_jobs, L.d_jobs, count * sizeof(pmd::Job));
+    span(L.h_in_segs, n_in, L.h_items, L.d_items, items.size() * sizeof(pmd::Item));
+    span(L.h_in_segs, n_in, L.h_units, L.d_units, std::max<size_t>(units.size(), 1) * sizeof(pmd::Unit));
+    if (e == hipSuccess) e = pmd::launch_copy_segments(L.h_in_segs, n_in, max_words, false, L.stream);
     pmd::BatchArgs ba{};
     ba.jobs = L.d_jobs;
     ba.items = L.d_items;
@@ -196,13 +221,14 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
         return false;
     }
     if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, false, false, L.stream);
-    // results into page-locked caller buffers go back asynchronously, behind
-    // the kernels on the same stream; other buffers are copied on completion
-    for (uint32_t j = 0; j < count && e == hipSuccess; ++j) {
-        if (!reqs[j]->c_pinned) continue;
-        const size_t bytes = (size_t(1) << reqs[j]->logn) * kLimbs * 4;
-        e = hipMemcpyAsync(reqs[j]->c, L.h_jobs[j].c, bytes, hipMemcpyDeviceToHost, L.stream);
-    }
+    // results into page-locked caller buffers go back behind the kernels on
+    // the same stream; other buffers are copied on completion
+    uint32_t n_out = 0;
+    max_words = 0;
+    for (uint32_t j = 0; j < count; ++j)
+        if (reqs[j]->c_pinned)
+            span(L.h_out_segs, n_out, L.h_jobs[j].c, reqs[j]->c_dev, (size_t(1) << reqs[j]->logn) * kLimbs * 4);
+    if (e == hipSuccess) e = pmd::launch_copy_segments(L.h_out_segs, n_out, max_words, true, L.stream);
     if (e == hipSuccess) e = hipEventRecord(L.finished, L.stream);
     if (e != hipSuccess) {
         error = std::string("launch: ") + hipGetErrorString(e);
@@ -214,10 +240,12 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
 }
 
 void pm_server::finish_launch(Launch& L) {
-    // phased launches have no in-kernel waits, so no device error flag to read
+    // called once L.finished has completed: every kernel and copy of the
+    // launch is done.  Phased launches have no in-kernel waits, so no device
+    // error flag to read.
     const uint32_t err = 0;
-    hipError_t e = hipStreamSynchronize(L.stream);
-    // results to pageable callers' buffers (the kernels have finished)
+    hipError_t e = hipSuccess;
+    // results to pageable callers' buffers
     for (size_t j = 0; j < L.requests.size() && e == hipSuccess; ++j) {
         if (L.requests[j]->c_pinned) continue;
         const size_t bytes = (size_t(1) << L.requests[j]->logn) * kLimbs * 4;
@@ -342,6 +370,10 @@ int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_ba
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_jobs), max_batch * sizeof(pmd::Job));
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_items), s->max_items * sizeof(pmd::Item));
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L.h_units), s->max_units * sizeof(pmd::Unit));
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&L.h_in_segs), (2 * size_t(max_batch) + 3) * sizeof(pmd::CopySeg));
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&L.h_out_segs), size_t(max_batch) * sizeof(pmd::CopySeg));
     }
     if (e != hipSuccess) {
         const int st = pm_hip(ctx, e, "serving launch slots");
@@ -365,9 +397,20 @@ int pm_server_submit(pm_server* s, uint32_t n, const uint32_t* a, const uint32_t
     r->b = b;
     r->c = c;
     r->bytes = s->request_bytes(logn);
-    hipPointerAttribute_t attr{};
-    if (hipPointerGetAttributes(&attr, c) == hipSuccess) r->c_pinned = attr.type == hipMemoryTypeHost;
-    else (void)hipGetLastError();  // pageable memory: not an error
+    // the device address of page-locked host memory, null for pageable memory
+    auto page_locked = [](const void* p) -> void* {
+        hipPointerAttribute_t attr{};
+        if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+            (void)hipGetLastError();  // pageable memory: not an error
+            return nullptr;
+        }
+        return attr.type == hipMemoryTypeHost ? attr.devicePointer : nullptr;
+    };
+    r->a_dev = static_cast<const uint32_t*>(page_locked(a));
+    r->b_dev = static_cast<const uint32_t*>(page_locked(b));
+    r->c_dev = static_cast<uint32_t*>(page_locked(c));
+    r->ab_pinned = r->a_dev && r->b_dev;
+    r->c_pinned = r->c_dev != nullptr;
     if (r->bytes > s->budget) {
         delete r;
         return pm_fail(s->ctx, PM_EBUDGET, "request exceeds the HBM budget");
@@ -428,6 +471,8 @@ void pm_server_destroy(pm_server* s) {
         (void)hipHostFree(L.h_jobs);
         (void)hipHostFree(L.h_items);
         (void)hipHostFree(L.h_units);
+        (void)hipHostFree(L.h_in_segs);
+        (void)hipHostFree(L.h_out_segs);
     }
     (void)hipFree(s->arena_base);
     for (auto& kv : s->all) delete kv.second;

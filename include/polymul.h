@@ -190,7 +190,10 @@ int pm_plan_worklist(uint32_t k, uint32_t count, const uint32_t* ns, uint32_t* n
  * launch (at most max_batch products), and keeps up to max_inflight launches
  * running concurrently on separate streams (each launch runs the phased
  * kernels of the batch path on the launch's own stream).  a, b, c are host buffers of
- * n*28 words that must stay valid until pm_server_wait returns for the id. */
+ * n*28 words that must stay valid until pm_server_wait returns for the id.
+ * Page-locked a and b (both) are read, and a page-locked c written, by
+ * kernels of the launch, so a launch of such requests is kernels only; other
+ * buffers take runtime copies (pageable c after the launch completes). */
 typedef struct pm_server pm_server;
 int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_batch, uint32_t max_inflight,
                      pm_server** out);

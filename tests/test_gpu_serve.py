@@ -94,3 +94,43 @@ def test_served_page_locked_outputs(pm, workload):
         finally:
             server2.close()
     ctx.close()
+
+
+def test_served_page_locked_inputs_reused(pm, workload):
+    """Page-locked inputs are read by the launch's gather kernel and page-locked
+    outputs written by its scatter kernel.  The same page-locked buffers are
+    refilled between rounds (a stale read would repeat the last round's
+    product), one pair sits 4 bytes past a 16-byte boundary (the word-wise
+    copy), and pageable requests run in the same launches."""
+    import torch
+    q = workload.modulus(8)
+    ctx = pm.Context(q, 1 << 15)
+    ns = [1 << 15, 1 << 10, 1 << 13, 1 << 12, 1 << 14, 1 << 11]
+
+    def pinned_words(n, offset):
+        t = torch.empty(n * 28 + offset, dtype=torch.int32).pin_memory()
+        return t.numpy().view(np.uint32)[offset:].reshape(n, 28)
+
+    bufs = [(pinned_words(n, 1 if i == 1 else 0), pinned_words(n, 1 if i == 1 else 0),
+             pinned_words(n, 1 if i == 1 else 0)) for i, n in enumerate(ns)]
+    server = pm.Server(ctx, 2 << 30, max_batch=4, max_inflight=2)
+    for rnd in range(3):
+        reqs = []
+        for i, n in enumerate(ns):
+            a, b = workload.family("dense-random-1", n, q, 300 + 10 * rnd + i)
+            if i % 3 == 2:  # pageable inputs and output
+                reqs.append((a, b, None, server.submit(a, b)))
+                continue
+            pa, pb, pc = bufs[i]
+            pa[...] = a
+            pb[...] = b
+            pc.fill(0xFFFFFFFF)
+            reqs.append((a, b, pc, server.submit(pa, pb, out=pc)))
+        say("round", rnd, "submitted", len(reqs))
+        for i, (a, b, out, rid) in enumerate(reqs):
+            c, _ = server.wait(rid)
+            if out is not None:
+                assert c is out
+            assert (c == ctx.multiply(a, b)).all(), (rnd, i, ns[i])
+    server.close()
+    ctx.close()
