@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "fherma-polymul-baseline_amd")
 METRIC = "N=2^15 868-bit negacyclic mults/s, batched vs plain (bit-exact); p99 ms"
 HBM_PEAK_GBS = 8000.0
+VALU_SIMDS = 1024          # 256 CUs x 4 SIMDs
+VALU_CLOCK_HZ = 2.4e9      # peak engine clock
 
 
 def package():
@@ -166,12 +168,20 @@ def run_headline(args):
     per_product = algorithmic_bytes(n, k)
     launch_ms = percentile(step_ms, 50)
     achieved = phase_gbs[dominant]
-    traffic = None
+    traffic = valu_instructions = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_batched_n{n}_b{B}.json")
     if os.path.exists(tfile):
         # HBM bytes per launch of the dominant phase from the FETCH_SIZE /
         # WRITE_SIZE profile runs, scaled by a copy kernel of known size
-        traffic = json.load(open(tfile)).get("phases", {}).get(phase_key(dominant), {}).get("hbm_bytes_per_launch")
+        tphase = json.load(open(tfile)).get("phases", {}).get(phase_key(dominant), {})
+        traffic = tphase.get("hbm_bytes_per_launch")
+        valu_instructions = tphase.get("valu_instructions_per_launch")
+    # The tile is bound by the vector ALUs: busy fraction = wave64 VALU
+    # instructions per launch (SQ_INSTS_VALU, deterministic for this workload)
+    # x 4 SIMD cycles each / (1024 SIMDs x 2.4 GHz x the phase time)
+    valu_frac = None
+    if valu_instructions:
+        valu_frac = valu_instructions * 4 / (VALU_SIMDS * VALU_CLOCK_HZ * phase_ms[dominant] * 1e-3)
 
     # ---- CPU baseline: the oracle port on a bounded sample -------------------
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -242,6 +252,8 @@ def run_headline(args):
             "phases_ms": {k_: round(v, 4) for k_, v in phase_ms.items()},
             "phases_gbs": {k_: round(v, 1) for k_, v in phase_gbs.items()},
             "whole_launch_gbs": round(per_product * B / (launch_ms * 1e-3) / 1e9, 1),
+            "valu_busy_frac": None if valu_frac is None else round(valu_frac, 4),
+            "valu_instructions_per_launch": valu_instructions,
         },
         "cpu_baseline": {
             "value": round(par_rate, 4), "unit": "products/s", "cores": threads, "kind": "port",
