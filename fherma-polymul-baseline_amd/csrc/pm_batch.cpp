@@ -42,6 +42,7 @@ constexpr double kPartCut[kSplitParts + 1] = {0.0, 0.40, 0.75, 1.0};  // cumulat
 }  // namespace
 
 pm_batch::~pm_batch() {
+    if (last_run) (void)hipEventDestroy(last_run);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void* p : allocations) (void)hipFree(p);
@@ -166,6 +167,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
         e = le != hipSuccess ? le : ce;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->last_run, hipEventDisableTiming);
     for (uint32_t q = 0; q < parts; ++q) {
         if (first_done[q]) (void)hipEventDestroy(first_done[q]);
         if (part_done[q]) (void)hipEventDestroy(part_done[q]);
@@ -286,6 +288,8 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
 void pm_batch_destroy(pm_batch* b) {
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
+    // the last replay may be on a caller's stream: wait for it by its event
+    if (b->last_run) (void)hipEventSynchronize(b->last_run);
     (void)hipStreamSynchronize(b->ctx->stream);
     delete b;
 }
@@ -315,7 +319,9 @@ int pm_batch_download(pm_batch* b, uint32_t i, uint32_t* c) {
 int pm_batch_run(pm_batch* b, void* hip_stream) {
     if (!b) return PM_EINVAL;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->ctx->stream;
-    return pm_hip(b->ctx, hipGraphLaunch(b->exec, stream), "launch batch graph");
+    hipError_t e = hipGraphLaunch(b->exec, stream);
+    if (e == hipSuccess) e = hipEventRecord(b->last_run, stream);
+    return pm_hip(b->ctx, e, "launch batch graph");
 }
 
 int pm_batch_wait(pm_batch* b, void* hip_stream) {

@@ -29,6 +29,7 @@ struct pm_batch {
     uint64_t device_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t last_run = nullptr;  // recorded behind every replay, on whichever stream it ran
 
     void* alloc(size_t bytes);
     ~pm_batch();

@@ -164,8 +164,9 @@ int pm_batch_io(pm_batch* batch, uint32_t i, uint32_t** d_a, uint32_t** d_b, uin
 /* synchronous host <-> device copies of product i's operands / result */
 int pm_batch_upload(pm_batch* batch, uint32_t i, const uint32_t* a, const uint32_t* b);
 int pm_batch_download(pm_batch* batch, uint32_t i, uint32_t* c);
-/* enqueue one replay (0 = context stream); pm_batch_wait syncs and reports
- * PM_ETIMEOUT if a dependency wait inside the kernel hit its bound */
+/* enqueue one replay (0 = context stream); pm_batch_wait synchronizes that
+ * stream and reports a device error.  pm_batch_destroy waits for the last
+ * replay, on whichever stream it was enqueued. */
 int pm_batch_run(pm_batch* batch, void* hip_stream);
 int pm_batch_wait(pm_batch* batch, void* hip_stream);
 int pm_batch_info(const pm_batch* batch, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes);
