@@ -145,18 +145,18 @@ struct pm_server {
     std::thread worker;
     // statistics
     uint64_t completed = 0, launches_done = 0, max_concurrent = 0, failed = 0;
-    std::string error;
+    std::string error;  // reason of the last failed launch (guarded by mu)
 
     size_t request_bytes(uint32_t logn) const {
         const size_t n = size_t(1) << logn;
         return align_up(3 * n * kLimbs * 4) + align_up(2 * size_t(ctx->params.k) * n * 4);
     }
     void run();
-    bool start_launch(Launch& L, std::vector<Request*>& reqs);
-    void finish_launch(Launch& L);
+    bool start_launch(Launch& L, std::vector<Request*>& reqs, std::string& why);
+    void finish_launch(Launch& L, hipError_t status);
 };
 
-bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
+bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string& why) {
     const uint32_t k = ctx->params.k;
     std::vector<pmd::Item> items;
     std::vector<pmd::Unit> units;
@@ -176,7 +176,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
     }
     pm_build_worklist(k, L.h_jobs, count, false, items, units);
     if (items.size() > max_items || units.size() > max_units) {
-        error = "work list exceeds the launch slot";
+        why = "work list exceeds the launch slot";
         return false;
     }
     std::memcpy(L.h_items, items.data(), items.size() * sizeof(pmd::Item));
@@ -217,7 +217,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
     ba.n_items = uint32_t(items.size());
     pmd::PhaseRanges ranges;
     if (!pm_phase_ranges(items, ranges)) {
-        error = "work list phases are not contiguous";
+        why = "work list phases are not contiguous";
         return false;
     }
     if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, false, false, L.stream);
@@ -231,7 +231,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
     if (e == hipSuccess) e = pmd::launch_copy_segments(L.h_out_segs, n_out, max_words, true, L.stream);
     if (e == hipSuccess) e = hipEventRecord(L.finished, L.stream);
     if (e != hipSuccess) {
-        error = std::string("launch: ") + hipGetErrorString(e);
+        why = std::string("launch: ") + hipGetErrorString(e);
         return false;
     }
     L.requests = reqs;
@@ -239,12 +239,11 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs) {
     return true;
 }
 
-void pm_server::finish_launch(Launch& L) {
-    // called once L.finished has completed: every kernel and copy of the
-    // launch is done.  Phased launches have no in-kernel waits, so no device
-    // error flag to read.
-    const uint32_t err = 0;
-    hipError_t e = hipSuccess;
+void pm_server::finish_launch(Launch& L, hipError_t status) {
+    // called once L.finished is no longer pending: with status hipSuccess
+    // every kernel and copy of the launch is done; otherwise the device
+    // reported an error and the launch's requests fail with it
+    hipError_t e = status;
     // results to pageable callers' buffers
     for (size_t j = 0; j < L.requests.size() && e == hipSuccess; ++j) {
         if (L.requests[j]->c_pinned) continue;
@@ -253,9 +252,10 @@ void pm_server::finish_launch(Launch& L) {
     }
     const auto now = Clock::now();
     std::lock_guard<std::mutex> lock(mu);
+    if (e != hipSuccess) error = std::string("device: ") + hipGetErrorString(e);
     for (Request* r : L.requests) {
         r->finished = now;
-        r->status = e != hipSuccess ? PM_EDEVICE : (err ? PM_ETIMEOUT : PM_OK);
+        r->status = e != hipSuccess ? PM_EDEVICE : PM_OK;
         if (r->status != PM_OK) ++failed;
         r->done = true;
         arena.release(r->offset, r->bytes);
@@ -301,8 +301,10 @@ void pm_server::run() {
             }
         }
         if (!batch.empty()) {
-            if (!start_launch(*slot, batch)) {
+            std::string why;
+            if (!start_launch(*slot, batch, why)) {
                 std::lock_guard<std::mutex> lock(mu);
+                error = why;
                 for (Request* r : batch) {
                     r->status = PM_EDEVICE;
                     r->done = true;
@@ -323,8 +325,9 @@ void pm_server::run() {
         bool progressed = false;
         for (Launch& L : launches) {
             if (!L.busy) continue;
-            if (hipEventQuery(L.finished) == hipSuccess) {
-                finish_launch(L);
+            const hipError_t q = hipEventQuery(L.finished);
+            if (q != hipErrorNotReady) {
+                finish_launch(L, q);
                 progressed = true;
             }
         }
@@ -438,7 +441,9 @@ int pm_server_wait(pm_server* s, uint64_t id, double* latency_seconds) {
     const int st = r->status;
     s->all.erase(it);
     delete r;
-    if (st != PM_OK) s->ctx->last_error = "served product failed";
+    if (st != PM_OK)
+        s->ctx->last_error = s->error.empty() ? std::string("served product failed")
+                                              : "served product failed: " + s->error;
     return st;
 }
 
