@@ -99,6 +99,29 @@ def run_headline(args):
     print(f"[bench] batched: {batched_rate:.0f} products/s, step p50 {percentile(step_ms, 50):.3f} ms",
           file=sys.stderr, flush=True)
 
+    # ---- double-buffered: a second batch of the same products replayed on a
+    # second stream, alternating, as a stream of batches would run (reported in
+    # config only; value keeps one launch at a time)
+    stream2 = torch.cuda.Stream()
+    batch2 = pm.Batch(ctx, [n] * B, mode=args.mode)
+    for i in range(B):
+        s_ = wl.mix64(np.uint64(args.seed * 1000003 + i)).item()
+        batch2.upload(i, wl.poly(s_, 0, n, q), wl.poly(s_, 1, n, q))
+    pair = ((batch, sptr), (batch2, stream2.cuda_stream))
+    for j in range(2 * args.warmup):
+        pair[j % 2][0].run(pair[j % 2][1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in range(args.steps):
+        pair[j % 2][0].run(pair[j % 2][1])
+    torch.cuda.synchronize()
+    double_rate = B * args.steps / (time.perf_counter() - t0)
+    batch.wait(sptr)
+    batch2.wait(stream2.cuda_stream)
+    double_exact = all(np.array_equal(batch.download(i), batch2.download(i)) for i in (0, B // 2, B - 1))
+    batch2.close()
+    print(f"[bench] double-buffered: {double_rate:.0f} products/s, equal {double_exact}", file=sys.stderr, flush=True)
+
     # ---- plain path (one product per launch sequence, inputs in HBM) --------
     da = torch.from_numpy(inputs[0][0].view(np.int32)).cuda()
     db = torch.from_numpy(inputs[0][1].view(np.int32)).cuda()
@@ -233,6 +256,8 @@ def run_headline(args):
             "plain_products_per_s": round(plain_rate, 1),
             "plain_p50_ms": round(percentile(plain_ms, 50), 4), "plain_p99_ms": round(percentile(plain_ms, 99), 4),
             "batched_over_plain": round(batched_rate / plain_rate, 3),
+            "double_buffered_products_per_s": round(double_rate, 1),
+            "double_buffered_equal": bool(double_exact),
             "dropin_host_buffers_p50_ms": round(percentile(host_ms, 50), 4),
             "dropin_host_buffers_p99_ms": round(percentile(host_ms, 99), 4),
             "dropin_fherma_run_p50_ms": round(percentile(harness_ms, 50), 4),
