@@ -175,3 +175,39 @@ def test_batch_in_staggered_parts(pm, workload):
             assert (batch.download(i) == ctx.multiply(a, b)).all(), (replay, i, n)
     batch.close()
     ctx.close()
+
+
+def test_batches_on_two_streams_and_destroy_waits(pm, workload):
+    """Two batches of the same products replayed alternately on two caller
+    streams (double buffering) both equal the plain path; a batch closed right
+    after a replay on a caller stream waits for that replay before it frees
+    its memory."""
+    import torch
+    q = workload.modulus(9)
+    ctx = pm.Context(q, 1 << 13)
+    ns = [1 << 13] * 8 + [1 << 11] * 8
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    inputs = [workload.family("dense-random-1", n, q, 500 + i) for i, n in enumerate(ns)]
+    batches = [pm.Batch(ctx, ns), pm.Batch(ctx, ns)]
+    for bt in batches:
+        for i, (a, b) in enumerate(inputs):
+            bt.upload(i, a, b)
+    say("two streams", len(ns))
+    for r in range(6):
+        batches[r % 2].run(streams[r % 2].cuda_stream)
+    for bt, st in zip(batches, streams):
+        bt.wait(st.cuda_stream)
+    for i, (a, b) in enumerate(inputs):
+        want = ctx.multiply(a, b)
+        for j, bt in enumerate(batches):
+            assert (bt.download(i) == want).all(), (j, i)
+    say("close after a replay on a caller stream")
+    last = pm.Batch(ctx, ns)
+    for i, (a, b) in enumerate(inputs):
+        last.upload(i, a, b)
+    last.run(streams[1].cuda_stream)
+    last.close()
+    torch.cuda.synchronize()
+    for bt in batches:
+        bt.close()
+    ctx.close()
