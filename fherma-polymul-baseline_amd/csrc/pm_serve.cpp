@@ -179,6 +179,11 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
         why = "work list exceeds the launch slot";
         return false;
     }
+    pmd::PhaseRanges ranges;
+    if (!pm_phase_ranges(items, ranges)) {  // checked before anything is enqueued
+        why = "work list phases are not contiguous";
+        return false;
+    }
     std::memcpy(L.h_items, items.data(), items.size() * sizeof(pmd::Item));
     std::memcpy(L.h_units, units.data(), std::max<size_t>(units.size(), 1) * sizeof(pmd::Unit));
     // Inputs and the work list go in through one gather kernel, page-locked
@@ -215,11 +220,6 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
     ba.items = L.d_items;
     ba.units = L.d_units;
     ba.n_items = uint32_t(items.size());
-    pmd::PhaseRanges ranges;
-    if (!pm_phase_ranges(items, ranges)) {
-        why = "work list phases are not contiguous";
-        return false;
-    }
     if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, false, false, L.stream);
     // results into page-locked caller buffers go back behind the kernels on
     // the same stream; other buffers are copied on completion
@@ -231,6 +231,9 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
     if (e == hipSuccess) e = pmd::launch_copy_segments(L.h_out_segs, n_out, max_words, true, L.stream);
     if (e == hipSuccess) e = hipEventRecord(L.finished, L.stream);
     if (e != hipSuccess) {
+        // what was enqueued still uses the requests' memory: let it drain
+        // before the caller releases that memory
+        (void)hipStreamSynchronize(L.stream);
         why = std::string("launch: ") + hipGetErrorString(e);
         return false;
     }
