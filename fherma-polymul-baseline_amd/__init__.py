@@ -350,6 +350,7 @@ class Batch:
             ptrs = [(ctypes.c_void_p * count)(*[_ptr(x) for x in v]) for v in (d_a, d_b, d_c)]
         handle = ctypes.c_void_p()
         flags = {"phased": self.PHASED, "persistent": self.PERSISTENT,
+                 "phased-scalar-conversion": self.PHASED | self.SCALAR_CONVERSION,
                  "phased-scalar": self.PHASED | self.SCALAR_CONVERSION | self.SCALAR_CRT}[mode]
         ctx._check("pm_batch_create", self._lib.pm_batch_create(ctx.handle, count, arr, ptrs[0], ptrs[1], ptrs[2],
                                                                  flags, ctypes.byref(handle)))

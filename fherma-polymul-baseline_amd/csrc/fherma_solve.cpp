@@ -40,13 +40,15 @@ void* fherma_init(const fherma::Point& p) {
 fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
     auto* ctx = static_cast<pm_context*>(state);
     if (!ctx) throw std::runtime_error("fherma_run: null state");
+    // the Point's N is the context's max_n; a and b must hold exactly N x L
+    // words, as the reference checks (rns/solve.cu:723-724)
     pm_info_t info{};
-    pm_get_info(ctx, &info);
-    const size_t words = in.a.data.size();
-    if (words == 0 || words % PM_LIMBS != 0 || in.b.data.size() != words)
-        throw std::runtime_error("fherma_run: a and b must both hold N x 28 words");
-    const uint32_t n = uint32_t(words / PM_LIMBS);
-    int status = pm_multiply_submit(ctx, n, in.a.data.data(), in.b.data.data());
+    int status = pm_get_info(ctx, &info);
+    if (status != PM_OK) raise("fherma_run", ctx, status);
+    const uint32_t n = info.max_n;
+    const size_t words = size_t(n) * PM_LIMBS;
+    if (in.a.data.size() != words || in.b.data.size() != words) throw std::runtime_error("RNS input size");
+    status = pm_multiply_submit(ctx, n, in.a.data.data(), in.b.data.data());
     if (status != PM_OK) raise("fherma_run", ctx, status);
     // the fresh output vector is allocated (and zero-filled) while the
     // device computes

@@ -305,7 +305,10 @@ int pm_batch_io(pm_batch* b, uint32_t i, uint32_t** d_a, uint32_t** d_b, uint32_
 int pm_batch_upload(pm_batch* b, uint32_t i, const uint32_t* a, const uint32_t* bb) {
     if (!b || i >= b->count || !a || !bb) return PM_EINVAL;
     const size_t bytes = (size_t(1) << b->logn[i]) * kLimbs * sizeof(uint32_t);
-    hipError_t e = hipMemcpy(const_cast<uint32_t*>(b->jobs[i].a), a, bytes, hipMemcpyHostToDevice);
+    // a replay runs on a non-blocking stream, which these copies do not order
+    // behind: wait for the last one so its inputs are not overwritten mid-run
+    hipError_t e = hipEventSynchronize(b->last_run);
+    if (e == hipSuccess) e = hipMemcpy(const_cast<uint32_t*>(b->jobs[i].a), a, bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(const_cast<uint32_t*>(b->jobs[i].b), bb, bytes, hipMemcpyHostToDevice);
     return pm_hip(b->ctx, e, "batch upload");
 }
@@ -313,7 +316,9 @@ int pm_batch_upload(pm_batch* b, uint32_t i, const uint32_t* a, const uint32_t* 
 int pm_batch_download(pm_batch* b, uint32_t i, uint32_t* c) {
     if (!b || i >= b->count || !c) return PM_EINVAL;
     const size_t bytes = (size_t(1) << b->logn[i]) * kLimbs * sizeof(uint32_t);
-    return pm_hip(b->ctx, hipMemcpy(c, b->jobs[i].c, bytes, hipMemcpyDeviceToHost), "batch download");
+    hipError_t e = hipEventSynchronize(b->last_run);  // the result of the last replay is whole
+    if (e == hipSuccess) e = hipMemcpy(c, b->jobs[i].c, bytes, hipMemcpyDeviceToHost);
+    return pm_hip(b->ctx, e, "batch download");
 }
 
 int pm_batch_run(pm_batch* b, void* hip_stream) {

@@ -176,7 +176,10 @@ std::string build_params(uint32_t max_logn, const uint32_t* q_words, uint32_t li
     trim(q);
     if (q.empty() || (q[0] & 1u) == 0) return "q must be odd";
     if (bits(q) > kWidth) return "q must be below 2^868";
-    if (bits(q) < 64) return "q must exceed 2^64";
+    // The CRT's Barrett step estimates the quotient from floor(X / 2^864) and
+    // floor(2^908 / q); its error is below 1 + X/2^908 + 2^864/q, so a single
+    // conditional subtraction finishes it only when q > 2^864.
+    if (bits(q) < kMinModulusBits) return "q must have 865 to 868 bits (q > 2^864)";
     s = Params{};
     s.max_logn = max_logn;
     s.q.assign(q_words, q_words + kLimbs);

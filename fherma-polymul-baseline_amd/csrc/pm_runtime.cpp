@@ -194,7 +194,8 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
         pm_destroy(ctx);
         return create_failed(PM_ENOMEM, "device workspace allocation failed");
     }
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->workspace_free, hipEventDisableTiming) != hipSuccess) {
         pm_destroy(ctx);
         return create_failed(PM_EDEVICE, "stream creation failed");
     }
@@ -291,6 +292,11 @@ int pm_plan_rns(uint32_t max_n, const uint32_t* q, uint32_t* k, uint32_t* primes
 void pm_destroy(pm_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    if (ctx->workspace_free) {
+        // the last plain product may have run on a caller's stream
+        if (ctx->workspace_used) (void)hipEventSynchronize(ctx->workspace_free);
+        (void)hipEventDestroy(ctx->workspace_free);
+    }
     if (ctx->stream) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
@@ -338,10 +344,22 @@ int pm_check_n(pm_context* ctx, uint32_t n, uint32_t* logn) {
     return PM_OK;
 }
 
+hipError_t pm_workspace_acquire(pm_context* ctx, hipStream_t stream) {
+    return ctx->workspace_used ? hipStreamWaitEvent(stream, ctx->workspace_free, 0) : hipSuccess;
+}
+
+hipError_t pm_workspace_release(pm_context* ctx, hipStream_t stream) {
+    const hipError_t e = hipEventRecord(ctx->workspace_free, stream);
+    if (e == hipSuccess) ctx->workspace_used = true;
+    return e;
+}
+
 int pm_run_plain(pm_context* ctx, const pmd::Job& job, uint32_t outer, hipStream_t stream) {
-    hipError_t e = pmd::launch_convert(ctx->tables, job, stream);
+    hipError_t e = pm_workspace_acquire(ctx, stream);
+    if (e == hipSuccess) e = pmd::launch_convert(ctx->tables, job, stream);
     if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, outer, stream);
     if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, stream);
+    if (e == hipSuccess) e = pm_workspace_release(ctx, stream);
     return pm_hip(ctx, e, "launch plain product");
 }
 
@@ -424,7 +442,8 @@ int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uin
                    ctx->d_io_counters, ctx->d_io_counters + pmd::kMaxIoChunks, ctx->io_seq, in_chunks,
                    groups_per_chunk, 0, ctx->wait_ticks};
     pmd::Job job{logn, 0, ctx->hd_in, ctx->hd_in + words, ctx->d_c, ctx->d_ra, ctx->d_rb};
-    e = pmd::launch_ingest_convert(ctx->tables, job, io, ctx->stream);
+    e = pm_workspace_acquire(ctx, ctx->stream);
+    if (e == hipSuccess) e = pmd::launch_ingest_convert(ctx->tables, job, io, ctx->stream);
     // input chunk j: coefficients [j, j+1) * span of the concatenated a|b.  The
     // other launches come after the copies: a launch costs several us of host
     // time, and the copies are what the device is waiting for
@@ -443,6 +462,7 @@ int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uin
     if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, ctx->outer[logn], ctx->stream);
     if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, ctx->stream);
     if (e == hipSuccess) e = pmd::launch_egress(ctx->d_c, ctx->hd_out, words / out_chunks, out_chunks, io, ctx->stream);
+    if (e == hipSuccess) e = pm_workspace_release(ctx, ctx->stream);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(ctx->stream);
         return pm_hip(ctx, e, "launch product");

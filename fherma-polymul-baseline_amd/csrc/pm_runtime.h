@@ -21,6 +21,11 @@ struct pm_context {
     hipStream_t stream = nullptr;
     // plain-path workspace
     uint32_t *d_a = nullptr, *d_b = nullptr, *d_c = nullptr, *d_ra = nullptr, *d_rb = nullptr;
+    // recorded behind every plain-path product on the stream it ran on; the
+    // next product (device or host-buffer, on any stream) waits for it, so two
+    // products never share the residue workspace at once
+    hipEvent_t workspace_free = nullptr;
+    bool workspace_used = false;
     // outer radix-2 stages run as separate passes, per log2 N: plain path,
     // and batches / served launches
     uint32_t outer[pm::kMaxLogN + 1] = {};
@@ -55,6 +60,11 @@ int pm_fail(pm_context* ctx, int status, const std::string& why);
 int pm_hip(pm_context* ctx, hipError_t e, const char* what);
 int pm_check_n(pm_context* ctx, uint32_t n, uint32_t* logn);
 int pm_run_plain(pm_context* ctx, const pmd::Job& job, uint32_t outer, hipStream_t stream);
+// order a plain-path product on `stream` after the previous one (whose
+// residues share the workspace), and mark the workspace in use until the
+// work enqueued on `stream` so far has run
+hipError_t pm_workspace_acquire(pm_context* ctx, hipStream_t stream);
+hipError_t pm_workspace_release(pm_context* ctx, hipStream_t stream);
 uint32_t pm_default_outer(uint32_t logn);
 uint32_t pm_default_batch_outer(uint32_t logn);
 // whether `outer` passes leave a tile of size 2^lo..2^15 for products of 2^logn

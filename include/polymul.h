@@ -1,6 +1,6 @@
 /*
  * polymul.h -- C-ABI of the MI355X (gfx950) exact negacyclic product
- *     c = a * b  mod (X^N + 1, q),   q odd, 64 < bits(q) <= 868,
+ *     c = a * b  mod (X^N + 1, q),   q odd, 2^864 < q < 2^868,
  * with coefficients as 28 little-endian uint32 limbs (fherma.h:31-40).
  *
  * Plain pointers and sizes only; every call returns a pm_status and the
@@ -23,7 +23,7 @@ typedef struct pm_context pm_context;
 typedef enum {
     PM_OK = 0,
     PM_EINVAL = 1,        /* bad argument (null pointer, N not a power of two, ...) */
-    PM_EUNSUPPORTED = 2,  /* outside coverage: N > max_n, q too large/small/even     */
+    PM_EUNSUPPORTED = 2,  /* outside coverage: N > max_n, q even or not in (2^864, 2^868) */
     PM_EDEVICE = 3,       /* HIP runtime error (text in pm_last_error)              */
     PM_ENOMEM = 4,        /* device or host allocation failed                       */
     PM_EBUDGET = 5,       /* request exceeds the admission budget                   */
@@ -161,7 +161,8 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
                     const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out);
 void pm_batch_destroy(pm_batch* batch);
 int pm_batch_io(pm_batch* batch, uint32_t i, uint32_t** d_a, uint32_t** d_b, uint32_t** d_c);
-/* synchronous host <-> device copies of product i's operands / result */
+/* synchronous host <-> device copies of product i's operands / result; each
+ * first waits for the batch's last replay (on whichever stream it ran) */
 int pm_batch_upload(pm_batch* batch, uint32_t i, const uint32_t* a, const uint32_t* b);
 int pm_batch_download(pm_batch* batch, uint32_t i, uint32_t* c);
 /* enqueue one replay (0 = context stream); pm_batch_wait synchronizes that

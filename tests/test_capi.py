@@ -50,6 +50,10 @@ def test_plan_matches_oracle_rule(pm, orc, workload):
     (1 << 18, lambda w: w.modulus(1), "power-of-two"),
     (1024, lambda w: np.zeros(28, np.uint32), "odd"),
     (1024, lambda w: w.from_int((1 << 800) + 2), "odd"),
+    # the CRT's one-step Barrett finish needs q > 2^864 (an 864-bit q gives
+    # wrong products in a CPU emulation of that step)
+    (1024, lambda w: w.from_int((1 << 863) + 12345), "865 to 868 bits"),
+    (1024, lambda w: w.from_int((1 << 99) + 1), "865 to 868 bits"),
 ])
 def test_plan_rejects(pm, workload, max_n, q_fn, why):
     with pytest.raises(pm.PolymulError) as err:
@@ -62,3 +66,8 @@ def test_plan_rejects_wide_modulus(pm, workload):
     with pytest.raises(pm.PolymulError) as err:
         pm.plan_rns(q, 1024)
     assert "2^868" in str(err.value)
+
+
+def test_plan_accepts_smallest_modulus(pm, workload):
+    k, primes, psi, bits = pm.plan_rns(workload.from_int((1 << 864) + 1), 1024)  # 865 bits
+    assert k > 0 and bits > 2 * 865

@@ -211,3 +211,33 @@ def test_batches_on_two_streams_and_destroy_waits(pm, workload):
     for bt in batches:
         bt.close()
     ctx.close()
+
+
+def test_large_n_conversion_mfma_vs_scalar(pm, workload):
+    """Batched large N (32 products at N = 2^17 is the benchmark; 8 here): the
+    residue conversion run two ways on the same inputs -- int8 MFMA against
+    scalar limb reduction (and with the vector-ALU CRT) -- gives bit-identical
+    products, and the two GMP fixtures at N = 2^17 match their SHA-256."""
+    n, count = 1 << 17, 8
+    golden = [c for c in GOLDEN["cases"] if c["n"] == n]
+    assert len(golden) == 2
+    q = workload.modulus(int(golden[0]["modulus"].split(":")[1]))
+    ctx = pm.Context(q, n)
+    inputs = [workload.family(c["family"], n, q, c["seed"]) for c in golden]
+    inputs += [workload.family("dense-random-1", n, q, 500 + i) for i in range(count - len(golden))]
+    outs = {}
+    for mode in ("phased", "phased-scalar-conversion", "phased-scalar"):
+        say("large N batch", mode)
+        batch = pm.Batch(ctx, [n] * count, mode=mode)
+        for i, (a, b) in enumerate(inputs):
+            batch.upload(i, a, b)
+        batch.run()
+        batch.wait()
+        outs[mode] = [batch.download(i) for i in range(count)]
+        batch.close()
+    for i, case in enumerate(golden):
+        assert sha(outs["phased"][i]) == case["sha256"]["c"], case["family"]
+    for mode in ("phased-scalar-conversion", "phased-scalar"):
+        for i in range(count):
+            assert (outs[mode][i] == outs["phased"][i]).all(), (mode, i)
+    ctx.close()

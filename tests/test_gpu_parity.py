@@ -266,6 +266,41 @@ def test_solution_harness(workload, tmp_path):
         assert hashlib.sha256(got).hexdigest() == case["sha256"]["c"]
 
 
+def test_plain_products_on_two_streams_share_workspace_safely(pm, workload, torch_cuda):
+    """Plain products enqueued back to back on two streams (and then a
+    host-buffer product) share the context's residue workspace: each is
+    ordered after the previous one, so every result equals the same product
+    computed alone."""
+    say("two streams")
+    n = 32768
+    q = workload.modulus(21)
+    ctx = context(pm, q, 1 << 15)
+    ops = [workload.family("dense-random-1", n, q, 40 + i) for i in range(3)]
+    alone = [ctx.multiply(a, b) for a, b in ops]
+    s1, s2 = torch_cuda.cuda.Stream(), torch_cuda.cuda.Stream()
+    dev = [(to_dev(torch_cuda, a), to_dev(torch_cuda, b)) for a, b in ops]
+    outs = [torch_cuda.zeros(n * 28, dtype=torch_cuda.int32, device="cuda") for _ in ops]
+    torch_cuda.cuda.synchronize()
+    for rep in range(3):
+        for i, ((da, db), dc) in enumerate(zip(dev, outs)):
+            ctx.multiply_device(n, da, db, dc, (s1 if i % 2 == 0 else s2).cuda_stream)
+        host = ctx.multiply(*ops[1])  # the context's own stream, same workspace
+        torch_cuda.cuda.synchronize()
+        for i, dc in enumerate(outs):
+            assert (to_host(dc).reshape(n, 28) == alone[i]).all(), (rep, i)
+        assert (host == alone[1]).all(), rep
+
+
+def test_fherma_run_input_size():
+    """fherma_run refuses a and b that are not exactly N x L words for the
+    N of fherma_init ("RNS input size", rns/solve.cu:723-724) and answers the
+    next well-formed call (tests/native/fherma_size_check.cpp)."""
+    say("fherma_run input size")
+    out = subprocess.run([os.path.join(PKG_DIR, "fherma_size_check")], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip().endswith("ok")
+
+
 def test_plain_schedule_and_autotune(pm, workload, tmp_path):
     """Every plain-path schedule (outer passes 0..2) gives the same product;
     the autotuner keeps the fastest per size and its JSON loads back."""
