@@ -83,6 +83,8 @@ def library():
     lib.pm_get_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_set_batch_outer.argtypes = [P, u32, u32]
     lib.pm_get_batch_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
+    lib.pm_set_batch_tiles_per_block.argtypes = [P, u32, u32]
+    lib.pm_get_batch_tiles_per_block.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_stage_convert.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_convert_scalar.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
@@ -116,6 +118,7 @@ EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_i
             "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_set_io_wait_limit",
             "pm_inject_io_stall", "pm_place_near_device", "pm_multiply_device",
             "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
+            "pm_set_batch_tiles_per_block", "pm_get_batch_tiles_per_block",
             "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
@@ -297,6 +300,18 @@ class Context:
         self._check("pm_get_batch_outer", self._lib.pm_get_batch_outer(self._h, int(n), ctypes.byref(v)))
         return v.value
 
+    def set_batch_tiles_per_block(self, tile_n, tiles):
+        """Batch schedule: tiles of size tile_n per block (0 = the default,
+        a 1024-thread block) in batches and served launches created afterwards."""
+        self._check("pm_set_batch_tiles_per_block",
+                    self._lib.pm_set_batch_tiles_per_block(self._h, int(tile_n), int(tiles)))
+
+    def get_batch_tiles_per_block(self, tile_n):
+        v = ctypes.c_uint32()
+        self._check("pm_get_batch_tiles_per_block",
+                    self._lib.pm_get_batch_tiles_per_block(self._h, int(tile_n), ctypes.byref(v)))
+        return v.value
+
     def schedule(self):
         """{n: outer} for every power of two up to max_n."""
         return {1 << lg: self.get_outer(1 << lg) for lg in range(1, self.info()["max_n"].bit_length())}
@@ -338,7 +353,7 @@ class Context:
 class Batch:
     """Products of uneven N in one captured hipGraph (one launch per phase)."""
 
-    PERSISTENT, PHASED, SCALAR_CONVERSION, SCALAR_CRT = 1, 2, 4, 8
+    PERSISTENT, PHASED, SCALAR_CONVERSION, SCALAR_CRT, SEPARATE_OUTER = 1, 2, 4, 8, 16
 
     def __init__(self, ctx, ns, d_a=None, d_b=None, d_c=None, mode="phased"):
         self._ctx, self._lib = ctx, ctx._lib
@@ -350,6 +365,7 @@ class Batch:
             ptrs = [(ctypes.c_void_p * count)(*[_ptr(x) for x in v]) for v in (d_a, d_b, d_c)]
         handle = ctypes.c_void_p()
         flags = {"phased": self.PHASED, "persistent": self.PERSISTENT,
+                 "phased-separate-outer": self.PHASED | self.SEPARATE_OUTER,
                  "phased-scalar-conversion": self.PHASED | self.SCALAR_CONVERSION,
                  "phased-scalar": self.PHASED | self.SCALAR_CONVERSION | self.SCALAR_CRT}[mode]
         ctx._check("pm_batch_create", self._lib.pm_batch_create(ctx.handle, count, arr, ptrs[0], ptrs[1], ptrs[2],
@@ -397,7 +413,7 @@ class Batch:
         out = []
         for kd, t in zip(kinds[:count.value], ms[:count.value]):
             kd = int(kd)
-            name = f"tile_2^{kd >> 8}" if (kd & 0xFF) == 2 else self.PHASE_NAMES[kd & 0xFF]
+            name = f"tile_2^{(kd >> 8) & 0xFF}" if (kd & 0xFF) == 2 else self.PHASE_NAMES[kd & 0xFF]
             out.append((name, float(t)))
         return out
 

@@ -64,7 +64,7 @@ static int upload_worklist(pm_context* ctx, pm_batch* b, const pmd::Job* d_jobs,
     const uint32_t k = ctx->params.k;
     std::vector<pmd::Item> items;
     std::vector<pmd::Unit> units;
-    pm_build_worklist(k, b->jobs.data() + first, count, true, items, units);
+    pm_build_worklist(k, b->jobs.data() + first, count, true, items, units, ctx->batch_tiles);
     if (!pm_phase_ranges(items, ranges)) return pm_fail(ctx, PM_EINVAL, "work list phases are not contiguous");
     if (units.empty()) units.push_back({0, 0});
     auto* d_items = static_cast<pmd::Item*>(b->alloc(items.size() * sizeof(pmd::Item)));
@@ -94,6 +94,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
         pmd::Job& j = jobs[i];
         j.logn = b->logn[i];
         j.outer = ctx->batch_outer[b->logn[i]];
+        j.fused = !b->separate_outer && pmd::outer_fusable(j.logn, j.outer) ? 1u : 0u;
         const bool caller_io = d_a && d_b && d_c;
         if (caller_io) {
             j.a = d_a[i];
@@ -193,8 +194,10 @@ bool pm_phase_ranges(const std::vector<pmd::Item>& items, pmd::PhaseRanges& r) {
             case pmd::kItemConv: ok = add(i, r.conv_first, r.conv_count); break;
             case pmd::kItemOuterFwd: ok = add(i, r.fwd_first, r.fwd_count); break;
             case pmd::kItemTile: {
-                const uint32_t tlog = it.a0 & 0xFFu;
-                ok = tlog >= 5 && tlog <= 15 && add(i, r.tile_first[tlog], r.tile_count[tlog]);
+                const uint32_t tlog = it.a0 & 0xFFu, units = it.a0 >> 8;
+                ok = tlog >= 5 && tlog <= 15 && units >= 1 && (units << tlog) <= 32768u &&
+                     add(i, r.tile_first[tlog], r.tile_count[tlog]);
+                if (ok && units > r.tile_units[tlog]) r.tile_units[tlog] = units;
                 break;
             }
             case pmd::kItemOuterInv: ok = add(i, r.inv_first, r.inv_count); break;
@@ -207,7 +210,8 @@ bool pm_phase_ranges(const std::vector<pmd::Item>& items, pmd::PhaseRanges& r) {
 }
 
 void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool largest_first,
-                       std::vector<pmd::Item>& items, std::vector<pmd::Unit>& units) {
+                       std::vector<pmd::Item>& items, std::vector<pmd::Unit>& units,
+                       const uint32_t* tiles_per_block) {
     // Order: conversions, outer forward stages, fused tiles by size class,
     // outer inverse stages, CRT blocks.  Every dependency of an item points to
     // an earlier item, so claiming items in order cannot deadlock.
@@ -224,10 +228,11 @@ void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool la
             for (uint32_t poly = 0; poly < 2; ++poly) items.push_back({pmd::kItemConv, j, poly, blk});
     }
     for (uint32_t j : order)
-        if (jobs[j].outer)
+        if (jobs[j].outer && !jobs[j].fused)
             for (uint32_t x = 0; x < 2 * k; ++x) items.push_back({pmd::kItemOuterFwd, j, x, 0});
     for (uint32_t tlog = 15; tlog >= 5; --tlog) {
-        const uint32_t per_item = 1u << (15 - tlog);  // 1024 threads x 32 residues / tile size
+        // tiles per block: by default 1024 threads x 32 residues / tile size
+        const uint32_t per_item = tiles_per_block ? tiles_per_block[tlog] : pm_default_batch_tiles(tlog);
         // prime-major: blocks that run at the same time (and the blocks one
         // XCD receives round-robin) share a prime, so its twiddle tables stay
         // in that XCD's L2 instead of every block streaming its own
@@ -244,7 +249,7 @@ void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool la
         }
     }
     for (uint32_t j : order)
-        if (jobs[j].outer)
+        if (jobs[j].outer && !jobs[j].fused)
             for (uint32_t prime = 0; prime < k; ++prime) items.push_back({pmd::kItemOuterInv, j, prime, 0});
     for (uint32_t j : order) {
         const uint32_t blocks = ceil_div(1u << jobs[j].logn, pmd::kBlock);
@@ -257,11 +262,10 @@ extern "C" {
 int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
                     const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out) {
     if (!ctx || !ns || !out || count == 0) return PM_EINVAL;
-    if (flags & ~uint32_t(PM_BATCH_PERSISTENT | PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION | PM_BATCH_SCALAR_CRT))
-        return PM_EINVAL;
-    if ((flags & PM_BATCH_PERSISTENT) &&
-        (flags & (PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION | PM_BATCH_SCALAR_CRT)))
-        return PM_EINVAL;
+    constexpr uint32_t kPhasedFlags = PM_BATCH_PHASED | PM_BATCH_SCALAR_CONVERSION | PM_BATCH_SCALAR_CRT |
+                                      PM_BATCH_SEPARATE_OUTER;
+    if (flags & ~uint32_t(PM_BATCH_PERSISTENT | kPhasedFlags)) return PM_EINVAL;
+    if ((flags & PM_BATCH_PERSISTENT) && (flags & kPhasedFlags)) return PM_EINVAL;
     *out = nullptr;
     // The persistent kernel hung the GPU on a uniform batch after a change to
     // the tile code it shares; no cause was found, so the mode was removed.
@@ -276,6 +280,8 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
     b->count = count;
     b->scalar_conversion = (flags & PM_BATCH_SCALAR_CONVERSION) != 0;
     b->scalar_crt = (flags & PM_BATCH_SCALAR_CRT) != 0;
+    // the scalar conversion and CRT keep consecutive coefficients: outer stages as passes of their own
+    b->separate_outer = (flags & (PM_BATCH_SEPARATE_OUTER | PM_BATCH_SCALAR_CONVERSION | PM_BATCH_SCALAR_CRT)) != 0;
     const int st = pm_batch_build(ctx, b, ns, d_a, d_b, d_c);
     if (st != PM_OK) {
         delete b;
@@ -346,6 +352,7 @@ int pm_plan_worklist(uint32_t k, uint32_t count, const uint32_t* ns, uint32_t* n
         if (ns[i] < 32 || (ns[i] & (ns[i] - 1)) || ns[i] > (1u << 17)) return PM_EUNSUPPORTED;
         jobs[i].logn = uint32_t(__builtin_ctz(ns[i]));
         jobs[i].outer = pm_default_batch_outer(jobs[i].logn);
+        jobs[i].fused = pmd::outer_fusable(jobs[i].logn, jobs[i].outer) ? 1u : 0u;
     }
     std::vector<pmd::Item> it;
     std::vector<pmd::Unit> un;

@@ -13,8 +13,11 @@ struct pm_context;
 // Dependency-ordered work list of a set of products (shared by the batch and
 // the serving loop).
 bool pm_phase_ranges(const std::vector<pmd::Item>& items, pmd::PhaseRanges& ranges);
+// tiles_per_block[tlog]: tiles (units) per tile item of each size class,
+// NULL for the default (a 1024-thread block: 2^(15 - tlog) tiles)
 void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool largest_first,
-                       std::vector<pmd::Item>& items, std::vector<pmd::Unit>& units);
+                       std::vector<pmd::Item>& items, std::vector<pmd::Unit>& units,
+                       const uint32_t* tiles_per_block = nullptr);
 
 struct pm_batch {
     pm_context* ctx = nullptr;
@@ -25,6 +28,7 @@ struct pm_batch {
     pmd::PhaseRanges ranges{};
     bool scalar_conversion = false; // phased only: scalar limb reduction instead of MFMA
     bool scalar_crt = false;        // phased only: vector-ALU CRT sums instead of MFMA
+    bool separate_outer = false;    // outer NTT stages as passes of their own, not inside conversion / CRT
     std::vector<void*> allocations;
     uint64_t device_bytes = 0;
     hipGraph_t graph = nullptr;

@@ -63,7 +63,16 @@ struct Job {
     uint32_t* c;         // [n][28] result (device)
     uint32_t* ra;        // [k][n] residues of a, reused for CRT inputs
     uint32_t* rb;        // [k][n] residues of b
+    // 1: the outer stages (outer = 1 or 2, n >= 256) run inside the
+    // conversion (forward) and the CRT (inverse) instead of as passes of
+    // their own; both then take a group's four column tiles of 16
+    // coefficients a quarter of n apart, so each lane holds a whole radix-4
+    // column of its prime
+    uint32_t fused;
 };
+
+// whether a product of 2^logn with `outer` outer stages can fuse them
+constexpr bool outer_fusable(uint32_t logn, uint32_t outer) { return outer >= 1 && outer <= 2 && logn >= 8; }
 
 // Batch work list: the items of each phase (conversion groups, outer-stage
 // columns, tiles, CRT groups); a tile item covers one or more units.

@@ -45,10 +45,12 @@ struct PhaseRanges {
     uint32_t conv_first = 0, conv_count = 0;
     uint32_t fwd_first = 0, fwd_count = 0;
     uint32_t tile_first[16] = {}, tile_count[16] = {};
+    uint32_t tile_units[16] = {};  // most units (tiles) in one item of the class: tiles per block
     uint32_t inv_first = 0, inv_count = 0;
     uint32_t crt_first = 0, crt_count = 0;
 };
-// phase kinds (tile phases carry the tile size class: kPhaseTile | tlog << 8)
+// phase kinds (tile phases carry the tile size class and log2 of the tiles
+// per block: kPhaseTile | tlog << 8 | log2(tiles) << 16)
 enum PhaseKind : uint32_t {
     kPhaseConv = 0, kPhaseOuterFwd = 1, kPhaseTile = 2, kPhaseOuterInv = 3, kPhaseCrt = 4, kPhaseConvScalar = 5,
     kPhaseCrtScalar = 6

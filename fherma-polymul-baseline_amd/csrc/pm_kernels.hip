@@ -415,6 +415,7 @@ struct ConvLds {
     uint32_t p[64];
     uint32_t sh16[64];
     int4 corr[64];
+    Tw otw[64][3];  // forward twiddles 1..3 of each prime: the fused outer stages
 };
 
 // whole block; ends with a barrier.  Slice [quad_begin, quad_end) of the quads.
@@ -427,6 +428,8 @@ __device__ __forceinline__ void conv_tables_to_lds(const Tables& tb, ConvLds& L,
         L.p[i] = tb.primes[pi];
         L.sh16[i] = tb.conv_sh16[pi];
         L.corr[i] = i < tb.quads * 4u ? reinterpret_cast<const int4*>(tb.conv_corr)[i] : make_int4(0, 0, 0, 0);
+        for (uint32_t j = 0; j < 3; ++j)  // twiddle 1 + j of the 2^max_logn table (every N shares its prefix)
+            L.otw[i][j] = (1u + j) < (1u << tb.max_logn) ? tb.fwd[(size_t(pi) << tb.max_logn) + 1u + j] : Tw{0u, 0u};
     }
     __syncthreads();
 }
@@ -441,14 +444,16 @@ typedef __attribute__((address_space(1))) uint32_t* global_store_u32;
 // coefficient (l >= 48, second load) and coefficients past c_end are read as
 // zero by the hardware range check, without a memory access.
 
+// Column tile ct holds coefficients c0 + ct * cstride + (0..15): cstride 16
+// for 64 consecutive coefficients, n/4 for the fused outer stages' layout.
 __device__ __forceinline__ void conv_load(const uint32_t* __restrict__ coef, uint32_t c_end, uint32_t c0,
-                                          uint32_t lane, v4u (&v0)[4], v4u (&v1)[4]) {
+                                          uint32_t lane, v4u (&v0)[4], v4u (&v1)[4], uint32_t cstride = 16u) {
     const uint32_t col = lane & 15u, kg = lane >> 4;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<uint32_t*>(coef), short(0), int(c_end * (kLimbs * 4u)), int(kRawBufferWord3));
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
-        const uint32_t byte0 = (c0 + 16u * ct + col) * (kLimbs * 4u) + 16u * kg;
+        const uint32_t byte0 = (c0 + cstride * ct + col) * (kLimbs * 4u) + 16u * kg;
         v0[ct] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, byte0, 0, 0);
         v1[ct] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, kg < 3 ? byte0 + 64u : kOutOfRange, 0, 0);
     }
@@ -475,11 +480,16 @@ __device__ __forceinline__ void conv_load_host(const uint32_t* __restrict__ coef
 
 // Residues of the group for quads [quad_begin, quad_end) of primes; n is the
 // row stride of the residue array, coefficients at or past c_end are not
-// stored.
+// stored.  With outer = 1 or 2 (cstride = n/4) the lane's four residues are
+// one column of the outer radix-2 / radix-4 Cooley-Tukey stages of its prime
+// (positions x, x + n/4, x + n/2, x + 3n/4), which are applied before the
+// store -- the outer stages of the forward NTT without an HBM pass of their
+// own.
 __device__ __forceinline__ void conv_compute(const Tables& tb, const v4u (&v0)[4], const v4u (&v1)[4], uint32_t n,
                                              uint32_t c_end, uint32_t c0, uint32_t* __restrict__ res_any,
                                              const ConvLds& L, uint32_t lane, uint32_t quad_begin = 0,
-                                             uint32_t quad_end = 0xFFFFFFFFu) {
+                                             uint32_t quad_end = 0xFFFFFFFFu, uint32_t cstride = 16u,
+                                             uint32_t outer = 0u) {
     const global_store_u32 res = (global_store_u32)res_any;
     const uint32_t col = lane & 15u, kg = lane >> 4;
     v4i b0[4], b1[4];
@@ -500,21 +510,38 @@ __device__ __forceinline__ void conv_compute(const Tables& tb, const v4u (&v0)[4
         const uint32_t p = L.p[prime], p2 = 2u * p;
         const int4 corr = L.corr[prime];
         const Tw c16{65536u, L.sh16[prime]};
+        uint32_t r[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
             v4i acc = v4i{0, 0, 0, 0};
             acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w0, b0[ct], acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w1, b1[ct], acc, 0, 0, 0);
-            const uint32_t c = c0 + 16u * ct + col;
             // exact digit sums T_d = sum_j a_j * digit_d(256^j mod p)
             const int32_t t0 = acc[0] + corr.x, t1 = acc[1] + corr.y;
             const int32_t t2 = acc[2] + corr.z, t3 = acc[3] + corr.w;
             const int32_t u = t0 + t1 * 256;  // |u| < 2^30
             const int32_t w = t2 + t3 * 256;  // |w| < 2^30
             uint32_t x = csub(shoup(uint32_t(w) + p2, c16, p), p);   // w * 2^16 mod p, [0,p)
-            uint32_t r = uint32_t(u) + p2 + x;                          // < 2^32, > 0
-            r = csub(csub(r, p2), p2);                                  // [0,2p)
-            if (live && c < c_end) res[size_t(prime) * n + c] = r;
+            uint32_t v = uint32_t(u) + p2 + x;                          // < 2^32, > 0
+            r[ct] = csub(csub(v, p2), p2);                              // [0,2p)
+        }
+        if (outer >= 1u) {
+            // bit log2(n)-1 pairs x with x + n/2 (twiddle 1); bit log2(n)-2
+            // pairs x with x + n/4 in each half (twiddles 2 and 3): the
+            // stages of the separate outer pass, in the same order; outputs
+            // in [0,4p) as the tile's first stage takes them
+            const Tw w1 = L.otw[prime][0];
+            ct_bfly(r[0], r[2], w1, p, p2);
+            ct_bfly(r[1], r[3], w1, p, p2);
+            if (outer >= 2u) {
+                ct_bfly(r[0], r[1], L.otw[prime][1], p, p2);
+                ct_bfly(r[2], r[3], L.otw[prime][2], p, p2);
+            }
+        }
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            const uint32_t c = c0 + cstride * ct + col;
+            if (live && c < c_end) res[size_t(prime) * n + c] = r[ct];
         }
     }
 }
@@ -523,10 +550,11 @@ __device__ __forceinline__ void conv_compute(const Tables& tb, const v4u (&v0)[4
 __device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* __restrict__ coef, uint32_t n,
                                              uint32_t c_end, uint32_t c0, uint32_t* __restrict__ res,
                                              const ConvLds& L, uint32_t lane, uint32_t quad_begin = 0,
-                                             uint32_t quad_end = 0xFFFFFFFFu) {
+                                             uint32_t quad_end = 0xFFFFFFFFu, uint32_t cstride = 16u,
+                                             uint32_t outer = 0u) {
     v4u v0[4], v1[4];
-    conv_load(coef, c_end, c0, lane, v0, v1);
-    conv_compute(tb, v0, v1, n, c_end, c0, res, L, lane, quad_begin, quad_end);
+    conv_load(coef, c_end, c0, lane, v0, v1, cstride);
+    conv_compute(tb, v0, v1, n, c_end, c0, res, L, lane, quad_begin, quad_end, cstride, outer);
 }
 
 // Scalar limb reduction, the comparison path of the MFMA conversion (role of
@@ -669,52 +697,87 @@ __device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], 
 // coefficients past n are never stored).
 typedef const __attribute__((address_space(1))) uint32_t* global_u32;
 
+// Column tile ct holds coefficients c0 + ct * cstride + (0..15) (cstride 16:
+// 64 consecutive coefficients; n/4: the fused outer stages' layout).
+// v[s][ct][j]: residue of prime 16s + 4(lane/16) + j at column tile ct.
 __device__ __forceinline__ void crt_load_group(const uint32_t* t_res_any, uint32_t n, uint32_t k,
-                                               uint32_t c0, uint32_t lane, v4i (&bop)[4][4]) {
+                                               uint32_t c0, uint32_t lane, uint32_t (&v)[4][4][4],
+                                               uint32_t cstride = 16u) {
     const global_u32 t_res = (global_u32)t_res_any;  // device memory: global (not flat) loads
     typedef const __attribute__((address_space(1))) char* global_bytes;
     const uint32_t g = lane >> 4, col = lane & 15u;
-    if (c0 + kCrtWaveCoefs <= n) {
-        // whole group in range: one offset per prime, the 4 column tiles at
-        // immediate offsets of 64 bytes (< 64 * 2^17 words: 32-bit byte offsets)
-        uint32_t v[4][4][4];
+    if (c0 + 3u * cstride + 16u <= n) {
+        // whole group in range: buffer loads, one offset register per prime
+        // and the 4 column tiles 4 * cstride bytes apart as a scalar offset
+        // (k * n * 4 <= 64 * 2^17 * 4 bytes: 32-bit offsets)
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint32_t*>(wave_uniform(t_res_any)), short(0), int(k * n * 4u), int(kRawBufferWord3));
+        const uint32_t cbytes = __builtin_amdgcn_readfirstlane(4u * cstride);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 uint32_t i = 16u * s + 4u * g + j;
                 i = i < k ? i : k - 1u;
-                const global_bytes row = reinterpret_cast<global_bytes>(t_res) + ((i * n + c0 + col) << 2);
+                const uint32_t row = (i * n + c0 + col) << 2;
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct) v[s][ct][j] = *reinterpret_cast<global_u32>(row + 64 * ct);
+                for (int ct = 0; ct < 4; ++ct)
+                    v[s][ct][j] = __builtin_amdgcn_raw_buffer_load_b32(rsrc, row, cbytes * uint32_t(ct), 0);
             }
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int ct = 0; ct < 4; ++ct)
-                bop[s][ct] = v4i{int(v[s][ct][0]), int(v[s][ct][1]), int(v[s][ct][2]), int(v[s][ct][3])};
         return;
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
-            uint32_t c = c0 + 16u * ct + col;
+            uint32_t c = c0 + cstride * ct + col;
             c = c < n ? c : n - 1u;
-            uint32_t v[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 uint32_t i = 16u * s + 4u * g + j;
                 i = i < k ? i : k - 1u;
-                v[j] = *reinterpret_cast<global_u32>(reinterpret_cast<global_bytes>(t_res) + ((i * n + c) << 2));
+                v[s][ct][j] =
+                    *reinterpret_cast<global_u32>(reinterpret_cast<global_bytes>(t_res) + ((i * n + c) << 2));
             }
-            bop[s][ct] = v4i{int(v[0]), int(v[1]), int(v[2]), int(v[3])};
         }
 }
 
-__device__ __forceinline__ void crt_mfma_group(const Tables& tb, const v4i (&bop)[4][4], const uint4* a_lds,
+// The outer inverse stages of the fused layout on a loaded group: for each of
+// the lane's 16 primes its four values (positions x, x + n/4, x + n/2,
+// x + 3n/4 of the tile's scaled output, in [0,p)) go through the radix-2 or
+// radix-4 Gentleman-Sande stages of the separate outer pass, in the same
+// order, and back to [0,p) (the CRT needs canonical residues).  ot holds per
+// prime {p, w1, w1', w2} {w2', w3, w3', 0}: the inverse twiddles 1..3.
+__device__ __forceinline__ void crt_outer_inverse(uint32_t (&b)[4][4][4], const uint4* ot, uint32_t k, uint32_t g,
+                                                  uint32_t outer) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t i = 16u * s + 4u * g + j;
+            i = i < k ? i : k - 1u;  // padding rows meet zero columns of the digit matrix
+            const uint4 c0 = ot[2u * i], c1 = ot[2u * i + 1u];
+            const uint32_t p = c0.x, p2 = 2u * p;
+            uint32_t x0 = b[s][0][j], x1 = b[s][1][j], x2 = b[s][2][j], x3 = b[s][3][j];
+            if (outer >= 2u) {
+                gs_bfly(x0, x1, Tw{c0.w, c1.x}, p, p2);
+                gs_bfly(x2, x3, Tw{c1.y, c1.z}, p, p2);
+            }
+            gs_bfly(x0, x2, Tw{c0.y, c0.z}, p, p2);
+            gs_bfly(x1, x3, Tw{c0.y, c0.z}, p, p2);
+            b[s][0][j] = csub(x0, p);
+            b[s][1][j] = csub(x1, p);
+            b[s][2][j] = csub(x2, p);
+            b[s][3][j] = csub(x3, p);
+            // one prime's constants at a time: left to itself the compiler
+            // loads all 16 primes' ahead and spills
+            __builtin_amdgcn_sched_barrier(0);
+        }
+}
+
+__device__ __forceinline__ void crt_mfma_group(const Tables& tb, uint32_t (&bop)[4][4][4], const uint4* a_lds,
                                                const int4* corr_lds, uint8_t* wave_lds, uint32_t lane, uint32_t n,
-                                               uint32_t c0, uint32_t* __restrict__ out) {
+                                               uint32_t c0, uint32_t* __restrict__ out, uint32_t cstride = 16u) {
     const uint32_t g = lane >> 4, col = lane & 15u;
     uint32_t* lo = reinterpret_cast<uint32_t*>(wave_lds);
     int16_t* hi = reinterpret_cast<int16_t*>(wave_lds + kCrtWords * kCrtLdsStride * 4);
@@ -723,7 +786,8 @@ __device__ __forceinline__ void crt_mfma_group(const Tables& tb, const v4i (&bop
     for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
-            b[s][ct] = bop[s][ct] ^ v4i{int(0x80808080u), int(0x80808080u), int(0x80808080u), int(0x80808080u)};
+            b[s][ct] = v4i{int(bop[s][ct][0]), int(bop[s][ct][1]), int(bop[s][ct][2]), int(bop[s][ct][3])} ^
+                       v4i{int(0x80808080u), int(0x80808080u), int(0x80808080u), int(0x80808080u)};
 #pragma unroll
     for (int rt = 0; rt < 8; ++rt) {
         const uint32_t word = 4u * rt + g;
@@ -757,7 +821,7 @@ __device__ __forceinline__ void crt_mfma_group(const Tables& tb, const v4i (&bop
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t c = c0 + lane;
+    const uint32_t c = c0 + (lane >> 4) * cstride + (lane & 15u);  // lane = 16 * column tile + column
     uint32_t x[29];
     int64_t carry = 0;
 #pragma unroll
@@ -775,17 +839,29 @@ __device__ __forceinline__ void crt_mfma_group(const Tables& tb, const v4i (&bop
     __builtin_amdgcn_wave_barrier();
 }
 
-// Shared operand tables to LDS (whole block), then a barrier.
-__device__ __forceinline__ void crt_tables_to_lds(const Tables& tb, uint4* a_lds, int4* corr_lds) {
+// Shared operand tables to LDS (whole block), then a barrier.  ot_lds: per
+// prime the modulus and inverse twiddles 1..3 of the fused outer stages.
+__device__ __forceinline__ void crt_tables_to_lds(const Tables& tb, uint4* a_lds, int4* corr_lds, uint4* ot_lds) {
     for (uint32_t i = threadIdx.x; i < 8u * 4u * 64u; i += blockDim.x) a_lds[i] = tb.crt_a[i];
     for (uint32_t i = threadIdx.x; i < 32u; i += blockDim.x) corr_lds[i] = tb.crt_corr[i];
+    for (uint32_t i = threadIdx.x; i < 64u; i += blockDim.x) {
+        const uint32_t pi = i < tb.k ? i : 0u;
+        const Tw* inv = tb.inv + (size_t(pi) << tb.max_logn);
+        Tw w[3] = {};
+        for (uint32_t j = 0; j < 3; ++j)
+            if ((1u + j) < (1u << tb.max_logn)) w[j] = inv[1u + j];
+        ot_lds[2u * i] = make_uint4(tb.primes[pi], w[0].value, w[0].shoup, w[1].value);
+        ot_lds[2u * i + 1u] = make_uint4(w[1].shoup, w[2].value, w[2].shoup, 0u);
+    }
     __syncthreads();
 }
 
-// LDS of a CRT block with W waves: A operands, row corrections, per-wave rows
+// LDS of a CRT block with W waves: A operands, row corrections, outer-stage
+// constants, per-wave rows
+constexpr size_t kCrtSharedLds = 8u * 4u * 64u * sizeof(uint4) + 32u * sizeof(int4) + 128u * sizeof(uint4);
 template <int W>
 constexpr size_t crt_block_lds() {
-    return 8u * 4u * 64u * sizeof(uint4) + 32u * sizeof(int4) + size_t(W) * kCrtWaveLds;
+    return kCrtSharedLds + size_t(W) * kCrtWaveLds;
 }
 
 // ---------------------------------------------------------------------------
@@ -805,7 +881,11 @@ __global__ void __launch_bounds__(kBlock) k_convert(Tables tb, Job job, uint32_t
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t c0 = c_first + blockIdx.x * blockDim.x + wave * kConvPerWave;  // one coefficient per thread
     if (c0 >= c_end) return;
-    convert_wave(tb, poly ? job.b : job.a, n, c_end, c0, poly ? job.rb : job.ra, L, lane, quad_begin, quad_end);
+    if (job.fused)  // whole range (c_first = 0, c_end = n): group c0/64 at quarter offsets, outer stages applied
+        convert_wave(tb, poly ? job.b : job.a, n, n, c0 / 4u, poly ? job.rb : job.ra, L, lane, quad_begin, quad_end,
+                     n / 4u, job.outer);
+    else
+        convert_wave(tb, poly ? job.b : job.a, n, c_end, c0, poly ? job.rb : job.ra, L, lane, quad_begin, quad_end);
 }
 
 __global__ void __launch_bounds__(256) k_convert_scalar(Tables tb, Job job) {
@@ -983,17 +1063,20 @@ __global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, 
     struct Where {
         const uint32_t* coef;
         uint32_t* res;
-        uint32_t n, c0;
+        uint32_t n, c0, cstride, outer;
+        bool valid;
     };
     auto where = [&](uint32_t gi) {
         const Item it = ba.items[first + gi / per_item];
         const Job job = ba.jobs[it.job];
-        return Where{it.a0 ? job.b : job.a, it.a0 ? job.rb : job.ra, 1u << job.logn,
-                     it.a1 * kConvPerBlock + (gi % per_item) * kConvPerWave};
+        const uint32_t n = 1u << job.logn, c0 = it.a1 * kConvPerBlock + (gi % per_item) * kConvPerWave;
+        // fused outer stages: group c0/64 takes 16 coefficients from each quarter
+        return job.fused ? Where{it.a0 ? job.b : job.a, it.a0 ? job.rb : job.ra, n, c0 / 4u, n / 4u, job.outer, c0 < n}
+                         : Where{it.a0 ? job.b : job.a, it.a0 ? job.rb : job.ra, n, c0, 16u, 0u, c0 < n};
     };
     Where cur = where(g);
     v4u v0[4], v1[4];
-    conv_load(cur.coef, cur.n, cur.c0, lane, v0, v1);
+    conv_load(cur.coef, cur.n, cur.c0, lane, v0, v1, cur.cstride);
     for (;;) {
         const uint32_t gn = g + stride;
         const bool more = gn < total;
@@ -1001,9 +1084,10 @@ __global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, 
         v4u n0[4], n1[4];
         if (more) {
             nxt = where(gn);
-            conv_load(nxt.coef, nxt.n, nxt.c0, lane, n0, n1);
+            conv_load(nxt.coef, nxt.n, nxt.c0, lane, n0, n1, nxt.cstride);
         }
-        if (cur.c0 < cur.n) conv_compute(tb, v0, v1, cur.n, cur.n, cur.c0, cur.res, L, lane);
+        if (cur.valid) conv_compute(tb, v0, v1, cur.n, cur.n, cur.c0, cur.res, L, lane, 0u, 0xFFFFFFFFu, cur.cstride,
+                                    cur.outer);
         if (!more) break;
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
@@ -1047,6 +1131,15 @@ __global__ void __launch_bounds__(tile_block_threads(TLOG)) k_items_tile(Tables 
     tile_item<TLOG>(tb, ba, it, lds);
 }
 
+// One group of 64 coefficients of the CRT: residues t ([k][n]), output, its
+// first coefficient and column-tile stride, and the fused outer stages.
+// A group with nothing to store has c0 >= n and cstride 16.
+struct CrtGroup {
+    const uint32_t* t;
+    uint32_t* out;
+    uint32_t n, c0, cstride, outer;
+};
+
 // Groups of 64 coefficients, looped by W-wave blocks (wave w of block b takes
 // groups b*W + w, + grid*W, ...); the next group's residues load while the
 // current one is computed.
@@ -1055,41 +1148,43 @@ __device__ __forceinline__ void crt_group_loop(const Tables& tb, uint32_t groups
     extern __shared__ uint8_t crt_lds[];
     uint4* a_lds = reinterpret_cast<uint4*>(crt_lds);
     int4* corr_lds = reinterpret_cast<int4*>(crt_lds + 8u * 4u * 64u * sizeof(uint4));
-    uint8_t* rows = crt_lds + 8u * 4u * 64u * sizeof(uint4) + 32u * sizeof(int4);
-    crt_tables_to_lds(tb, a_lds, corr_lds);
+    uint4* ot_lds = reinterpret_cast<uint4*>(crt_lds + 8u * 4u * 64u * sizeof(uint4) + 32u * sizeof(int4));
+    uint8_t* rows = crt_lds + kCrtSharedLds;
+    crt_tables_to_lds(tb, a_lds, corr_lds, ot_lds);
     // the wave index is the same in every lane: make it (and all that follows
     // from it: group, product, pointers) scalar
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * W;
     uint32_t gi = blockIdx.x * W + wave;
     if (gi >= groups) return;  // whole wave; no block barriers follow
-    const uint32_t* t;
-    uint32_t* out;
-    uint32_t n, c0;
-    where(gi, t, out, n, c0);
-    v4i cur[4][4];
-    crt_load_group(t, n, tb.k, c0, lane, cur);
+    CrtGroup cur_g;
+    where(gi, cur_g);
+    uint32_t cur[4][4][4];
+    crt_load_group(cur_g.t, cur_g.n, tb.k, cur_g.c0, lane, cur, cur_g.cstride);
     uint8_t* my_rows = rows + wave * kCrtWaveLds;
     for (;;) {
+        // the fused outer stages on this group before the next one's loads
+        // are issued: both sets of 64 values at once do not fit the registers
+        if (cur_g.outer) {
+            crt_outer_inverse(cur, ot_lds, tb.k, lane >> 4, cur_g.outer);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         const uint32_t gn = gi + stride;
         const bool more = gn < groups;
-        const uint32_t* t2;
-        uint32_t* out2;
-        uint32_t n2, c2;
-        where(more ? gn : gi, t2, out2, n2, c2);
-        v4i nxt[4][4];
-        crt_load_group(t2, n2, tb.k, c2, lane, nxt);  // a repeat of this group when none is left
-        crt_mfma_group(tb, cur, a_lds, corr_lds, my_rows, lane, n, c0, out);
+        CrtGroup nxt_g;
+        where(more ? gn : gi, nxt_g);
+        uint32_t nxt[4][4][4];
+        crt_load_group(nxt_g.t, nxt_g.n, tb.k, nxt_g.c0, lane, nxt, nxt_g.cstride);  // a repeat when none is left
+        crt_mfma_group(tb, cur, a_lds, corr_lds, my_rows, lane, cur_g.n, cur_g.c0, cur_g.out, cur_g.cstride);
         if (!more) break;
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) cur[s][ct] = nxt[s][ct];
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cur[s][ct][j] = nxt[s][ct][j];
         gi = gn;
-        t = t2;
-        out = out2;
-        n = n2;
-        c0 = c2;
+        cur_g = nxt_g;
     }
 }
 
@@ -1100,11 +1195,10 @@ __global__ void __launch_bounds__(kCrtPlainWaves * 64) k_crt_mfma(Tables tb, Job
                                                                   uint32_t c_end) {
     const uint32_t n = 1u << job.logn;
     crt_group_loop<kCrtPlainWaves>(tb, (c_end - c_first + kCrtWaveCoefs - 1) / kCrtWaveCoefs,
-                                  [&](uint32_t gi, const uint32_t*& t, uint32_t*& out, uint32_t& nn, uint32_t& c0) {
-                                      t = job.ra;
-                                      out = job.c;
-                                      nn = n;
-                                      c0 = c_first + gi * kCrtWaveCoefs;
+                                  [&](uint32_t gi, CrtGroup& w) {
+                                      // fused: whole range, group gi at quarter offsets
+                                      w = job.fused ? CrtGroup{job.ra, job.c, n, 16u * gi, n / 4u, job.outer}
+                                                    : CrtGroup{job.ra, job.c, n, c_first + gi * kCrtWaveCoefs, 16u, 0u};
                                   });
 }
 
@@ -1113,15 +1207,14 @@ __global__ void __launch_bounds__(kCrtPlainWaves * 64) k_crt_mfma(Tables tb, Job
 __global__ void __launch_bounds__(kCrtMfmaWaves * 64) k_items_crt_mfma(Tables tb, BatchArgs ba, uint32_t first,
                                                                        uint32_t count) {
     constexpr uint32_t per_item = kBlock / kCrtWaveCoefs;
-    crt_group_loop<kCrtMfmaWaves>(tb, count * per_item,
-                                  [&](uint32_t gi, const uint32_t*& t, uint32_t*& out, uint32_t& nn, uint32_t& c0) {
-                                      const Item it = ba.items[first + gi / per_item];
-                                      const Job job = ba.jobs[it.job];
-                                      t = job.ra;
-                                      out = job.c;
-                                      nn = 1u << job.logn;
-                                      c0 = it.a0 * kBlock + (gi % per_item) * kCrtWaveCoefs;
-                                  });
+    crt_group_loop<kCrtMfmaWaves>(tb, count * per_item, [&](uint32_t gi, CrtGroup& w) {
+        const Item it = ba.items[first + gi / per_item];
+        const Job job = ba.jobs[it.job];
+        const uint32_t n = 1u << job.logn, c0 = it.a0 * kBlock + (gi % per_item) * kCrtWaveCoefs;
+        // fused: group c0/64 at quarter offsets; a group past n stores nothing
+        w = job.fused && c0 < n ? CrtGroup{job.ra, job.c, n, c0 / 4u, n / 4u, job.outer}
+                                : CrtGroup{job.ra, job.c, n, c0, 16u, 0u};
+    });
 }
 
 __global__ void __launch_bounds__(kBlock) k_items_crt(Tables tb, BatchArgs ba, uint32_t first) {
@@ -1349,6 +1442,7 @@ hipError_t launch_convert_range(const Tables& tb, const Job& job, uint32_t c_fir
                                 uint32_t poly_first, uint32_t polys, hipStream_t stream) {
     const uint32_t n = 1u << job.logn;
     if (c_first >= c_end || c_end > n || polys < 1 || poly_first + polys > 2) return hipErrorInvalidValue;
+    if (job.fused && (c_first != 0 || c_end != n || !outer_fusable(job.logn, job.outer))) return hipErrorInvalidValue;
     const uint32_t count = c_end - c_first;
     // 4-wave blocks, and the primes split over up to 4 slices while that
     // leaves fewer than ~4 waves per CU: a single product (or one chunk of it)
@@ -1370,8 +1464,9 @@ hipError_t launch_convert(const Tables& tb, const Job& job, hipStream_t stream) 
 
 hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t stream) {
     if (u > job.logn || job.logn - u > 15 || u > 2) return hipErrorInvalidValue;
+    if (job.fused && (u != job.outer || !outer_fusable(job.logn, u))) return hipErrorInvalidValue;
     const uint32_t tlog = job.logn - u;
-    if (u > 0) {
+    if (u > 0 && !job.fused) {  // fused: the conversion and the CRT run the outer stages
         const uint32_t T = 1u << tlog;
         const dim3 grid((T + 255) / 256, 2 * tb.k);
         if (u == 1) hipLaunchKernelGGL(k_outer_forward<1>, grid, dim3(256), 0, stream, tb, job);
@@ -1381,7 +1476,7 @@ hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t 
     }
     hipError_t e = launch_tile_dispatch(tlog, tb, job, u, stream);
     if (e != hipSuccess) return e;
-    if (u > 0) {
+    if (u > 0 && !job.fused) {
         const uint32_t T = 1u << tlog;
         const dim3 grid((T + 255) / 256, tb.k);
         if (u == 1) hipLaunchKernelGGL(k_outer_inverse<1>, grid, dim3(256), 0, stream, tb, job);
@@ -1391,12 +1486,18 @@ hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t 
     return e;
 }
 
+// blocks of `tiles` tiles (whole waves; padding tiles, if any, store nothing):
+// fewer tiles per block leave room for several blocks per CU, so one block's
+// loads and barriers overlap another's arithmetic
 template <int TLOG>
-static void launch_items_tile(const Tables& tb, const BatchArgs& ba, uint32_t first, uint32_t count,
-                              hipStream_t stream) {
-    hipLaunchKernelGGL(k_items_tile<TLOG>, dim3(count), dim3(tile_block_threads(TLOG)),
-                       size_t(kLdsWords) * sizeof(uint32_t), stream,
-                       tb, ba, first);
+static hipError_t launch_items_tile(const Tables& tb, const BatchArgs& ba, uint32_t first, uint32_t count,
+                                    uint32_t tiles, hipStream_t stream) {
+    using G = Geo<TLOG>;
+    while ((tiles << G::TPTLOG) < kWaveSize) tiles <<= 1;
+    if ((tiles << G::TPTLOG) > tile_block_threads(TLOG)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_items_tile<TLOG>, dim3(count), dim3(tiles << G::TPTLOG),
+                       size_t(tiles) * G::STRIDE * sizeof(uint32_t), stream, tb, ba, first);
+    return hipSuccess;
 }
 
 hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, uint32_t first, uint32_t count,
@@ -1426,22 +1527,26 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
         case kPhaseCrtScalar:
             hipLaunchKernelGGL(k_items_crt, dim3(count), dim3(kBlock), 0, stream, tb, ba, first);
             break;
-        case kPhaseTile:
-            switch (kind >> 8) {
-                case 5: launch_items_tile<5>(tb, ba, first, count, stream); break;
-                case 6: launch_items_tile<6>(tb, ba, first, count, stream); break;
-                case 7: launch_items_tile<7>(tb, ba, first, count, stream); break;
-                case 8: launch_items_tile<8>(tb, ba, first, count, stream); break;
-                case 9: launch_items_tile<9>(tb, ba, first, count, stream); break;
-                case 10: launch_items_tile<10>(tb, ba, first, count, stream); break;
-                case 11: launch_items_tile<11>(tb, ba, first, count, stream); break;
-                case 12: launch_items_tile<12>(tb, ba, first, count, stream); break;
-                case 13: launch_items_tile<13>(tb, ba, first, count, stream); break;
-                case 14: launch_items_tile<14>(tb, ba, first, count, stream); break;
-                case 15: launch_items_tile<15>(tb, ba, first, count, stream); break;
+        case kPhaseTile: {
+            const uint32_t tiles = 1u << ((kind >> 16) & 0xFu);
+            hipError_t e = hipSuccess;
+            switch ((kind >> 8) & 0xFFu) {
+                case 5: e = launch_items_tile<5>(tb, ba, first, count, tiles, stream); break;
+                case 6: e = launch_items_tile<6>(tb, ba, first, count, tiles, stream); break;
+                case 7: e = launch_items_tile<7>(tb, ba, first, count, tiles, stream); break;
+                case 8: e = launch_items_tile<8>(tb, ba, first, count, tiles, stream); break;
+                case 9: e = launch_items_tile<9>(tb, ba, first, count, tiles, stream); break;
+                case 10: e = launch_items_tile<10>(tb, ba, first, count, tiles, stream); break;
+                case 11: e = launch_items_tile<11>(tb, ba, first, count, tiles, stream); break;
+                case 12: e = launch_items_tile<12>(tb, ba, first, count, tiles, stream); break;
+                case 13: e = launch_items_tile<13>(tb, ba, first, count, tiles, stream); break;
+                case 14: e = launch_items_tile<14>(tb, ba, first, count, tiles, stream); break;
+                case 15: e = launch_items_tile<15>(tb, ba, first, count, tiles, stream); break;
                 default: return hipErrorInvalidValue;
             }
+            if (e != hipSuccess) return e;
             break;
+        }
         default:
             return hipErrorInvalidValue;
     }
@@ -1452,8 +1557,12 @@ std::vector<Phase> phase_list(const PhaseRanges& r, bool scalar_conversion, bool
     std::vector<Phase> out;
     if (r.conv_count) out.push_back({scalar_conversion ? kPhaseConvScalar : kPhaseConv, r.conv_first, r.conv_count});
     if (r.fwd_count) out.push_back({kPhaseOuterFwd, r.fwd_first, r.fwd_count});
-    for (uint32_t tlog = 15; tlog >= 5; --tlog)
-        if (r.tile_count[tlog]) out.push_back({kPhaseTile | (tlog << 8), r.tile_first[tlog], r.tile_count[tlog]});
+    for (uint32_t tlog = 15; tlog >= 5; --tlog) {
+        if (!r.tile_count[tlog]) continue;
+        uint32_t lg = 0;  // tiles per block: the class's largest item, rounded up to a power of two
+        while ((1u << lg) < r.tile_units[tlog]) ++lg;
+        out.push_back({kPhaseTile | (tlog << 8) | (lg << 16), r.tile_first[tlog], r.tile_count[tlog]});
+    }
     if (r.inv_count) out.push_back({kPhaseOuterInv, r.inv_first, r.inv_count});
     if (r.crt_count) out.push_back({scalar_crt ? kPhaseCrtScalar : kPhaseCrt, r.crt_first, r.crt_count});
     return out;
@@ -1486,6 +1595,7 @@ hipError_t launch_crt_range(const Tables& tb, const Job& job, uint32_t c_first, 
     const uint32_t n = 1u << job.logn;
     if (c_first >= c_end || c_end > n || c_first % kCrtWaveCoefs || (c_end != n && c_end % kCrtWaveCoefs))
         return hipErrorInvalidValue;
+    if (job.fused && (c_first != 0 || c_end != n || !outer_fusable(job.logn, job.outer))) return hipErrorInvalidValue;
     const uint32_t groups = (c_end - c_first + kCrtWaveCoefs - 1) / kCrtWaveCoefs;
     uint32_t grid = (groups + kCrtPlainWaves - 1) / kCrtPlainWaves;
     grid = grid < tb.cus ? grid : tb.cus;

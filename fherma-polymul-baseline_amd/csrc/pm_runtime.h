@@ -30,6 +30,9 @@ struct pm_context {
     // and batches / served launches
     uint32_t outer[pm::kMaxLogN + 1] = {};
     uint32_t batch_outer[pm::kMaxLogN + 1] = {};
+    // tiles per block of each tile size class 2^tlog in batches and served
+    // launches (blocks of tiles << (tlog - 5) threads)
+    uint32_t batch_tiles[16] = {};
     // host-buffer path: page-locked staging (inputs a|b and the output) with
     // its device views, output chunk events, and the product between submit
     // and collect
@@ -67,6 +70,8 @@ hipError_t pm_workspace_acquire(pm_context* ctx, hipStream_t stream);
 hipError_t pm_workspace_release(pm_context* ctx, hipStream_t stream);
 uint32_t pm_default_outer(uint32_t logn);
 uint32_t pm_default_batch_outer(uint32_t logn);
+// tiles per block that fill 1024 threads: 2^(15 - tlog)
+uint32_t pm_default_batch_tiles(uint32_t tlog);
 // whether `outer` passes leave a tile of size 2^lo..2^15 for products of 2^logn
 bool pm_outer_valid(uint32_t logn, uint32_t outer, uint32_t lo);
 uint32_t pm_io_chunks(const pm_context* ctx, uint32_t logn);

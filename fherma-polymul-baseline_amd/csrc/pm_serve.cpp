@@ -168,13 +168,14 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
         pmd::Job& job = L.h_jobs[j];
         job.logn = r->logn;
         job.outer = ctx->batch_outer[r->logn];
+        job.fused = pmd::outer_fusable(job.logn, job.outer) ? 1u : 0u;
         job.a = reinterpret_cast<const uint32_t*>(base);
         job.b = job.a + n * kLimbs;
         job.c = reinterpret_cast<uint32_t*>(base) + 2 * n * kLimbs;
         job.ra = reinterpret_cast<uint32_t*>(base + align_up(3 * n * kLimbs * 4));
         job.rb = job.ra + size_t(k) * n;
     }
-    pm_build_worklist(k, L.h_jobs, count, false, items, units);
+    pm_build_worklist(k, L.h_jobs, count, false, items, units, ctx->batch_tiles);
     if (items.size() > max_items || units.size() > max_units) {
         why = "work list exceeds the launch slot";
         return false;

@@ -121,6 +121,16 @@ int pm_get_outer(pm_context* ctx, uint32_t n, uint32_t* outer);
  * tile).  Set by `autotune_batch` in the Python mirror. */
 int pm_set_batch_outer(pm_context* ctx, uint32_t n, uint32_t outer);
 int pm_get_batch_outer(pm_context* ctx, uint32_t n, uint32_t* outer);
+/* Batch schedule (no counterpart in the reference; the role of the
+ * reference's compile-time tile and primes-per-block knobs,
+ * quartic/host_setup.h:11-12, quartic/solve.cu:325): tiles of size tile_n
+ * (a power of two in [32, 32768]) that one block holds, in batches and
+ * served launches created afterwards.  A block has tiles x tile_n / 32
+ * threads (32 residues each): a power of two with 2048 <= tiles x tile_n <=
+ * 32768; 0 restores the default, a 1024-thread block.  Smaller blocks let
+ * several run on one compute unit.  Set by `autotune_batch`. */
+int pm_set_batch_tiles_per_block(pm_context* ctx, uint32_t tile_n, uint32_t tiles);
+int pm_get_batch_tiles_per_block(pm_context* ctx, uint32_t tile_n, uint32_t* tiles);
 
 /* ---- stage entry points (device buffers, context stream) ------------------
  * The three kernels of the plain path, exposed so each can be checked alone.
@@ -157,6 +167,10 @@ typedef struct pm_batch pm_batch;
 #define PM_BATCH_PHASED 2u
 #define PM_BATCH_SCALAR_CONVERSION 4u /* phased: scalar limb reduction instead of int8 MFMA */
 #define PM_BATCH_SCALAR_CRT 8u        /* phased: CRT sums on the vector ALUs instead of int8 MFMA */
+/* phased: the outer NTT stages (N > tile size) as launches of their own over
+ * the residues in HBM, instead of inside the conversion (forward) and the
+ * MFMA CRT (inverse); implied by either scalar flag (comparison path) */
+#define PM_BATCH_SEPARATE_OUTER 16u
 int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const uint32_t* const* d_a,
                     const uint32_t* const* d_b, uint32_t* const* d_c, uint32_t flags, pm_batch** out);
 void pm_batch_destroy(pm_batch* batch);

@@ -241,3 +241,57 @@ def test_large_n_conversion_mfma_vs_scalar(pm, workload):
         for i in range(count):
             assert (outs[mode][i] == outs["phased"][i]).all(), (mode, i)
     ctx.close()
+
+
+@pytest.mark.parametrize("outer", [1, 2])
+def test_fused_outer_stages_match_separate_passes(pm, orc, workload, outer):
+    """Outer NTT stages run inside the conversion and the CRT (quarter-offset
+    column tiles) give the same words as the same stages run as passes of
+    their own, for every N the fusion covers (2^8 .. 2^17), and the oracle's
+    product where the oracle is quick."""
+    q = workload.modulus(3)
+    ctx = pm.Context(q, 1 << 17)
+    ns = [n for n in (256, 512, 1024, 4096, 16384, 32768, 65536, 131072) if (n.bit_length() - 1) - outer <= 15]
+    for n in ns:
+        ctx.set_batch_outer(n, outer)
+    say("fused vs separate outer", outer, ns)
+    inputs = [workload.family("dense-random-1", n, q, 700 + i) for i, n in enumerate(ns)]
+    outs = {}
+    for mode in ("phased", "phased-separate-outer"):
+        batch = pm.Batch(ctx, ns, mode=mode)
+        for i, (a, b) in enumerate(inputs):
+            batch.upload(i, a, b)
+        batch.run()
+        batch.wait()
+        outs[mode] = [batch.download(i) for i in range(len(ns))]
+        batch.close()
+    for i, n in enumerate(ns):
+        assert (outs["phased"][i] == outs["phased-separate-outer"][i]).all(), n
+        if n <= 4096:
+            a, b = inputs[i]
+            assert (outs["phased"][i] == orc.rns_multiply(q, a, b, orc.primes_general(n, q))).all(), n
+    ctx.close()
+
+
+@pytest.mark.parametrize("outer", [1, 2])
+def test_plain_device_fused_outer_matches_host_path(pm, workload, outer):
+    """The plain device path (outer stages fused into its conversion and CRT)
+    against the host-buffer path of the same context (outer stages as
+    passes of their own), N = 2^8 .. 2^17."""
+    import torch
+    q = workload.modulus(4)
+    ctx = pm.Context(q, 1 << 17)
+    for n in (256, 2048, 16384, 32768, 65536, 131072):
+        if (n.bit_length() - 1) - outer > 15:
+            continue
+        say("plain fused", n, outer)
+        ctx.set_outer(n, outer)
+        a, b = workload.family("dense-random-1", n, q, 800 + n.bit_length())
+        want = ctx.multiply(a, b)
+        da = torch.from_numpy(a.view(np.int32)).cuda()
+        db = torch.from_numpy(b.view(np.int32)).cuda()
+        dc = torch.zeros_like(da)
+        ctx.multiply_device(n, da, db, dc)
+        ctx.synchronize()
+        assert (dc.cpu().numpy().view(np.uint32).reshape(n, 28) == want).all(), n
+    ctx.close()

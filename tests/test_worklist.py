@@ -9,6 +9,12 @@ CONV, OUTER_FWD, TILE, OUTER_INV, CRT = range(5)
 BLOCK = 1024
 
 
+def default_outer(logn):
+    """Default batch schedule: outer stages down to a 2^13 tile, at most two
+    (run inside the conversion and the CRT, so no outer-pass items)."""
+    return 0 if logn <= 13 else min(2, logn - 13)
+
+
 @pytest.mark.parametrize("ns", [
     [32768] * 5,
     [4096, 32768, 1024, 8192, 32, 16384, 2048, 1024, 65536, 131072, 512],
@@ -30,10 +36,8 @@ def test_worklist_coverage(pm, ns):
         if t == CONV:
             assert a0 in (0, 1) and a1 * BLOCK < n
             seen_conv.add((j, a0, a1))
-        elif t == OUTER_FWD:
-            assert ns[j] > 32768 and a0 < 2 * k
-        elif t == OUTER_INV:
-            assert ns[j] > 32768 and a0 < k
+        elif t in (OUTER_FWD, OUTER_INV):
+            raise AssertionError("default outer stages are fused: no separate outer-pass items")
         elif t == CRT:
             assert a0 * BLOCK < n
             seen_crt.add((j, a0))
@@ -43,7 +47,7 @@ def test_worklist_coverage(pm, ns):
             for u in units[a1:a1 + count]:
                 job, ps = int(u[0]), int(u[1])
                 logn = ns[job].bit_length() - 1
-                outer = max(0, logn - 15)
+                outer = default_outer(logn)
                 assert logn - outer == tlog
                 prime, s = ps >> 2, ps & 3
                 assert prime < k and s < (1 << outer)
@@ -54,9 +58,9 @@ def test_worklist_coverage(pm, ns):
         blocks = -(-n // BLOCK)
         assert all((j, p, b) in seen_conv for p in (0, 1) for b in range(blocks))
         assert all((j, b) in seen_crt for b in range(blocks))
-        outer = max(0, (n.bit_length() - 1) - 15)
+        outer = default_outer(n.bit_length() - 1)
         assert all((j, p, s) in seen_unit for p in range(k) for s in range(1 << outer))
-    assert len(seen_unit) == sum(k << max(0, (n.bit_length() - 1) - 15) for n in ns)
+    assert len(seen_unit) == sum(k << default_outer(n.bit_length() - 1) for n in ns)
 
 
 def test_worklist_rejects(pm):
