@@ -78,6 +78,7 @@ def library():
     lib.pm_inject_io_stall.argtypes = [P, ctypes.c_int32]
     lib.pm_place_near_device.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.pm_multiply_device.argtypes = [P, u32, P, P, P, P]
+    lib.pm_multiply_device_noalloc.argtypes = [P, u32, P, P, P, P]
     lib.pm_synchronize.argtypes = [P]
     lib.pm_set_outer.argtypes = [P, u32, u32]
     lib.pm_get_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
@@ -116,7 +117,7 @@ def library():
 # every extern "C" entry point declared in include/polymul.h
 EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
             "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_set_io_wait_limit",
-            "pm_inject_io_stall", "pm_place_near_device", "pm_multiply_device",
+            "pm_inject_io_stall", "pm_place_near_device", "pm_multiply_device", "pm_multiply_device_noalloc",
             "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
             "pm_set_batch_tiles_per_block", "pm_get_batch_tiles_per_block",
             "pm_stage_convert",
@@ -277,6 +278,12 @@ class Context:
     def multiply_device(self, n, a, b, c, stream=None):
         self._check("pm_multiply_device",
                     self._lib.pm_multiply_device(self._h, n, _ptr(a), _ptr(b), _ptr(c), stream))
+
+    def multiply_device_noalloc(self, n, a, b, c, stream=None):
+        """pm_multiply_device with the residues in the library's static
+        scratch: no allocation; calls overlapping on one GPU race."""
+        self._check("pm_multiply_device_noalloc",
+                    self._lib.pm_multiply_device_noalloc(self._h, n, _ptr(a), _ptr(b), _ptr(c), stream))
 
     def synchronize(self):
         self._check("pm_synchronize", self._lib.pm_synchronize(self._h))

@@ -29,6 +29,10 @@ constexpr uint32_t kCrtWords = 31;
 constexpr uint32_t kCrtLdsStride = 80;
 constexpr uint32_t kCrtWaveLds = kCrtWords * kCrtLdsStride * 6;  // bytes: u32 low + i16 high
 
+// Static residue scratch of the allocation-free plain product: ra and rb of
+// up to 2 x 64 primes x 2^15 residues (16 MiB per device).
+constexpr uint32_t kScratchWords = 2u * 64u * 32768u;
+
 struct Tw {
     uint32_t value, shoup;
 };

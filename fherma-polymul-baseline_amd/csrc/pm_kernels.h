@@ -64,6 +64,9 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
 // one launch per phase (and per tile size class) over the same work list
 hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& ranges, bool scalar_conversion,
                          bool scalar_crt, hipStream_t stream);
+// device address of the static residue scratch (kScratchWords words) on
+// `device`, which must be the current device; looked up once per device
+hipError_t scratch_address(int device, uint32_t** out);
 // raise the dynamic LDS limit of the tile kernels (call once per device)
 hipError_t configure_kernels();
 

@@ -136,6 +136,11 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
         delete ctx;
         return create_failed(PM_EDEVICE, std::string("kernel configuration: ") + hipGetErrorString(g_configure_status));
     }
+    e = pmd::scratch_address(device, &ctx->scratch);
+    if (e != hipSuccess) {
+        delete ctx;
+        return create_failed(PM_EDEVICE, std::string("static scratch address: ") + hipGetErrorString(e));
+    }
     int status = PM_OK;
     const pm::Params& s = ctx->params;
     const uint32_t k = s.k, quads = (k + 3) / 4;
@@ -393,6 +398,29 @@ int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const u
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, d_a, d_b, d_c, ctx->d_ra, ctx->d_rb};
     return pm_run_plain(ctx, job, ctx->outer[logn], stream);
+}
+
+int pm_multiply_device_noalloc(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b,
+                               uint32_t* d_c, void* hip_stream) {
+    if (!ctx || !d_a || !d_b || !d_c) return PM_EINVAL;
+    uint32_t logn = 0;
+    int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    const uint64_t words = 2ull * ctx->params.k * n;
+    if (words > pmd::kScratchWords)
+        return pm_fail(ctx, PM_EUNSUPPORTED,
+                       "2 x k x N = " + std::to_string(words) + " residues exceed the static scratch of " +
+                           std::to_string(pmd::kScratchWords) + " (16 MiB: N <= 32768 for k <= 64)");
+    hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
+    // every residue the kernels read was written by this call's conversion:
+    // nothing in the scratch needs clearing
+    const uint32_t outer = ctx->outer[logn];
+    pmd::Job job{logn, outer, d_a, d_b, d_c, ctx->scratch, ctx->scratch + size_t(ctx->params.k) * n,
+                 pmd::outer_fusable(logn, outer) ? 1u : 0u};
+    hipError_t e = pmd::launch_convert(ctx->tables, job, stream);
+    if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, outer, stream);
+    if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, stream);
+    return pm_hip(ctx, e, "launch allocation-free product");
 }
 
 int pm_multiply(pm_context* ctx, uint32_t n, const uint32_t* a, const uint32_t* b, uint32_t* c) {

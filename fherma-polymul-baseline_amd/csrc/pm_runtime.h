@@ -26,6 +26,8 @@ struct pm_context {
     // products never share the residue workspace at once
     hipEvent_t workspace_free = nullptr;
     bool workspace_used = false;
+    // this device's static residue scratch (pm_multiply_device_noalloc)
+    uint32_t* scratch = nullptr;
     // outer radix-2 stages run as separate passes, per log2 N: plain path,
     // and batches / served launches
     uint32_t outer[pm::kMaxLogN + 1] = {};

@@ -106,6 +106,17 @@ int pm_inject_io_stall(pm_context* ctx, int32_t chunk);
  * context's stream); returns once the work is enqueued. */
 int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_c,
                        void* hip_stream);
+/* Allocation-free variant of pm_multiply_device (no counterpart in the
+ * reference), for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops.  Same product and arguments; its
+ * residues live in a static 16 MiB array of the library (one per device,
+ * held until the process ends) instead of the context's workspace, so the
+ * call makes no allocation.  Limit: 2 x k x n <= 4,194,304 residues (n <=
+ * 32768 for every q; PM_EUNSUPPORTED beyond).  Every call on one GPU shares
+ * that array: calls that overlap on a device (other streams, other
+ * contexts, other threads) race and must be serialized by the caller. */
+int pm_multiply_device_noalloc(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b,
+                               uint32_t* d_c, void* hip_stream);
 /* Waits for the context's stream. */
 int pm_synchronize(pm_context* ctx);
 /* Plain-path schedule (no counterpart in the reference): the number of outer
