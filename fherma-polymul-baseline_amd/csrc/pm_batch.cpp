@@ -223,7 +223,7 @@ void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool la
     items.clear();
     units.clear();
     for (uint32_t j : order) {
-        const uint32_t blocks = ceil_div(1u << jobs[j].logn, pmd::kConvPerBlock);
+        const uint32_t blocks = ceil_div(1u << jobs[j].logn, pmd::kConvItemCoefs);
         for (uint32_t blk = 0; blk < blocks; ++blk)
             for (uint32_t poly = 0; poly < 2; ++poly) items.push_back({pmd::kItemConv, j, poly, blk});
     }

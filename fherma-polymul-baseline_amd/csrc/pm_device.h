@@ -13,6 +13,10 @@ constexpr uint32_t kTileElems = 32768;     // residues held by one NTT block (32
 constexpr uint32_t kLdsWords = kTileElems + kTileElems / 32;  // padded exchange buffer
 constexpr uint32_t kConvPerWave = 64;      // coefficients per wave in the conversion
 constexpr uint32_t kConvPerBlock = kConvPerWave * (kBlock / kWaveSize);  // 1024
+// batch conversion: an item is 2048 coefficients of one polynomial, 16 slots
+// of 128 that a wave converts as two halves of 64
+constexpr uint32_t kConvSlotCoefs = 128;
+constexpr uint32_t kConvItemCoefs = 2048;
 constexpr uint32_t kConvSmallBlock = 256;  // threads of the plain-path conversion blocks
 constexpr uint32_t kCrtPerBlock = 256;
 // MFMA CRT: groups of 64 coefficients per wave, 8 waves per block; per wave

@@ -546,6 +546,97 @@ __device__ __forceinline__ void conv_compute(const Tables& tb, const v4u (&v0)[4
     }
 }
 
+// Residue (in [0,2p)) of one column tile from its two MFMA steps: the exact
+// digit sums T_d = sum_j a_j * digit_d(256^j mod p) recombined as
+// (T0 + 256 T1) + 2^16 (T2 + 256 T3) with one Shoup step.
+__device__ __forceinline__ uint32_t conv_residue(const v4i& w0, const v4i& w1, const v4i& b0, const v4i& b1,
+                                                 const int4& corr, uint32_t p, uint32_t p2, Tw c16) {
+    v4i acc = v4i{0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w0, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w1, b1, acc, 0, 0, 0);
+    const int32_t u = (acc[0] + corr.x) + (acc[1] + corr.y) * 256;  // |u| < 2^30
+    const int32_t w = (acc[2] + corr.z) + (acc[3] + corr.w) * 256;  // |w| < 2^30
+    const uint32_t x = csub(shoup(uint32_t(w) + p2, c16, p), p);   // w * 2^16 mod p, [0,p)
+    const uint32_t v = uint32_t(u) + p2 + x;                       // < 2^32, > 0
+    return csub(csub(v, p2), p2);
+}
+
+// A slot of 128 coefficients as two halves A and B, each four column tiles of
+// 16 (operand bytes va / vb as conv_load gives them).  Contiguous layout
+// (fused = false, cstride 16): A = [cA, cA + 64), B = [cA + 64, cA + 128).
+// Fused layout (cstride = n/4): half h holds columns x + 16h .. x + 16h + 15
+// of each quarter, so every (prime, quarter) run of the slot is 32 residues;
+// after the outer stages a row swap (v_permlane16_swap: rows 1, 3 of A with
+// rows 0, 2 of B) gives each lane pair of rows one prime's 32 consecutive
+// residues, and each store writes whole 128-byte lines (16-residue runs left
+// half-written lines that the L2 wrote back as more HBM traffic).
+__device__ __forceinline__ void conv_slot_compute(const Tables& tb, const v4u (&va0)[4], const v4u (&va1)[4],
+                                                  const v4u (&vb0)[4], const v4u (&vb1)[4], uint32_t n, uint32_t cA,
+                                                  uint32_t cstride, bool fused, uint32_t outer,
+                                                  uint32_t* __restrict__ res_any, const ConvLds& L, uint32_t lane,
+                                                  uint32_t quad_begin = 0, uint32_t quad_end = 0xFFFFFFFFu) {
+    const global_store_u32 res = (global_store_u32)res_any;
+    const uint32_t col = lane & 15u, kg = lane >> 4;
+    const v4u m = v4u{0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u};
+    v4i a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        a0[ct] = __builtin_bit_cast(v4i, va0[ct] ^ m);
+        a1[ct] = __builtin_bit_cast(v4i, va1[ct] ^ m);
+        b0[ct] = __builtin_bit_cast(v4i, vb0[ct] ^ m);
+        b1[ct] = __builtin_bit_cast(v4i, vb1[ct] ^ m);
+    }
+    quad_end = quad_end < tb.quads ? quad_end : tb.quads;
+    for (uint32_t quad = quad_begin; quad < quad_end; ++quad) {
+        const v4i w0 = __builtin_bit_cast(v4i, L.w[(quad * 2 + 0) * 64 + lane]);
+        const v4i w1 = __builtin_bit_cast(v4i, L.w[(quad * 2 + 1) * 64 + lane]);
+        const uint32_t prime = quad * 4u + kg;
+        const uint32_t p = L.p[prime], p2 = 2u * p;
+        const int4 corr = L.corr[prime];
+        const Tw c16{65536u, L.sh16[prime]};
+        uint32_t ra[4], rb[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            ra[ct] = conv_residue(w0, w1, a0[ct], a1[ct], corr, p, p2, c16);
+            rb[ct] = conv_residue(w0, w1, b0[ct], b1[ct], corr, p, p2, c16);
+        }
+        if (!fused) {
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const uint32_t c = cA + 16u * ct + col;
+                if (prime < tb.k && c < n) res[size_t(prime) * n + c] = ra[ct];
+                if (prime < tb.k && c + 64u < n) res[size_t(prime) * n + c + 64u] = rb[ct];
+            }
+            continue;
+        }
+        if (outer >= 1u) {  // the outer stages, as conv_compute applies them
+            const Tw t1 = L.otw[prime][0];
+            ct_bfly(ra[0], ra[2], t1, p, p2);
+            ct_bfly(ra[1], ra[3], t1, p, p2);
+            ct_bfly(rb[0], rb[2], t1, p, p2);
+            ct_bfly(rb[1], rb[3], t1, p, p2);
+            if (outer >= 2u) {
+                const Tw t2 = L.otw[prime][1], t3 = L.otw[prime][2];
+                ct_bfly(ra[0], ra[1], t2, p, p2);
+                ct_bfly(ra[2], ra[3], t3, p, p2);
+                ct_bfly(rb[0], rb[1], t2, p, p2);
+                ct_bfly(rb[2], rb[3], t3, p, p2);
+            }
+        }
+        // rows (lane / 16) of A: primes 4q..4q+3 at columns x..x+15; of B the
+        // same primes at x+16..x+31.  After the swap A holds rows {A0, B0, A2,
+        // B2} (primes 4q, 4q, 4q+2, 4q+2) and B rows {A1, B1, A3, B3}.
+        const uint32_t pa = quad * 4u + (kg & 2u), pb = pa + 1u;
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            const auto sw = __builtin_amdgcn_permlane16_swap(ra[ct], rb[ct], false, false);
+            const uint32_t c = cA + 16u * (kg & 1u) + cstride * ct + col;
+            if (pa < tb.k) res[size_t(pa) * n + c] = sw[0];
+            if (pb < tb.k) res[size_t(pb) * n + c] = sw[1];
+        }
+    }
+}
+
 // One group, loaded then converted (no prefetch).
 __device__ __forceinline__ void convert_wave(const Tables& tb, const uint32_t* __restrict__ coef, uint32_t n,
                                              uint32_t c_end, uint32_t c0, uint32_t* __restrict__ res,
@@ -881,11 +972,27 @@ __global__ void __launch_bounds__(kBlock) k_convert(Tables tb, Job job, uint32_t
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t c0 = c_first + blockIdx.x * blockDim.x + wave * kConvPerWave;  // one coefficient per thread
     if (c0 >= c_end) return;
-    if (job.fused)  // whole range (c_first = 0, c_end = n): group c0/64 at quarter offsets, outer stages applied
-        convert_wave(tb, poly ? job.b : job.a, n, n, c0 / 4u, poly ? job.rb : job.ra, L, lane, quad_begin, quad_end,
-                     n / 4u, job.outer);
-    else
-        convert_wave(tb, poly ? job.b : job.a, n, c_end, c0, poly ? job.rb : job.ra, L, lane, quad_begin, quad_end);
+    convert_wave(tb, poly ? job.b : job.a, n, c_end, c0, poly ? job.rb : job.ra, L, lane, quad_begin, quad_end);
+}
+
+// The plain path with the outer stages fused: wave w of the grid converts
+// slot w (columns 32w .. 32w + 31 of each quarter) of a (blockIdx.y = 0)
+// or b; blockIdx.z picks a slice of the primes.
+__global__ void __launch_bounds__(kBlock) k_convert_fused(Tables tb, Job job) {
+    __shared__ ConvLds L;
+    const uint32_t per = (tb.quads + gridDim.z - 1) / gridDim.z;
+    const uint32_t quad_begin = blockIdx.z * per, quad_end = quad_begin + per;
+    conv_tables_to_lds(tb, L, quad_begin, quad_end);
+    const uint32_t n = 1u << job.logn, lane = threadIdx.x & 63u;
+    const uint32_t slot = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (slot * kConvSlotCoefs >= n) return;
+    const uint32_t* coef = blockIdx.y ? job.b : job.a;
+    const uint32_t cA = 32u * slot;
+    v4u a0[4], a1[4], b0[4], b1[4];
+    conv_load(coef, n, cA, lane, a0, a1, n / 4u);
+    conv_load(coef, n, cA + 16u, lane, b0, b1, n / 4u);
+    conv_slot_compute(tb, a0, a1, b0, b1, n, cA, n / 4u, true, job.outer, blockIdx.y ? job.rb : job.ra, L, lane,
+                      quad_begin, quad_end);
 }
 
 __global__ void __launch_bounds__(256) k_convert_scalar(Tables tb, Job job) {
@@ -1048,54 +1155,32 @@ __global__ void __launch_bounds__(kCrtPerBlock) k_crt(Tables tb, Job job) {
 // phased batch: one launch per phase over a contiguous range of the work list
 // (the launches order the phases; block b runs item first + b)
 // ---------------------------------------------------------------------------
-// Conversion items (kConvPerBlock coefficients of one polynomial each, 16
-// groups of 64) run by resident blocks: wave w of block b takes groups
-// b*W + w, + grid*W, ..., and loads the next group's coefficients while it
-// converts the current one; the weights go to LDS once per block.
+// Conversion items (kConvItemCoefs coefficients of one polynomial each, 16
+// slots of 128) run by resident blocks: wave w of block b takes slots b*W + w,
+// + grid*W, ...; the weights go to LDS once per block.  A slot's two halves
+// are loaded together (64 b128 loads per lane in flight), then converted.
 __global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, uint32_t first, uint32_t count) {
     __shared__ ConvLds L;
     conv_tables_to_lds(tb, L);
-    constexpr uint32_t per_item = kConvPerBlock / kConvPerWave;
+    constexpr uint32_t per_item = kConvItemCoefs / kConvSlotCoefs;
     const uint32_t waves = blockDim.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t total = count * per_item, stride = gridDim.x * waves;
-    uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (threadIdx.x >> 6));
-    if (g >= total) return;
-    struct Where {
-        const uint32_t* coef;
-        uint32_t* res;
-        uint32_t n, c0, cstride, outer;
-        bool valid;
-    };
-    auto where = [&](uint32_t gi) {
-        const Item it = ba.items[first + gi / per_item];
+    for (uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (threadIdx.x >> 6)); g < total;
+         g += stride) {
+        const Item it = ba.items[first + g / per_item];
         const Job job = ba.jobs[it.job];
-        const uint32_t n = 1u << job.logn, c0 = it.a1 * kConvPerBlock + (gi % per_item) * kConvPerWave;
-        // fused outer stages: group c0/64 takes 16 coefficients from each quarter
-        return job.fused ? Where{it.a0 ? job.b : job.a, it.a0 ? job.rb : job.ra, n, c0 / 4u, n / 4u, job.outer, c0 < n}
-                         : Where{it.a0 ? job.b : job.a, it.a0 ? job.rb : job.ra, n, c0, 16u, 0u, c0 < n};
-    };
-    Where cur = where(g);
-    v4u v0[4], v1[4];
-    conv_load(cur.coef, cur.n, cur.c0, lane, v0, v1, cur.cstride);
-    for (;;) {
-        const uint32_t gn = g + stride;
-        const bool more = gn < total;
-        Where nxt = cur;
-        v4u n0[4], n1[4];
-        if (more) {
-            nxt = where(gn);
-            conv_load(nxt.coef, nxt.n, nxt.c0, lane, n0, n1, nxt.cstride);
-        }
-        if (cur.valid) conv_compute(tb, v0, v1, cur.n, cur.n, cur.c0, cur.res, L, lane, 0u, 0xFFFFFFFFu, cur.cstride,
-                                    cur.outer);
-        if (!more) break;
-#pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-            v0[ct] = n0[ct];
-            v1[ct] = n1[ct];
-        }
-        g = gn;
-        cur = nxt;
+        const uint32_t n = 1u << job.logn, slot = it.a1 * per_item + g % per_item;
+        if (slot * kConvSlotCoefs >= n) continue;  // the item's tail past a small n
+        const uint32_t* coef = it.a0 ? job.b : job.a;
+        uint32_t* res = it.a0 ? job.rb : job.ra;
+        const bool fused = job.fused != 0;
+        // fused: slot s covers columns 32s .. 32s + 31 of each quarter
+        const uint32_t cA = fused ? 32u * slot : kConvSlotCoefs * slot, cstride = fused ? n / 4u : 16u;
+        const uint32_t cB = fused ? cA + 16u : cA + 64u;
+        v4u a0[4], a1[4], b0[4], b1[4];
+        conv_load(coef, n, cA, lane, a0, a1, cstride);
+        conv_load(coef, n, cB, lane, b0, b1, cstride);
+        conv_slot_compute(tb, a0, a1, b0, b1, n, cA, cstride, fused, job.outer, res, L, lane);
     }
 }
 
@@ -1103,9 +1188,9 @@ __global__ void __launch_bounds__(kBlock) k_items_conv_scalar(Tables tb, BatchAr
     const Item it = ba.items[first + blockIdx.x];
     const Job job = ba.jobs[it.job];
     const uint32_t n = 1u << job.logn;
-    const uint32_t c = it.a1 * kConvPerBlock + threadIdx.x;
-    if (c >= n) return;
-    convert_scalar_one(tb, it.a0 ? job.b : job.a, n, c, it.a0 ? job.rb : job.ra);
+    for (uint32_t c = it.a1 * kConvItemCoefs + threadIdx.x; c < n && c < (it.a1 + 1u) * kConvItemCoefs;
+         c += blockDim.x)
+        convert_scalar_one(tb, it.a0 ? job.b : job.a, n, c, it.a0 ? job.rb : job.ra);
 }
 
 __global__ void __launch_bounds__(kBlock) k_items_outer(Tables tb, BatchArgs ba, uint32_t first) {
@@ -1443,6 +1528,18 @@ hipError_t launch_convert_range(const Tables& tb, const Job& job, uint32_t c_fir
     const uint32_t n = 1u << job.logn;
     if (c_first >= c_end || c_end > n || polys < 1 || poly_first + polys > 2) return hipErrorInvalidValue;
     if (job.fused && (c_first != 0 || c_end != n || !outer_fusable(job.logn, job.outer))) return hipErrorInvalidValue;
+    if (job.fused) {
+        if (poly_first != 0 || polys != 2) return hipErrorInvalidValue;
+        // 4-wave blocks of one slot (128 coefficients) per wave; primes split
+        // over up to 4 slices while that leaves fewer than ~4 waves per CU
+        const uint32_t slots = n / kConvSlotCoefs, threads = kConvSmallBlock;
+        const uint32_t per_block = threads / kWaveSize, blocks = (slots + per_block - 1) / per_block;
+        const uint32_t waves = blocks * polys * per_block;
+        uint32_t slices = 1;
+        while (slices < 4 && slices < tb.quads && waves * slices < 4 * tb.cus) slices *= 2;
+        hipLaunchKernelGGL(k_convert_fused, dim3(blocks, 2, slices), dim3(threads), 0, stream, tb, job);
+        return hipGetLastError();
+    }
     const uint32_t count = c_end - c_first;
     // 4-wave blocks, and the primes split over up to 4 slices while that
     // leaves fewer than ~4 waves per CU: a single product (or one chunk of it)

@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 CONV, OUTER_FWD, TILE, OUTER_INV, CRT = range(5)
-BLOCK = 1024
+BLOCK = 1024       # coefficients of a CRT item
+CONV_BLOCK = 2048  # coefficients of a conversion item
 
 
 def default_outer(logn):
@@ -34,7 +35,7 @@ def test_worklist_coverage(pm, ns):
     for t, j, a0, a1 in items:
         n = ns[j] if t != TILE else None
         if t == CONV:
-            assert a0 in (0, 1) and a1 * BLOCK < n
+            assert a0 in (0, 1) and a1 * CONV_BLOCK < n
             seen_conv.add((j, a0, a1))
         elif t in (OUTER_FWD, OUTER_INV):
             raise AssertionError("default outer stages are fused: no separate outer-pass items")
@@ -55,9 +56,8 @@ def test_worklist_coverage(pm, ns):
                 assert key not in seen_unit
                 seen_unit.add(key)
     for j, n in enumerate(ns):
-        blocks = -(-n // BLOCK)
-        assert all((j, p, b) in seen_conv for p in (0, 1) for b in range(blocks))
-        assert all((j, b) in seen_crt for b in range(blocks))
+        assert all((j, p, b) in seen_conv for p in (0, 1) for b in range(-(-n // CONV_BLOCK)))
+        assert all((j, b) in seen_crt for b in range(-(-n // BLOCK)))
         outer = default_outer(n.bit_length() - 1)
         assert all((j, p, s) in seen_unit for p in range(k) for s in range(1 << outer))
     assert len(seen_unit) == sum(k << default_outer(n.bit_length() - 1) for n in ns)
