@@ -26,6 +26,10 @@ METRIC = "N=2^15 868-bit negacyclic mults/s, batched vs plain (bit-exact); p99 m
 HBM_PEAK_GBS = 8000.0
 VALU_SIMDS = 1024          # 256 CUs x 4 SIMDs
 VALU_CLOCK_HZ = 2.4e9      # peak engine clock
+# int32 VALU lane-operations per second at peak: 1024 SIMDs x 16 lanes x 2.4 GHz
+# (a wave64 instruction occupies a SIMD for 4 cycles)
+VALU_PEAK_TOPS = VALU_SIMDS * 16 * VALU_CLOCK_HZ / 1e12
+MFMA_I8_PEAK_TOPS = 5000.0  # dense int8
 
 
 def package():
@@ -193,12 +197,17 @@ def run_headline(args):
     achieved = phase_gbs[dominant]
     traffic = valu_instructions = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_batched_n{n}_b{B}.json")
+    dominant_kernel = (f"pmd::k_items_tile<{dominant.split('^')[1]}>" if dominant.startswith("tile")
+                       else {"conversion": "pmd::k_items_conv", "crt": "pmd::k_items_crt_mfma"}.get(dominant))
     if os.path.exists(tfile):
         # HBM bytes per launch of the dominant phase from the FETCH_SIZE /
-        # WRITE_SIZE profile runs, scaled by a copy kernel of known size
+        # WRITE_SIZE profile runs, scaled by a copy kernel of known size, and
+        # its VALU instruction count -- used only when the profile is of the
+        # kernel this run's schedule makes dominant
         tphase = json.load(open(tfile)).get("phases", {}).get(phase_key(dominant), {})
-        traffic = tphase.get("hbm_bytes_per_launch")
-        valu_instructions = tphase.get("valu_instructions_per_launch")
+        if tphase.get("kernel") == dominant_kernel:
+            traffic = tphase.get("hbm_bytes_per_launch")
+            valu_instructions = tphase.get("valu_instructions_per_launch")
     # The tile is bound by the vector ALUs: busy fraction = wave64 VALU
     # instructions per launch (SQ_INSTS_VALU, deterministic for this workload)
     # x 4 SIMD cycles each / (1024 SIMDs x 2.4 GHz x the phase time)
@@ -211,11 +220,13 @@ def run_headline(args):
     import oracle_lib  # the checker; timed here as the CPU port of the reference path
     orc = oracle_lib.Oracle()
     primes = orc.reference_primes_for(n, q)
-    done, exact = 0, True
+    done, exact, cpu_ms = 0, True, []
     t0 = time.perf_counter()
     while done < len(batch_c) and (time.perf_counter() - t0) < args.cpu_seconds:
         a, b = inputs[done]
+        t1 = time.perf_counter()
         c = orc.rns_multiply(q, a, b, primes)
+        cpu_ms.append((time.perf_counter() - t1) * 1e3)  # one product, as main.cpp:181-186 times fherma_run
         exact = exact and bool(np.array_equal(c, batch_c[done]))
         done += 1
     cpu_elapsed = time.perf_counter() - t0
@@ -268,23 +279,16 @@ def run_headline(args):
             "special_q": "2^868 - 11993087 (the reference's published configuration)",
             "gpu_paths_agree": gpu_consistent and harness_exact and ref_exact,
         },
-        "roofline": {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": f"{dominant} phase of the {args.mode} batch ({B} products per launch)",
-            "kernel_ms": round(phase_ms[dominant], 4),
-            "algorithmic_bytes_per_launch": per_phase[phase_key(dominant)] * B,
-            "phases_ms": {k_: round(v, 4) for k_, v in phase_ms.items()},
-            "phases_gbs": {k_: round(v, 1) for k_, v in phase_gbs.items()},
-            "whole_launch_gbs": round(per_product * B / (launch_ms * 1e-3) / 1e9, 1),
-            "valu_busy_frac": None if valu_frac is None else round(valu_frac, 4),
-            "valu_instructions_per_launch": valu_instructions,
-        },
+        "roofline": roofline_record(dominant, phase_ms, phase_gbs, per_phase, phase_key, B, args.mode, traffic,
+                                    valu_instructions, valu_frac, per_product, launch_ms),
         "cpu_baseline": {
             "value": round(par_rate, 4), "unit": "products/s", "cores": threads, "kind": "port",
             "sample": f"{sample} products of the same workload (N={n}), oracle RNS port, one product per thread on "
-                      f"{threads} threads; single thread: {done} products",
+                      f"{threads} threads; single thread: {done} products, each timed",
             "single_thread_value": round(cpu_rate, 4),
+            "single_thread_product_p50_ms": round(percentile(cpu_ms, 50), 2) if cpu_ms else None,
+            "single_thread_product_p99_ms": round(percentile(cpu_ms, 99), 2) if cpu_ms else None,
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
         },
         "bit_exact": bool(exact and gpu_consistent and done > 0),
     }
@@ -299,10 +303,97 @@ def run_headline(args):
         torch.cuda.synchronize()
         line["config"]["copy_calibration"] = {"kernel": "torch int32 neg", "bytes_read": words * 4,
                                               "bytes_written": words * 4, "launches": 3}
-    print(json.dumps(line), flush=True)
     batch.close()
     ctx.close()
+    if not args.no_extra:
+        line["config"].update(extra_workloads(args))
+        line["bit_exact"] = bool(line["bit_exact"] and all(
+            line["config"][key].get("bit_exact", True) for key in ("uneven_batch", "large_n", "serving")))
+    print(json.dumps(line), flush=True)
     return 0
+
+
+def extra_workloads(args):
+    """The batched uneven, batched large-N and serving workloads, each run
+    once at a reduced repetition count and summarised for the headline line."""
+    import copy
+    t0 = time.perf_counter()
+    a = copy.copy(args)
+    a.count, a.steps, a.warmup = 0, 5, 1
+    u = run_uneven(a)
+    print(f"[bench] uneven: {u['batched']['phased']['products_per_s']} products/s", file=sys.stderr, flush=True)
+    lg = run_large(a)
+    print(f"[bench] large: {lg['modes']['phased']['products_per_s']} products/s", file=sys.stderr, flush=True)
+    a.serve_runs = 2
+    sv = run_serve(a)
+    ph = u["batched"]["phased"]
+    m, sc = lg["modes"]["phased"], lg["modes"]["phased-scalar-conversion"]
+    out = {
+        "uneven_batch": {
+            "workload": u["workload"], "products": u["products"], "sizes": u["sizes"],
+            "products_per_s": ph["products_per_s"], "ms_per_launch": ph["ms_per_launch"],
+            "plain_one_at_a_time_products_per_s": u["plain_one_at_a_time_products_per_s"],
+            "batched_over_plain": round(ph["products_per_s"] / u["plain_one_at_a_time_products_per_s"], 3),
+            "autotuned_batch_outer": u["autotuned_batch_outer"],
+            "autotuned_tiles_per_block": u["autotuned_tiles_per_block"], "bit_exact": u["bit_exact_vs_plain"]},
+        "large_n": {
+            "workload": lg["workload"], "k_primes": lg["k_primes"],
+            "mfma_conversion": {"products_per_s": m["products_per_s"], "conversion_ms": m["phases_ms"]["conversion"],
+                                "conversion_gbs": m["conversion_gbs"], "conversion_hbm_frac": m["conversion_hbm_frac"],
+                                "conversion_int8_tops": m["conversion_int8_tops"],
+                                "conversion_int8_frac": m["conversion_mfma_frac_of_5pops"]},
+            "scalar_conversion": {"products_per_s": sc["products_per_s"],
+                                  "conversion_ms": sc["phases_ms"]["conversion"],
+                                  "conversion_gbs": sc["conversion_gbs"]},
+            "bit_exact": lg["mfma_and_scalar_bit_identical"]},
+        "serving": {"workload": sv["workload"], "io": sv["io"], "runs_per_setting": sv["runs_per_setting"],
+                    "rates": {rate: {name: {k_: v[k_] for k_ in ("products_per_s", "p50_ms", "p99_ms",
+                                                                  "same_results")}
+                                     for name, v in modes.items()}
+                              for rate, modes in sv["rates"].items()},
+                    "bit_exact": all(v["same_results"] for modes in sv["rates"].values() for v in modes.values())},
+    }
+    out["extra_workloads_s"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
+def roofline_record(dominant, phase_ms, phase_gbs, per_phase, phase_key, B, mode, traffic, valu_instructions,
+                    valu_frac, per_product, launch_ms):
+    """Roofline of the dominant phase.  The NTT tile is bound by the vector
+    ALUs, not HBM: with its VALU instruction count (SQ_INSTS_VALU, from the
+    counter profile of this kernel) the binding roof is the VALU issue rate
+    and the HBM fraction is given beside it; without it, HBM."""
+    hbm = phase_gbs[dominant]
+    rec = {}
+    if valu_frac is not None:
+        tops = valu_instructions * 64 / (phase_ms[dominant] * 1e-3) / 1e12
+        rec.update({"bound": "valu", "achieved": round(tops, 2), "peak": round(VALU_PEAK_TOPS, 2),
+                    "unit": "Tops/s (int32 VALU lane-operations)", "frac": round(valu_frac, 4)})
+    else:
+        rec.update({"bound": "hbm", "achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(hbm / HBM_PEAK_GBS, 4)})
+    rec.update({
+        "traffic": traffic,
+        "kernel": f"{dominant} phase of the {mode} batch ({B} products per launch)",
+        "kernel_ms": round(phase_ms[dominant], 4),
+        "algorithmic_bytes_per_launch": per_phase[phase_key(dominant)] * B,
+        "hbm_achieved_gbs": round(hbm, 1), "hbm_peak_gbs": HBM_PEAK_GBS, "hbm_frac": round(hbm / HBM_PEAK_GBS, 4),
+        "valu_instructions_per_launch": valu_instructions,
+        "phases_ms": {k_: round(v, 4) for k_, v in phase_ms.items()},
+        "phases_gbs": {k_: round(v, 1) for k_, v in phase_gbs.items()},
+        "whole_launch_gbs": round(per_product * B / (launch_ms * 1e-3) / 1e9, 1),
+    })
+    return rec
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def dropin_harness(pm, q, n, cases, first_c):
@@ -351,7 +442,8 @@ def run_uneven(args):
     # per-N batch schedule (outer passes, hence tile size and tiles per block)
     # from the autotuner, on small batches of each size
     tuned = pm.autotune_batch(ctx, sorted(set(ns)), count=8, reps=3, stream=stream)
-    print(f"[bench] batch schedule: {tuned}", file=sys.stderr, flush=True)
+    tuned_tiles = tuned.pop("tiles_per_block")
+    print(f"[bench] batch schedule: {tuned} tiles per block {tuned_tiles}", file=sys.stderr, flush=True)
     results = {}
     outputs = {}
     for mode in ("phased",):
@@ -403,11 +495,11 @@ def run_uneven(args):
             "autotune_ms_per_launch_of_8": {str(n): {str(u): round(t, 4) for u, t in r["ms"].items()}
                                             for n, r in tuned.items()},
             "batched": results,
+            "autotuned_tiles_per_block": {str(t): r["tiles"] for t, r in tuned_tiles.items()},
             "plain_one_at_a_time_products_per_s": round(len(ns) / (plain_total_ms * 1e-3), 1),
             "bit_exact_vs_plain": True}
-    print(json.dumps(line), flush=True)
     ctx.close()
-    return 0
+    return line
 
 
 def run_large(args):
@@ -424,7 +516,7 @@ def run_large(args):
     inputs = make_inputs(pm, q, [n] * B, args.seed)
     res = {}
     outs = {}
-    for mode in ("phased", "phased-scalar"):
+    for mode in ("phased", "phased-scalar-conversion"):
         batch = pm.Batch(ctx, [n] * B, mode=mode)
         for i, (a, b) in enumerate(inputs):
             batch.upload(i, a, b)
@@ -449,12 +541,11 @@ def run_large(args):
             res[mode]["conversion_mfma_frac_of_5pops"] = round(tops / 5000.0, 5)
         outs[mode] = [batch.download(i) for i in range(B)]
         batch.close()
-    exact = all(np.array_equal(x, y) for x, y in zip(outs["phased"], outs["phased-scalar"]))
+    exact = all(np.array_equal(x, y) for x, y in zip(outs["phased"], outs["phased-scalar-conversion"]))
     line = {"workload": f"batched large N: {B} products at N=2^17, residue conversion int8 MFMA vs scalar",
             "k_primes": k, "modes": res, "mfma_and_scalar_bit_identical": exact}
-    print(json.dumps(line), flush=True)
     ctx.close()
-    return 0
+    return line
 
 
 def run_serve(args):
@@ -546,9 +637,8 @@ def run_serve(args):
             print(f"[bench] serve rate {rate} {name}: {out['rates'][str(rate)][name]}", file=sys.stderr, flush=True)
             gc.collect()  # between settings, outside the timed loops
     gc.enable()
-    print(json.dumps(out), flush=True)
     ctx.close()
-    return 0
+    return out
 
 
 def main():
@@ -569,6 +659,8 @@ def main():
     ap.add_argument("--count", type=int, default=0, help="products / requests for the extra workloads")
     ap.add_argument("--rates", type=float, nargs="*", default=[2000.0, 8000.0])
     ap.add_argument("--serve-runs", type=int, default=3, help="serving: runs of the request stream per setting")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline only: skip the uneven, large-N and serving records")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a known-size streaming kernel (for scaling profiler byte counters)")
     args = ap.parse_args()
@@ -576,7 +668,11 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "products are independent, so more GPUs would each run their own batch", file=sys.stderr)
         return 2
-    return {"headline": run_headline, "uneven": run_uneven, "large": run_large, "serve": run_serve}[args.workload](args)
+    if args.workload == "headline":
+        return run_headline(args)
+    record = {"uneven": run_uneven, "large": run_large, "serve": run_serve}[args.workload](args)
+    print(json.dumps(record), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

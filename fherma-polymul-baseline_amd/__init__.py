@@ -326,12 +326,14 @@ class Context:
     def load_schedule(self, path):
         """Apply a schedule written by `autotune_plain` / `autotune_batch`
         (sizes this context cannot hold are skipped); returns
-        {"outer": {n: u}, "batch_outer": {n: u}} as applied."""
+        {"outer": {n: u}, "batch_outer": {n: u}, "batch_tiles_per_block":
+        {tile_n: tiles}} as applied."""
         import json
         with open(path) as f:
             doc = json.load(f)
-        applied = {"outer": {}, "batch_outer": {}}
-        for key, setter in (("outer", self.set_outer), ("batch_outer", self.set_batch_outer)):
+        applied = {"outer": {}, "batch_outer": {}, "batch_tiles_per_block": {}}
+        for key, setter in (("outer", self.set_outer), ("batch_outer", self.set_batch_outer),
+                            ("batch_tiles_per_block", self.set_batch_tiles_per_block)):
             for n_str, outer in doc.get(key, {}).items():
                 n = int(n_str)
                 if n <= self.info()["max_n"]:
@@ -545,18 +547,87 @@ def _save_schedule(path, key, table):
         json.dump(doc, f, indent=1)
 
 
+def _time_batch(ctx, ns, inputs, reps, stream):
+    """Median ms of a graph launch of a batch of these sizes (schedule as set)."""
+    import torch
+    batch = Batch(ctx, ns)
+    for i, (a, b) in enumerate(inputs):
+        batch.upload(i, a, b)
+    with torch.cuda.stream(stream):
+        for _ in range(2):
+            batch.run(stream.cuda_stream)
+        batch.wait(stream.cuda_stream)
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            batch.run(stream.cuda_stream)
+            e1.record(stream)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        batch.wait(stream.cuda_stream)
+    batch.close()
+    return float(np.median(ms))
+
+
+def tiles_per_block_choices(tile_n):
+    """Valid tiles per block of a tile size: powers of two with
+    2048 <= tiles x tile_n <= 32768 (whole waves of 32 residues per thread,
+    at most 1024 threads)."""
+    out, t = [], 1
+    while t * tile_n <= 32768:
+        if t * tile_n >= 2048:
+            out.append(t)
+        t *= 2
+    return out
+
+
 def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
-    """Per-N batch schedule: times a graph launch of `count` products of each
-    size with every valid number of outer passes (0..2, tile 2^5..2^15), keeps
-    the fastest in the context (for batches and served launches created
-    afterwards) and returns {n: {"outer": u, "ms": {u: median ms per launch}}};
-    with `path` the choice is also written under "batch_outer" of the
-    schedule file `load_schedule` reads."""
+    """Per-N batch schedule, the role of the reference's compile-time tile,
+    radix and primes-per-block knobs (quartic/host_setup.h:11-12,
+    quartic/solve.cu:325,576), chosen on the device:
+
+    1. tiles per block of each tile size 2^t the sizes can use: a graph
+       launch of `count` products of N = 2^t (no outer stages) is timed with
+       every valid tiles-per-block count and the fastest kept;
+    2. outer stages (0..2: the tile size, and radix-2 or radix-4 outer
+       stages in the conversion and the CRT) for each N, timed the same way
+       with the tiles per block of step 1.
+
+    Both are kept in the context (for batches and served launches created
+    afterwards).  Returns {n: {"outer": u, "ms": {u: median ms per launch}},
+    "tiles_per_block": {tile_n: {"tiles": t, "ms": {t: ms}}}}; with `path`
+    the choice is also written to the schedule file `load_schedule` reads
+    ("batch_outer" and "batch_tiles_per_block")."""
     import torch
     from . import workload as wl
     s = stream if stream is not None else torch.cuda.Stream()
+    sizes = sorted(set(int(x) for x in ns))
+    max_n = ctx.info()["max_n"]
+    tiles = {}
+    for n in sizes:
+        lg = n.bit_length() - 1
+        for outer in (0, 1, 2):
+            if outer <= lg and 5 <= lg - outer <= 15:
+                tiles.setdefault(1 << (lg - outer), None)
+    tile_result = {}
+    for tile_n in sorted(tiles):
+        choices = tiles_per_block_choices(tile_n)
+        if tile_n > max_n or len(choices) < 2:
+            continue
+        a, b = wl.poly(tile_n, 0, tile_n, ctx.q), wl.poly(tile_n, 1, tile_n, ctx.q)
+        saved = ctx.get_batch_outer(tile_n)
+        ctx.set_batch_outer(tile_n, 0)
+        times = {}
+        for t in choices:
+            ctx.set_batch_tiles_per_block(tile_n, t)
+            times[t] = _time_batch(ctx, [tile_n] * count, [(a, b)] * count, reps, s)
+        best = min(times, key=times.get)
+        ctx.set_batch_tiles_per_block(tile_n, best)
+        ctx.set_batch_outer(tile_n, saved)
+        tile_result[tile_n] = {"tiles": best, "ms": times}
     result = {}
-    for n in sorted(set(int(x) for x in ns)):
+    for n in sizes:
         logn = n.bit_length() - 1
         a = wl.poly(n, 0, n, ctx.q)
         b = wl.poly(n, 1, n, ctx.q)
@@ -565,29 +636,14 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
             if outer > logn or logn - outer > 15 or logn - outer < 5:
                 continue
             ctx.set_batch_outer(n, outer)
-            batch = Batch(ctx, [n] * count)
-            for i in range(count):
-                batch.upload(i, a, b)
-            with torch.cuda.stream(s):
-                for _ in range(2):
-                    batch.run(s.cuda_stream)
-                batch.wait(s.cuda_stream)
-                ms = []
-                for _ in range(reps):
-                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    e0.record(s)
-                    batch.run(s.cuda_stream)
-                    e1.record(s)
-                    e1.synchronize()
-                    ms.append(e0.elapsed_time(e1))
-                batch.wait(s.cuda_stream)
-            batch.close()
-            times[outer] = float(np.median(ms))
+            times[outer] = _time_batch(ctx, [n] * count, [(a, b)] * count, reps, s)
         best = min(times, key=times.get)
         ctx.set_batch_outer(n, best)
         result[n] = {"outer": best, "ms": times}
     if path is not None:
         _save_schedule(path, "batch_outer", {n: r["outer"] for n, r in result.items()})
+        _save_schedule(path, "batch_tiles_per_block", {t: r["tiles"] for t, r in tile_result.items()})
+    result["tiles_per_block"] = tile_result
     return result
 
 

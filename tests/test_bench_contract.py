@@ -48,3 +48,24 @@ def test_traffic_summary_matches_the_formula():
         assert phases[name]["algorithmic_bytes_per_launch"] == per[name] * 128, name
         # measured HBM bytes within 10 % of the compulsory traffic
         assert 1.0 <= phases[name]["traffic_over_algorithmic"] < 1.1, name
+
+
+def test_roofline_names_the_binding_roof():
+    """With the dominant kernel's VALU instruction count the record's bound is
+    the VALU issue rate (frac = busy fraction) with the HBM fraction beside
+    it; without it, the HBM roof."""
+    bench = load_bench()
+    per = bench.phase_bytes(1 << 15, 59)
+    ms = {"conversion": 0.6, "tile_2^13": 1.2, "crt": 0.4}
+    gbs = {k: per["tile" if k.startswith("tile") else k] * 128 / (v * 1e-3) / 1e9 for k, v in ms.items()}
+    key = lambda name: "tile" if name.startswith("tile") else name  # noqa: E731
+    instr = 600_000_000
+    frac = instr * 4 / (bench.VALU_SIMDS * bench.VALU_CLOCK_HZ * 1.2e-3)
+    rec = bench.roofline_record("tile_2^13", ms, gbs, per, key, 128, "phased", 3.0e9, instr, frac,
+                                bench.algorithmic_bytes(1 << 15, 59), 2.5)
+    assert rec["bound"] == "valu" and abs(rec["frac"] - round(frac, 4)) < 1e-9
+    assert abs(rec["achieved"] / rec["peak"] - frac) < 1e-3  # lane-ops/s over the peak issue rate
+    assert abs(rec["hbm_frac"] - gbs["tile_2^13"] / bench.HBM_PEAK_GBS) < 1e-4
+    rec = bench.roofline_record("tile_2^13", ms, gbs, per, key, 128, "phased", None, None, None,
+                                bench.algorithmic_bytes(1 << 15, 59), 2.5)
+    assert rec["bound"] == "hbm" and rec["unit"] == "GB/s" and rec["frac"] == rec["hbm_frac"]

@@ -116,26 +116,31 @@ def test_batch_rejects_small_n(pm, workload):
 
 
 def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
-    """Every batch schedule (outer passes leaving a tile of 2^5..2^15) gives
-    the golden product; the batch autotuner keeps the fastest per size and its
-    choice round-trips through the schedule file."""
-    for n, outers in ((32768, (0, 1, 2)), (65536, (1, 2))):
+    """Every batch schedule -- outer stages leaving a tile of 2^5..2^15, and
+    every tiles-per-block count of that tile size -- gives the golden
+    product; the batch autotuner keeps the fastest and its choice
+    round-trips through the schedule file."""
+    for n, outers in ((4096, (0, 1, 2)), (32768, (0, 1, 2)), (65536, (1, 2))):
         case = [c for c in GOLDEN["cases"] if c["n"] == n and c["modulus"].startswith("seed:")][0]
         q = workload.modulus(int(case["modulus"].split(":")[1]))
         ctx = pm.Context(q, 1 << 16)
         a, b = workload.family(case["family"], n, q, case["seed"])
         for outer in outers:
-            say("batch schedule", n, outer)
-            ctx.set_batch_outer(n, outer)
-            assert ctx.get_batch_outer(n) == outer
-            batch = pm.Batch(ctx, [n] * 3)
-            for i in range(3):
-                batch.upload(i, a, b)
-            batch.run()
-            batch.wait()
-            for i in range(3):
-                assert sha(batch.download(i)) == case["sha256"]["c"], (n, outer, i)
-            batch.close()
+            tile_n = n >> outer
+            for tiles in pm.tiles_per_block_choices(tile_n):
+                say("batch schedule", n, outer, tiles)
+                ctx.set_batch_outer(n, outer)
+                ctx.set_batch_tiles_per_block(tile_n, tiles)
+                assert ctx.get_batch_outer(n) == outer and ctx.get_batch_tiles_per_block(tile_n) == tiles
+                batch = pm.Batch(ctx, [n] * 3)
+                for i in range(3):
+                    batch.upload(i, a, b)
+                batch.run()
+                batch.wait()
+                for i in range(3):
+                    assert sha(batch.download(i)) == case["sha256"]["c"], (n, outer, tiles, i)
+                batch.close()
+            ctx.set_batch_tiles_per_block(tile_n, 0)
         ctx.close()
     ctx = pm.Context(workload.modulus(1), 1 << 16)
     with pytest.raises(pm.PolymulError):
@@ -145,12 +150,18 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
     path = tmp_path / "schedule.json"
     say("batch autotune")
     res = pm.autotune_batch(ctx, [4096, 32768], count=4, reps=2, path=str(path))
+    tiles = res.pop("tiles_per_block")
     assert set(res) == {4096, 32768}
     for n, r in res.items():
         assert r["outer"] == ctx.get_batch_outer(n) and r["outer"] in r["ms"]
+    assert 8192 in tiles  # 2^13: the tile of 32768 with two outer stages
+    for tile_n, r in tiles.items():
+        assert r["tiles"] == ctx.get_batch_tiles_per_block(tile_n) and r["tiles"] in r["ms"]
     ctx.set_batch_outer(32768, 2 if res[32768]["outer"] != 2 else 0)
+    ctx.set_batch_tiles_per_block(8192, 4 if tiles[8192]["tiles"] != 4 else 1)
     applied = ctx.load_schedule(str(path))
     assert applied["batch_outer"][32768] == res[32768]["outer"] == ctx.get_batch_outer(32768)
+    assert applied["batch_tiles_per_block"][8192] == tiles[8192]["tiles"] == ctx.get_batch_tiles_per_block(8192)
     ctx.close()
 
 
