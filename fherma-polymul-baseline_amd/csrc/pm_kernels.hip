@@ -98,7 +98,7 @@ __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p,
 // NPH phases of (nearly) equal width; in phase P a thread's registers span
 // that phase's bits (plus filler bits when the width is below ELOG), so every
 // butterfly of the phase is register-local.  Phases are joined by an exchange
-// through padded LDS (address x + x/32).
+// through padded LDS (lds_pad).
 // ---------------------------------------------------------------------------
 // log2 of the residues per thread of a tile of 2^tlog: 32 per thread, in
 // 1024-thread blocks of 128 VGPRs.  (64 per thread in 512-thread blocks of
@@ -109,6 +109,22 @@ __host__ __device__ constexpr uint32_t tile_block_threads(int tlog) {
     return tlog < 5 ? kBlock : (kTileElems >> tile_elog(tlog));
 }
 
+// LDS address of tile index x: x + x/32 spreads every phase's half-wave
+// accesses over the 32 banks except in the 2^13 tile, whose 13 index bits
+// split 5 + 4 + 4: there phase 1's lanes hold index bits {1,2,3,8,9} and
+// x/32 puts bits 3 and 8 on the same bank.  x + 2(x/32) + x/512 gives the
+// bits of each phase weights of distinct 2-adic valuation mod 32 (phase 0:
+// bits 0-4; phase 1: 2, 4, 8, 272, 545; phase 2: 34, 68, 136, 272, 545), so
+// no exchange of that tile conflicts (checked on the CPU over every exchange
+// and register).  Both forms are additive over disjoint bit sets, so a
+// thread's base address plus a register's constant offset still gives the
+// slot.  (For 2^11 and 2^12 no form with two such terms is conflict-free.)
+template <int TLOG>
+__host__ __device__ constexpr uint32_t lds_pad(uint32_t x) {
+    if constexpr (TLOG == 13) return x + ((x >> 5) << 1) + (x >> 9);
+    else return x + (x >> 5);
+}
+
 template <int TLOG>
 struct Geo {
     static constexpr int ELOG = tile_elog(TLOG);
@@ -117,7 +133,7 @@ struct Geo {
     static constexpr int TPT = 1 << TPTLOG;
     static constexpr int NPH = (TLOG + ELOG - 1) / ELOG;
     static constexpr int T = 1 << TLOG;
-    static constexpr int STRIDE = T + (T >> 5);
+    static constexpr int STRIDE = int(lds_pad<TLOG>(uint32_t(T - 1))) + 1;  // LDS words of one tile
     static constexpr int width(int p) { return TLOG / NPH + (p < TLOG % NPH ? 1 : 0); }
     static constexpr int lo(int p) {
         int h = TLOG;
@@ -166,7 +182,6 @@ __device__ __forceinline__ uint32_t thr_scatter(uint32_t tau) {
     return x;
 }
 
-__device__ __forceinline__ uint32_t padded(uint32_t x) { return x + (x >> 5); }
 constexpr uint32_t kLdsHalf = 16384;  // words: 64 KB, the reach of an LDS immediate offset
 
 // Butterflies between scheduling barriers, which bound the twiddles in flight
@@ -195,13 +210,13 @@ template <int TLOG, int PFROM, int PTO, bool LEAD>
 __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* lds, uint32_t tau) {
     using G = Geo<TLOG>;
     constexpr bool kWave = wave_local<TLOG>(PFROM, PTO);
-    // thread and register bits are disjoint, so padded(x|c) = padded(x)+padded(c):
+    // thread and register bits are disjoint, so lds_pad(x|c) = lds_pad(x) + lds_pad(c):
     // one base address per thread plus immediate offsets.  An LDS immediate
     // offset is below 64 KB, so slots above that use a second base 64 KB up,
     // formed in this exchange (opaque to the optimiser): left to itself the
     // compiler gives each such slot an address register of its own, shared
     // by every exchange of the same layout, and spills them across the tile.
-    const uint32_t wx = padded(thr_scatter<TLOG, PFROM>(tau)), rx = padded(thr_scatter<TLOG, PTO>(tau));
+    const uint32_t wx = lds_pad<TLOG>(thr_scatter<TLOG, PFROM>(tau)), rx = lds_pad<TLOG>(thr_scatter<TLOG, PTO>(tau));
     uint32_t wx_hi = wx + kLdsHalf, rx_hi = rx + kLdsHalf;
     if constexpr (G::STRIDE > kLdsHalf) asm volatile("" : "+v"(wx_hi), "+v"(rx_hi));
     const auto slot = [&](uint32_t x, uint32_t x_hi, uint32_t c) -> uint32_t& {
@@ -209,7 +224,7 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* 
     };
     if constexpr (LEAD && !kWave) __syncthreads();
 #pragma unroll
-    for (int r = 0; r < G::E; ++r) slot(wx, wx_hi, padded(G::reg_scatter(PFROM, r))) = v[r];
+    for (int r = 0; r < G::E; ++r) slot(wx, wx_hi, lds_pad<TLOG>(G::reg_scatter(PFROM, r))) = v[r];
     if constexpr (kWave) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -218,7 +233,7 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* 
         __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < G::E; ++r) v[r] = slot(rx, rx_hi, padded(G::reg_scatter(PTO, r)));
+    for (int r = 0; r < G::E; ++r) v[r] = slot(rx, rx_hi, lds_pad<TLOG>(G::reg_scatter(PTO, r)));
     if constexpr (kWave) {
         __builtin_amdgcn_wave_barrier();
     } else {
@@ -833,6 +848,10 @@ __device__ __forceinline__ void crt_load_group(const uint32_t* t_res_any, uint32
         }
 }
 
+// Slot (uint4) of prime i's outer-stage constants in LDS: one uint4 of padding
+// per 4 primes, so the four lane groups' primes (4 apart) read distinct banks.
+__host__ __device__ constexpr uint32_t crt_ot_slot(uint32_t i) { return 2u * i + (i >> 2); }
+
 // The outer inverse stages of the fused layout on a loaded group: for each of
 // the lane's 16 primes its four values (positions x, x + n/4, x + n/2,
 // x + 3n/4 of the tile's scaled output, in [0,p)) go through the radix-2 or
@@ -847,7 +866,7 @@ __device__ __forceinline__ void crt_outer_inverse(uint32_t (&b)[4][4][4], const 
         for (int j = 0; j < 4; ++j) {
             uint32_t i = 16u * s + 4u * g + j;
             i = i < k ? i : k - 1u;  // padding rows meet zero columns of the digit matrix
-            const uint4 c0 = ot[2u * i], c1 = ot[2u * i + 1u];
+            const uint4 c0 = ot[crt_ot_slot(i)], c1 = ot[crt_ot_slot(i) + 1u];
             const uint32_t p = c0.x, p2 = 2u * p;
             uint32_t x0 = b[s][0][j], x1 = b[s][1][j], x2 = b[s][2][j], x3 = b[s][3][j];
             if (outer >= 2u) {
@@ -941,15 +960,16 @@ __device__ __forceinline__ void crt_tables_to_lds(const Tables& tb, uint4* a_lds
         Tw w[3] = {};
         for (uint32_t j = 0; j < 3; ++j)
             if ((1u + j) < (1u << tb.max_logn)) w[j] = inv[1u + j];
-        ot_lds[2u * i] = make_uint4(tb.primes[pi], w[0].value, w[0].shoup, w[1].value);
-        ot_lds[2u * i + 1u] = make_uint4(w[1].shoup, w[2].value, w[2].shoup, 0u);
+        ot_lds[crt_ot_slot(i)] = make_uint4(tb.primes[pi], w[0].value, w[0].shoup, w[1].value);
+        ot_lds[crt_ot_slot(i) + 1u] = make_uint4(w[1].shoup, w[2].value, w[2].shoup, 0u);
     }
     __syncthreads();
 }
 
 // LDS of a CRT block with W waves: A operands, row corrections, outer-stage
 // constants, per-wave rows
-constexpr size_t kCrtSharedLds = 8u * 4u * 64u * sizeof(uint4) + 32u * sizeof(int4) + 128u * sizeof(uint4);
+constexpr size_t kCrtSharedLds =
+    8u * 4u * 64u * sizeof(uint4) + 32u * sizeof(int4) + (crt_ot_slot(63) + 2u) * sizeof(uint4);
 template <int W>
 constexpr size_t crt_block_lds() {
     return kCrtSharedLds + size_t(W) * kCrtWaveLds;
@@ -1602,8 +1622,10 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
     if (!count) return hipSuccess;
     switch (kind & 0xFFu) {
         case kPhaseConv: {
-            const uint32_t grid = count < 2 * tb.cus ? count : 2 * tb.cus;  // two resident blocks per CU
-            hipLaunchKernelGGL(k_items_conv, dim3(grid), dim3(kBlock), 0, stream, tb, ba, first, count);
+            // 4-wave blocks, up to 8 per CU, each looping over slots: measured
+            // 2 % faster for the whole batch graph than 1024-thread blocks
+            const uint32_t grid = count * 4 < 8 * tb.cus ? count * 4 : 8 * tb.cus;
+            hipLaunchKernelGGL(k_items_conv, dim3(grid), dim3(kConvSmallBlock), 0, stream, tb, ba, first, count);
             break;
         }
         case kPhaseConvScalar:
@@ -1708,17 +1730,22 @@ hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
 }  // namespace pmd
 
 namespace pmd {
+// dynamic LDS of the largest tile block: 4 tiles of 2^13 (each lds_pad<13>(8191) + 1
+// words) or one tile of 2^15
+constexpr uint32_t kTileLdsMaxWords =
+    uint32_t(4 * Geo<13>::STRIDE) > kLdsWords ? uint32_t(4 * Geo<13>::STRIDE) : kLdsWords;
+static_assert(kTileLdsMaxWords * 4 <= 160 * 1024, "tile blocks fit the LDS of a CU");
 template <int TLOG>
 static hipError_t allow_lds() {
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv_tile<TLOG>),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(kLdsWords * sizeof(uint32_t)));
+                               int(kTileLdsMaxWords * sizeof(uint32_t)));
 }
 template <int TLOG>
 static hipError_t allow_items_lds() {
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_tile<TLOG>),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(kLdsWords * sizeof(uint32_t)));
+                               int(kTileLdsMaxWords * sizeof(uint32_t)));
 }
 
 hipError_t configure_kernels() {
