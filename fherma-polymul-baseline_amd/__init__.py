@@ -68,6 +68,7 @@ def library():
     lib.pm_last_error.argtypes = [P]
     lib.pm_last_error.restype = ctypes.c_char_p
     lib.pm_plan_rns.argtypes = [u32, P, ctypes.POINTER(u32), P, P, ctypes.POINTER(u32)]
+    lib.pm_plan_quartic.argtypes = [u32, P, ctypes.POINTER(u32), P, P, P, ctypes.POINTER(u32)]
     lib.pm_get_info.argtypes = [P, ctypes.POINTER(_Info)]
     lib.pm_get_prime.argtypes = [P, u32, ctypes.POINTER(u32), ctypes.POINTER(u32)]
     lib.pm_multiply.argtypes = [P, u32, P, P, P]
@@ -80,6 +81,8 @@ def library():
     lib.pm_multiply_device.argtypes = [P, u32, P, P, P, P]
     lib.pm_multiply_device_noalloc.argtypes = [P, u32, P, P, P, P]
     lib.pm_synchronize.argtypes = [P]
+    lib.pm_set_backend.argtypes = [P, u32]
+    lib.pm_get_backend.argtypes = [P, ctypes.POINTER(u32), ctypes.POINTER(u32)]
     lib.pm_set_outer.argtypes = [P, u32, u32]
     lib.pm_get_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_set_batch_outer.argtypes = [P, u32, u32]
@@ -115,10 +118,10 @@ def library():
 
 
 # every extern "C" entry point declared in include/polymul.h
-EXPORTED = ["pm_plan_rns", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
+EXPORTED = ["pm_plan_rns", "pm_plan_quartic", "pm_create", "pm_destroy", "pm_last_error", "pm_get_info", "pm_get_prime", "pm_multiply",
             "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_set_io_wait_limit",
             "pm_inject_io_stall", "pm_place_near_device", "pm_multiply_device", "pm_multiply_device_noalloc",
-            "pm_synchronize", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
+            "pm_synchronize", "pm_set_backend", "pm_get_backend", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
             "pm_set_batch_tiles_per_block", "pm_get_batch_tiles_per_block",
             "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
@@ -159,6 +162,24 @@ def plan_rns(q, max_n):
     if status != PM_OK:
         raise PolymulError("pm_plan_rns", status, lib.pm_last_error(None).decode())
     return k.value, primes[:k.value].copy(), psi[:k.value].copy(), bits.value
+
+
+def plan_quartic(q, max_n, iotas=False):
+    """Host-only plan of the quartic backend (q = 2^868 - delta, delta < 2^28):
+    (k, primes, fourth roots of delta, product_bits), and with iotas=True also
+    the fourth roots of unity of the channels; needs no GPU."""
+    lib = library()
+    q = np.ascontiguousarray(np.asarray(q, dtype=np.uint32).reshape(LIMBS))
+    k, bits = ctypes.c_uint32(), ctypes.c_uint32()
+    primes = np.zeros(32, dtype=np.uint32)
+    roots = np.zeros(32, dtype=np.uint32)
+    io = np.zeros(32, dtype=np.uint32)
+    status = lib.pm_plan_quartic(max_n, q.ctypes.data, ctypes.byref(k), primes.ctypes.data, roots.ctypes.data,
+                                 io.ctypes.data, ctypes.byref(bits))
+    if status != PM_OK:
+        raise PolymulError("pm_plan_quartic", status, lib.pm_last_error(None).decode())
+    out = (k.value, primes[:k.value].copy(), roots[:k.value].copy(), bits.value)
+    return out + (io[:k.value].copy(),) if iotas else out
 
 
 def plan_worklist(k, ns):
@@ -284,6 +305,19 @@ class Context:
         scratch: no allocation; calls overlapping on one GPU race."""
         self._check("pm_multiply_device_noalloc",
                     self._lib.pm_multiply_device_noalloc(self._h, n, _ptr(a), _ptr(b), _ptr(c), stream))
+
+    BACKENDS = {"general": 0, "quartic": 1}
+
+    def set_backend(self, name):
+        """'general' (any odd q) or 'quartic' (q = 2^868 - delta, delta < 2^28)
+        for every later product, batch and server of this context."""
+        self._check("pm_set_backend", self._lib.pm_set_backend(self._h, self.BACKENDS[name]))
+
+    def backend(self):
+        """(name, residue rows per polynomial) of the context's backend."""
+        b, rows = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check("pm_get_backend", self._lib.pm_get_backend(self._h, ctypes.byref(b), ctypes.byref(rows)))
+        return {v: k for k, v in self.BACKENDS.items()}[b.value], rows.value
 
     def synchronize(self):
         self._check("pm_synchronize", self._lib.pm_synchronize(self._h))

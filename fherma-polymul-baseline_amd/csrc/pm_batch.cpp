@@ -61,7 +61,7 @@ void* pm_batch::alloc(size_t bytes) {
 // ranges returned.
 static int upload_worklist(pm_context* ctx, pm_batch* b, const pmd::Job* d_jobs, uint32_t first, uint32_t count,
                            pmd::BatchArgs& ba, pmd::PhaseRanges& ranges) {
-    const uint32_t k = ctx->params.k;
+    const uint32_t k = b->tables.k;  // residue rows of the batch's backend
     std::vector<pmd::Item> items;
     std::vector<pmd::Unit> units;
     pm_build_worklist(k, b->jobs.data() + first, count, true, items, units, ctx->batch_tiles);
@@ -79,7 +79,8 @@ static int upload_worklist(pm_context* ctx, pm_batch* b, const pmd::Job* d_jobs,
 
 int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint32_t* const* d_a,
                    const uint32_t* const* d_b, uint32_t* const* d_c) {
-    const uint32_t k = ctx->params.k, count = b->count;
+    b->tables = pm_tables(ctx);  // the backend at creation stays the batch's
+    const uint32_t k = b->tables.k, count = b->count;
     b->logn.resize(count);
     for (uint32_t i = 0; i < count; ++i) {
         const uint32_t n = ns[i];
@@ -94,7 +95,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
         pmd::Job& j = jobs[i];
         j.logn = b->logn[i];
         j.outer = ctx->batch_outer[b->logn[i]];
-        j.fused = !b->separate_outer && pmd::outer_fusable(j.logn, j.outer) ? 1u : 0u;
+        j.fused = !b->separate_outer && !b->tables.quartic && pmd::outer_fusable(j.logn, j.outer) ? 1u : 0u;
         const bool caller_io = d_a && d_b && d_c;
         if (caller_io) {
             j.a = d_a[i];
@@ -157,7 +158,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
             if (q > 0) le = hipStreamWaitEvent(streams[q], first_done[q - 1], 0);  // fork
             const auto phases = pmd::phase_list(part_ranges[q], b->scalar_conversion, b->scalar_crt);
             for (size_t i = 0; i < phases.size() && le == hipSuccess; ++i) {
-                le = pmd::launch_phase(ctx->tables, part_args[q], phases[i].kind, phases[i].first, phases[i].count,
+                le = pmd::launch_phase(b->tables, part_args[q], phases[i].kind, phases[i].first, phases[i].count,
                                        streams[q]);
                 if (i == 0 && le == hipSuccess) le = hipEventRecord(first_done[q], streams[q]);
             }
@@ -381,7 +382,7 @@ int pm_batch_phase_times(pm_batch* b, void* hip_stream, uint32_t max, uint32_t* 
     for (size_t i = 0; i < phases.size() && e == hipSuccess; ++i) {
         e = hipEventRecord(ev[i], stream);
         if (e == hipSuccess)
-            e = pmd::launch_phase(b->ctx->tables, b->args, phases[i].kind, phases[i].first, phases[i].count, stream);
+            e = pmd::launch_phase(b->tables, b->args, phases[i].kind, phases[i].first, phases[i].count, stream);
     }
     if (e == hipSuccess) e = hipEventRecord(ev[phases.size()], stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[phases.size()]);

@@ -24,6 +24,7 @@ struct pm_batch {
     uint32_t count = 0, n_items = 0;
     std::vector<uint32_t> logn;
     std::vector<pmd::Job> jobs;     // host copy (device pointers)
+    pmd::Tables tables{};           // the context's backend when the batch was created
     pmd::BatchArgs args{};
     pmd::PhaseRanges ranges{};
     bool scalar_conversion = false; // phased only: scalar limb reduction instead of MFMA

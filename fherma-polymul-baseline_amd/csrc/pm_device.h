@@ -60,7 +60,23 @@ struct Tables {
     const uint32_t* q;            // [28]
     uint32_t mu0, mu1;            // floor(2^908/q)
     uint32_t k, quads, max_logn, cus;  // cus: compute units of the device
+    // Quartic backend (q = 2^868 - qdelta): the rows above (primes, mont,
+    // fwd, inv, scale, k) are then the 4 qk channels, row 4i + j channel j of
+    // prime i, and these hold the per-prime constants of the conversion and
+    // the CRT; quartic = 0 and null pointers for the general RNS.
+    uint32_t quartic, qk, qdelta, pad_;
+    const uint32_t* qprimes;      // [qk]
+    const Tw* q_limb_pow;         // [qk][7] 2^(32 l) mod p
+    const Tw* q_rpow;             // [qk][4] r^j
+    const Tw* q_rinv;             // [qk][4] r^-j
+    const Tw* q_iota;             // [qk] i, i^2 = -1
+    const uint32_t* q_m30;        // [qk][18] M_i in 30-bit limbs
+    const uint32_t* q_round;      // [qk] floor(2^42 / p_i)
+    const uint32_t* q_kp30;       // [qk+1][18] K P in 30-bit limbs
+    const uint32_t* q_negc;       // [28] (-qk P (1 + B + B^2 + B^3)) mod q
 };
+constexpr uint32_t kQuarticMaxPrimes = 17;
+constexpr uint32_t kQuarticLimbs30 = 18;
 
 // One product of the batch / the plain path.
 struct Job {

@@ -976,6 +976,223 @@ constexpr size_t crt_block_lds() {
 }
 
 // ---------------------------------------------------------------------------
+// quartic backend: q = 2^868 - delta, four 217-bit components, four channels
+// A(r i^k) per prime (role of quartic/solve.cu:289-316 and 906-1040)
+// ---------------------------------------------------------------------------
+// canonical a + b, a - b for a, b in [0,p)
+__device__ __forceinline__ uint32_t add_p(uint32_t a, uint32_t b, uint32_t p) { return csub(a + b, p); }
+__device__ __forceinline__ uint32_t sub_p(uint32_t a, uint32_t b, uint32_t p) { return csub(a - b + p, p); }
+
+// Channels of coefficient c (28 words at coef + 28 c) into rows 4i + j of
+// res (row stride n): the components a_j = bits [217 j, 217 j + 217), their
+// residues from 7 Shoup products with 2^(32 l) mod p each, z_j = a_j r^j and
+// the 4-point transform over i (channel j = sum_m z_m i^(j m)), canonical.
+__device__ __forceinline__ void quartic_convert_one(const Tables& tb, const uint32_t* __restrict__ coef,
+                                                    uint32_t n, uint32_t c, uint32_t* __restrict__ res) {
+    uint32_t w[kLimbs];
+    const uint4* src = reinterpret_cast<const uint4*>(coef + size_t(c) * kLimbs);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+        const uint4 x = src[v];
+        w[4 * v] = x.x;
+        w[4 * v + 1] = x.y;
+        w[4 * v + 2] = x.z;
+        w[4 * v + 3] = x.w;
+    }
+    uint32_t part[4][7];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        constexpr uint32_t kBits = 217;
+        const uint32_t shift = kBits * uint32_t(j), base = shift / 32, sh = shift % 32;
+#pragma unroll
+        for (int l = 0; l < 7; ++l) {
+            uint32_t v = w[base + l] >> sh;
+            if (sh && base + l + 1 < kLimbs) v |= w[base + l + 1] << (32 - sh);
+            part[j][l] = l == 6 ? (v & ((1u << 25) - 1u)) : v;
+        }
+    }
+    for (uint32_t i = 0; i < tb.qk; ++i) {
+        const uint32_t p = tb.qprimes[i], p2 = 2u * p;
+        const Tw* pw = tb.q_limb_pow + size_t(i) * 7;
+        uint32_t z[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t acc = 0;
+#pragma unroll
+            for (int l = 0; l < 7; ++l) acc = csub(acc + shoup(part[j][l], pw[l], p), p2);  // [0,2p)
+            acc = csub(acc, p);
+            z[j] = j ? csub(shoup(acc, tb.q_rpow[i * 4 + j], p), p) : acc;
+        }
+        const uint32_t u = add_p(z[0], z[2], p), v = sub_p(z[0], z[2], p);
+        const uint32_t s = add_p(z[1], z[3], p), t = csub(shoup(z[1] - z[3] + p, tb.q_iota[i], p), p);
+        uint32_t* row = res + size_t(4 * i) * n + c;
+        row[0] = add_p(u, s, p);
+        row[n] = add_p(v, t, p);
+        row[2 * size_t(n)] = sub_p(u, s, p);
+        row[3 * size_t(n)] = sub_p(v, t, p);
+    }
+}
+
+// Y_j = X_j + (qk - K_j) P for component J (see quartic_crt_one), added
+// into S at bit 217 J
+template <int J>
+__device__ __forceinline__ void quartic_crt_component(const Tables& tb, const uint32_t (&t)[kQuarticMaxPrimes][4],
+                                                      uint32_t (&S)[37]) {
+    constexpr int j = J;
+    const uint32_t qk = tb.qk;
+    uint64_t acc[kQuarticLimbs30];
+#pragma unroll
+    for (int l = 0; l < int(kQuarticLimbs30); ++l) acc[l] = 0;
+    uint64_t est = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kQuarticMaxPrimes; ++i) {
+        if (i >= qk) continue;
+        const uint64_t ti = t[i][j];
+        est += ti * tb.q_round[i];
+        const uint32_t* m = tb.q_m30 + size_t(i) * kQuarticLimbs30;
+#pragma unroll
+        for (int l = 0; l < int(kQuarticLimbs30); ++l) acc[l] += ti * m[l];  // each product < 2^60
+        if (i == 7) {
+            // 17 products could pass 2^64: carry the first 8 up, leaving
+            // each limb below 2^30 + 2^34 before the rest
+#pragma unroll
+            for (int l = 0; l < int(kQuarticLimbs30) - 1; ++l) {
+                acc[l + 1] += acc[l] >> 30;
+                acc[l] &= 0x3FFFFFFFull;
+            }
+        }
+    }
+    // K_j (the fixed-point error is below 17 * 2^30 / 2^42 and R_j / P in (-1/4, 1/4))
+    uint32_t K = uint32_t((est + (uint64_t(1) << 41)) >> 42);
+    K = K <= qk ? K : qk;
+    const uint32_t* kp = tb.q_kp30 + size_t(qk - K) * kQuarticLimbs30;
+#pragma unroll
+    for (int l = 0; l < int(kQuarticLimbs30); ++l) acc[l] += kp[l];
+    uint32_t limb[kQuarticLimbs30];  // exact 30-bit limbs of Y_j (< 2^515)
+#pragma unroll
+    for (int l = 0; l < int(kQuarticLimbs30) - 1; ++l) {
+        acc[l + 1] += acc[l] >> 30;
+        limb[l] = uint32_t(acc[l]) & 0x3FFFFFFFu;
+    }
+    limb[kQuarticLimbs30 - 1] = uint32_t(acc[kQuarticLimbs30 - 1]);
+    uint32_t y[17];  // 32-bit words
+#pragma unroll
+    for (int w = 0; w < 17; ++w) {
+        const int L0 = (32 * w) / 30, s0 = (32 * w) % 30;
+        y[w] = (limb[L0] >> s0) | (L0 + 1 < int(kQuarticLimbs30) ? (limb[L0 + 1] << (30 - s0)) : 0u) |
+               (s0 > 28 && L0 + 2 < int(kQuarticLimbs30) ? (limb[L0 + 2] << (60 - s0)) : 0u);
+    }
+    // S += Y_j 2^(217 j)
+    const int off = (217 * j) / 32, sh = (217 * j) % 32;
+    uint32_t cy = 0;
+#pragma unroll
+    for (int w = 0; w < 18; ++w) {
+        const uint32_t lo = w < 17 ? y[w] : 0u, prev = w > 0 ? y[w - 1] : 0u;
+        const uint32_t part = sh ? ((lo << sh) | (prev >> (32 - sh))) : lo;
+        S[off + w] = __builtin_addc(S[off + w], part, cy, &cy);
+    }
+#pragma unroll
+    for (int w = off + 18; w < 37; ++w) S[w] = __builtin_addc(S[w], 0u, cy, &cy);
+}
+
+// The 28-word product of coefficient c from the tile's scaled channel values
+// (rows 4i + j of t_res, in [0,p), already times (4N)^-1 (M_i mod p_i)^-1):
+// per prime the inverse 4-point transform and r^-j give t_ij = R_j (M_i mod
+// p_i)^-1 mod p_i; per component the CRT sum X_j = sum_i t_ij M_i and its
+// nearest-integer quotient K_j = round(sum_i t_ij / p_i) give Y_j = X_j +
+// (qk - K_j) P = R_j + qk P > 0; S = sum_j Y_j 2^(217 j) is folded modulo
+// q = 2^868 - delta, plus the constant (-qk P (1 + B + B^2 + B^3)) mod q.
+__device__ __forceinline__ void quartic_crt_one(const Tables& tb, const uint32_t* __restrict__ t_res, uint32_t n,
+                                                uint32_t c, uint32_t* __restrict__ out) {
+    const uint32_t qk = tb.qk;
+    uint32_t t[kQuarticMaxPrimes][4];
+#pragma unroll
+    for (uint32_t i = 0; i < kQuarticMaxPrimes; ++i) {
+        t[i][0] = t[i][1] = t[i][2] = t[i][3] = 0;
+        if (i >= qk) continue;  // (qk is the same for every lane)
+        const uint32_t p = tb.qprimes[i];
+        const uint32_t* row = t_res + size_t(4 * i) * n + c;
+        const uint32_t f0 = row[0], f1 = row[n], f2 = row[2 * size_t(n)], f3 = row[3 * size_t(n)];
+        const uint32_t u = add_p(f0, f2, p), v = add_p(f1, f3, p);
+        const uint32_t d0 = sub_p(f0, f2, p), d1 = csub(shoup(f1 - f3 + p, tb.q_iota[i], p), p);
+        t[i][0] = add_p(u, v, p);
+        t[i][1] = csub(shoup(sub_p(d0, d1, p), tb.q_rinv[i * 4 + 1], p), p);
+        t[i][2] = csub(shoup(sub_p(u, v, p), tb.q_rinv[i * 4 + 2], p), p);
+        t[i][3] = csub(shoup(add_p(d0, d1, p), tb.q_rinv[i * 4 + 3], p), p);
+    }
+    uint32_t S[37];
+#pragma unroll
+    for (int w = 0; w < 37; ++w) S[w] = 0;
+    quartic_crt_component<0>(tb, t, S);
+    quartic_crt_component<1>(tb, t, S);
+    quartic_crt_component<2>(tb, t, S);
+    quartic_crt_component<3>(tb, t, S);
+    // S mod (2^868 - delta): low 868 bits + high * delta + the constant
+    uint32_t T[29];
+#pragma unroll
+    for (int w = 0; w < 27; ++w) T[w] = S[w];
+    T[27] = S[27] & 0xFu;
+    T[28] = 0;
+    uint32_t H[10];
+#pragma unroll
+    for (int w = 0; w < 10; ++w) H[w] = (S[27 + w] >> 4) | (28 + w < 37 ? (S[28 + w] << 28) : 0u);
+    uint64_t carry = 0;
+#pragma unroll
+    for (int w = 0; w < 29; ++w) {
+        const uint64_t v = uint64_t(T[w]) + (w < 10 ? uint64_t(H[w]) * tb.qdelta : 0u) +
+                           (w < int(kLimbs) ? tb.q_negc[w] : 0u) + carry;
+        T[w] = uint32_t(v);
+        carry = v >> 32;
+    }
+    // T < 2^870: fold once more (T = h 2^868 + l, h <= 3), then one
+    // conditional subtraction of q
+    const uint32_t h = (T[27] >> 4) | (T[28] << 28);
+    T[27] &= 0xFu;
+    uint32_t cin = 0;
+    T[0] = __builtin_addc(T[0], h * tb.qdelta, cin, &cin);
+#pragma unroll
+    for (int w = 1; w < 28; ++w) T[w] = __builtin_addc(T[w], 0u, cin, &cin);
+    // T >= q  <=>  T + delta >= 2^868, and then T - q = (T + delta) mod 2^868
+    uint32_t U[28];
+    cin = 0;
+    U[0] = __builtin_addc(T[0], tb.qdelta, cin, &cin);
+#pragma unroll
+    for (int w = 1; w < 28; ++w) U[w] = __builtin_addc(T[w], 0u, cin, &cin);
+    const bool ge = (U[27] >> 4) != 0;
+    U[27] &= 0xFu;
+    uint4* dst = reinterpret_cast<uint4*>(out + size_t(c) * kLimbs);
+#pragma unroll
+    for (int w = 0; w < 28; w += 4)
+        dst[w / 4] = ge ? make_uint4(U[w], U[w + 1], U[w + 2], U[w + 3]) : make_uint4(T[w], T[w + 1], T[w + 2], T[w + 3]);
+}
+
+__global__ void __launch_bounds__(256) k_convert_quartic(Tables tb, Job job) {
+    const uint32_t n = 1u << job.logn, c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) quartic_convert_one(tb, blockIdx.y ? job.b : job.a, n, c, blockIdx.y ? job.rb : job.ra);
+}
+
+__global__ void __launch_bounds__(256) k_crt_quartic(Tables tb, Job job) {
+    const uint32_t n = 1u << job.logn, c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) quartic_crt_one(tb, job.ra, n, c, job.c);
+}
+
+// batch: conversion items of kConvItemCoefs coefficients, CRT items of kBlock
+__global__ void __launch_bounds__(256) k_items_conv_quartic(Tables tb, BatchArgs ba, uint32_t first) {
+    const Item it = ba.items[first + blockIdx.x];
+    const Job job = ba.jobs[it.job];
+    const uint32_t n = 1u << job.logn, end = min(n, (it.a1 + 1u) * kConvItemCoefs);
+    for (uint32_t c = it.a1 * kConvItemCoefs + threadIdx.x; c < end; c += blockDim.x)
+        quartic_convert_one(tb, it.a0 ? job.b : job.a, n, c, it.a0 ? job.rb : job.ra);
+}
+
+__global__ void __launch_bounds__(256) k_items_crt_quartic(Tables tb, BatchArgs ba, uint32_t first) {
+    const Item it = ba.items[first + blockIdx.x];
+    const Job job = ba.jobs[it.job];
+    const uint32_t n = 1u << job.logn, end = min(n, (it.a0 + 1u) * kBlock);
+    for (uint32_t c = it.a0 * kBlock + threadIdx.x; c < end; c += blockDim.x) quartic_crt_one(tb, job.ra, n, c, job.c);
+}
+
+// ---------------------------------------------------------------------------
 // standalone kernels (plain path)
 // ---------------------------------------------------------------------------
 // Coefficients [c_first, c_end) of a (poly 0) and/or b (1): blockIdx.y is
@@ -1504,6 +1721,7 @@ __global__ void __launch_bounds__(256) k_egress(const uint32_t* __restrict__ src
 }
 
 hipError_t launch_ingest_convert(const Tables& tb, const Job& job, const IoArgs& io, hipStream_t stream) {
+    if (tb.quartic) return hipErrorInvalidValue;  // the quartic backend converts from device memory
     const uint32_t n = 1u << job.logn;
     const uint32_t groups = 2u * ((n + kConvPerWave - 1) / kConvPerWave);
     const uint32_t waves_per_block = kConvSmallBlock / kWaveSize;
@@ -1547,6 +1765,11 @@ hipError_t launch_convert_range(const Tables& tb, const Job& job, uint32_t c_fir
                                 uint32_t poly_first, uint32_t polys, hipStream_t stream) {
     const uint32_t n = 1u << job.logn;
     if (c_first >= c_end || c_end > n || polys < 1 || poly_first + polys > 2) return hipErrorInvalidValue;
+    if (tb.quartic) {  // whole range, both polynomials, outer stages as passes of their own
+        if (c_first != 0 || c_end != n || poly_first != 0 || polys != 2 || job.fused) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_convert_quartic, dim3((n + 255) / 256, 2), dim3(256), 0, stream, tb, job);
+        return hipGetLastError();
+    }
     if (job.fused && (c_first != 0 || c_end != n || !outer_fusable(job.logn, job.outer))) return hipErrorInvalidValue;
     if (job.fused) {
         if (poly_first != 0 || polys != 2) return hipErrorInvalidValue;
@@ -1620,6 +1843,20 @@ static hipError_t launch_items_tile(const Tables& tb, const BatchArgs& ba, uint3
 hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, uint32_t first, uint32_t count,
                         hipStream_t stream) {
     if (!count) return hipSuccess;
+    if (tb.quartic) {  // the quartic backend's own conversion and CRT; tiles and outer passes are shared
+        switch (kind & 0xFFu) {
+            case kPhaseConv:
+            case kPhaseConvScalar:
+                hipLaunchKernelGGL(k_items_conv_quartic, dim3(count), dim3(256), 0, stream, tb, ba, first);
+                return hipGetLastError();
+            case kPhaseCrt:
+            case kPhaseCrtScalar:
+                hipLaunchKernelGGL(k_items_crt_quartic, dim3(count), dim3(256), 0, stream, tb, ba, first);
+                return hipGetLastError();
+            default:
+                break;
+        }
+    }
     switch (kind & 0xFFu) {
         case kPhaseConv: {
             // 4-wave blocks, up to 8 per CU, each looping over slots: measured
@@ -1697,12 +1934,14 @@ hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRange
 }
 
 hipError_t launch_convert_scalar(const Tables& tb, const Job& job, hipStream_t stream) {
+    if (tb.quartic) return launch_convert(tb, job, stream);  // the quartic conversion is scalar already
     const uint32_t n = 1u << job.logn;
     hipLaunchKernelGGL(k_convert_scalar, dim3((n + 255) / 256, 2), dim3(256), 0, stream, tb, job);
     return hipGetLastError();
 }
 
 hipError_t launch_crt_scalar(const Tables& tb, const Job& job, hipStream_t stream) {
+    if (tb.quartic) return launch_crt(tb, job, stream);  // one CRT on the quartic backend
     const uint32_t n = 1u << job.logn;
     const uint32_t threads = n < kCrtPerBlock ? ((n + 63) / 64) * 64 : kCrtPerBlock;
     hipLaunchKernelGGL(k_crt, dim3((n + threads - 1) / threads), dim3(threads), 0, stream, tb, job);
@@ -1715,6 +1954,11 @@ hipError_t launch_crt_range(const Tables& tb, const Job& job, uint32_t c_first, 
     if (c_first >= c_end || c_end > n || c_first % kCrtWaveCoefs || (c_end != n && c_end % kCrtWaveCoefs))
         return hipErrorInvalidValue;
     if (job.fused && (c_first != 0 || c_end != n || !outer_fusable(job.logn, job.outer))) return hipErrorInvalidValue;
+    if (tb.quartic) {
+        if (c_first != 0 || c_end != n || job.fused) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_crt_quartic, dim3((n + 255) / 256), dim3(256), 0, stream, tb, job);
+        return hipGetLastError();
+    }
     const uint32_t groups = (c_end - c_first + kCrtWaveCoefs - 1) / kCrtWaveCoefs;
     uint32_t grid = (groups + kCrtPlainWaves - 1) / kCrtPlainWaves;
     grid = grid < tb.cus ? grid : tb.cus;

@@ -347,4 +347,171 @@ std::string build_params(uint32_t max_logn, const uint32_t* q_words, uint32_t li
     return std::string();
 }
 
+namespace {
+
+// a (any size) as n limbs of 30 bits; false when it does not fit
+bool to_limbs30(const Big& a, uint32_t n, uint32_t* out) {
+    for (uint32_t l = 0; l < n; ++l) {
+        uint32_t v = 0;
+        for (uint32_t b = 0; b < 30; ++b) {
+            const uint32_t bit = 30 * l + b;
+            if (bit / 32 < a.size() && ((a[bit / 32] >> (bit % 32)) & 1u)) v |= 1u << b;
+        }
+        out[l] = v;
+    }
+    return bits(a) <= 30 * n;
+}
+
+// square root of a quadratic residue a mod an odd prime p (Tonelli-Shanks);
+// 0 when a is not a residue
+uint32_t sqrt_mod(uint32_t a, uint32_t p) {
+    a %= p;
+    if (a == 0) return 0;
+    if (pow_mod(a, (p - 1) / 2, p) != 1) return 0;
+    uint32_t q = p - 1, s = 0;
+    while ((q & 1u) == 0) { q >>= 1; ++s; }
+    uint32_t z = 2;
+    while (pow_mod(z, (p - 1) / 2, p) != p - 1) ++z;
+    uint32_t m = s, c = pow_mod(z, q, p), t = pow_mod(a, q, p), r = pow_mod(a, (q + 1) / 2, p);
+    while (t != 1) {
+        uint32_t i = 0, t2 = t;
+        while (t2 != 1 && i < m) { t2 = mulmod(t2, t2, p); ++i; }
+        if (i == m) return 0;
+        const uint32_t b = pow_mod(c, uint64_t(1) << (m - i - 1), p);
+        m = i;
+        c = mulmod(b, b, p);
+        t = mulmod(t, c, p);
+        r = mulmod(r, b, p);
+    }
+    return r;
+}
+
+}  // namespace
+
+std::string build_quartic_params(uint32_t max_logn, const uint32_t* q_words, QuarticParams& s) {
+    if (max_logn < 2 || max_logn > kMaxLogN) return "N must be a power of two in [4, 2^17]";
+    // q = 2^868 - delta with 0 < delta < 2^28
+    Big q(q_words, q_words + kLimbs);
+    trim(q);
+    Big two868 = pow2(kWidth);
+    if (cmp(q, two868) >= 0) return "q must be below 2^868";
+    Big d = sub(two868, q);
+    if (d.empty() || bits(d) > 28 || d[0] >= kQuarticMaxDelta) return "q is not 2^868 - delta with 0 < delta < 2^28";
+    const uint32_t delta = d[0];
+    if ((delta & 1u) == 0) return "q must be odd";
+    s = QuarticParams{};
+    s.max_logn = max_logn;
+    s.delta = delta;
+    const uint32_t max_n = 1u << max_logn, root_order_log = max_logn + 1;
+    // |R_j| <= N (B-1)^2 (1 + 3 delta); P > 4 of that
+    Big part = sub(pow2(kQuarticPartBits), Big{1});
+    Big bound = mul_small(mul_small(mul(part, part), max_n), 1u + 3u * delta);
+    Big four_bound = mul_small(bound, 4u);
+    Big product{1};
+    for (uint32_t c = (1u << 12) - 1; c > 0 && s.primes.size() < kQuarticMaxPrimes; --c) {
+        const uint32_t p = (c << 18) + 1u;
+        if (!is_prime(p) || pow_mod(delta, (p - 1) / 4, p) != 1) continue;  // delta must be a fourth power
+        s.primes.push_back(p);
+        product = mul_small(product, p);
+        if (cmp(product, four_bound) > 0) break;
+    }
+    if (cmp(product, four_bound) <= 0) return "not enough quartic primes for this modulus";
+    s.k = uint32_t(s.primes.size());
+    s.product_bits = bits(product);
+    const uint32_t k = s.k;
+    std::vector<uint32_t> basis_inverse(k);
+    s.m30.assign(size_t(k) * kQuarticLimbs30, 0);
+    s.round.resize(k);
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint32_t p = s.primes[i];
+        uint32_t psi = 0;
+        for (uint32_t g = 2; g < 100000 && !psi; ++g) {
+            const uint32_t r = pow_mod(g, (p - 1) >> root_order_log, p);
+            if (pow_mod(r, 1ull << (root_order_log - 1), p) == p - 1) psi = r;
+        }
+        if (!psi) return "quartic RNS root not found";
+        s.psi_max.push_back(psi);
+        uint32_t inv = p;
+        for (int t = 0; t < 5; ++t) inv *= 2u - p * inv;
+        s.mont_neg_inv.push_back(0u - inv);
+        // fourth root of delta: a square root of a square root (both exist:
+        // delta is a fourth power and -1 = i^2 is a square)
+        const uint32_t r = sqrt_mod(sqrt_mod(delta, p), p);
+        if (r == 0 || pow_mod(r, 4, p) != delta % p) return "fourth root of delta not found";
+        s.root4.push_back(r);
+        const uint32_t iota = pow_mod(psi, uint64_t(1) << (max_logn - 1), p);  // psi has order 2^(max_logn+1)
+        if (mulmod(iota, iota, p) != p - 1) return "fourth root of unity check failed";
+        s.iota.push_back(make_twiddle(iota, p));
+        const uint32_t rinv = pow_mod(r, p - 2, p);
+        for (uint32_t j = 0; j < 4; ++j) {
+            s.rpow.push_back(make_twiddle(pow_mod(r, j, p), p));
+            s.rinv.push_back(make_twiddle(pow_mod(rinv, j, p), p));
+        }
+        const uint32_t base = uint32_t((uint64_t(1) << 32) % p);
+        uint32_t v = 1;
+        for (uint32_t l = 0; l < kQuarticPartWords; ++l) {
+            s.limb_pow.push_back(make_twiddle(v, p));
+            v = mulmod(v, base, p);
+        }
+        uint32_t rem = 0;
+        Big basis = div_small(product, p, &rem);
+        if (rem) return "quartic CRT basis division not exact";
+        if (!to_limbs30(basis, kQuarticLimbs30, &s.m30[size_t(i) * kQuarticLimbs30]))
+            return "quartic CRT basis exceeds its limbs";
+        basis_inverse[i] = pow_mod(mod_small(basis, p), p - 2, p);
+        s.round[i] = uint32_t((uint64_t(1) << kRoundShift) / p);
+    }
+    s.kp30.assign(size_t(k + 1) * kQuarticLimbs30, 0);
+    for (uint32_t K = 0; K <= k; ++K)
+        if (!to_limbs30(mul_small(product, K), kQuarticLimbs30, &s.kp30[size_t(K) * kQuarticLimbs30]))
+            return "quartic K*P exceeds its limbs";
+    // (-k P (1 + B + B^2 + B^3)) mod q
+    Big w;
+    for (uint32_t j = 0; j < 4; ++j) {
+        Big t = pow2(kQuarticPartBits * j);
+        Big sum(std::max(w.size(), t.size()) + 1, 0);
+        uint64_t carry = 0;
+        for (size_t i = 0; i < sum.size(); ++i) {
+            const uint64_t x = (i < w.size() ? w[i] : 0u) + uint64_t(i < t.size() ? t[i] : 0u) + carry;
+            sum[i] = uint32_t(x);
+            carry = x >> 32;
+        }
+        trim(sum);
+        w = sum;
+    }
+    Big kpw;
+    divmod(mul(mul_small(product, k), w), q, nullptr, &kpw);
+    if (!kpw.empty()) kpw = sub(q, kpw);
+    s.negc.assign(kLimbs, 0);
+    for (size_t i = 0; i < kpw.size() && i < kLimbs; ++i) s.negc[i] = kpw[i];
+    // twiddles (shared by a prime's four channels) and the post-inverse scale
+    s.forward.resize(size_t(k) * max_n);
+    s.inverse.resize(size_t(k) * max_n);
+    std::vector<uint32_t> pw(max_n), ipw(max_n);
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint32_t p = s.primes[i], psi = s.psi_max[i], ipsi = pow_mod(psi, p - 2, p);
+        uint32_t f = 1, b = 1;
+        for (uint32_t e = 0; e < max_n; ++e) {
+            pw[e] = f;
+            ipw[e] = b;
+            f = mulmod(f, psi, p);
+            b = mulmod(b, ipsi, p);
+        }
+        for (uint32_t j = 0; j < max_n; ++j) {
+            const uint32_t r = bit_reverse(j, max_logn);
+            s.forward[size_t(i) * max_n + j] = make_twiddle(pw[r], p);
+            s.inverse[size_t(i) * max_n + j] = make_twiddle(ipw[r], p);
+        }
+    }
+    s.scale.resize(size_t(max_logn + 1) * k);
+    for (uint32_t logn = 0; logn <= max_logn; ++logn)
+        for (uint32_t i = 0; i < k; ++i) {
+            const uint32_t p = s.primes[i];
+            uint32_t v = mulmod(pow_mod(4u << logn, p - 2, p), basis_inverse[i], p);  // (4N)^-1 (M_i)^-1
+            v = mulmod(v, uint32_t((uint64_t(1) << 32) % p), p);
+            s.scale[size_t(logn) * k + i] = make_twiddle(v, p);
+        }
+    return std::string();
+}
+
 }  // namespace pm

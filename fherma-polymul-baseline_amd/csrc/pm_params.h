@@ -68,6 +68,47 @@ struct Params {
 // unsupported (mirrors the coverage checks of rns/solve.cu:581-590).
 std::string build_params(uint32_t max_logn, const uint32_t* q, uint32_t limbs, Params& out);
 
+// ---- special modulus q = 2^868 - delta, 0 < delta < 2^28: the quartic RNS
+// (role of quartic/host_setup.h:41-110).  With B = 2^217, q = B^4 - delta, a
+// coefficient is a0 + a1 B + a2 B^2 + a3 B^3 (a_j < 2^217) and the product is
+// taken in Z[X, Y]/(X^N + 1, Y^4 - delta): four signed integer components R_j,
+// |R_j| <= N (B-1)^2 (1 + 3 delta).  Modulo each prime p (delta a fourth
+// power mod p, fourth root r, i = psi^(N/2) with i^2 = -1), Y^4 - delta splits
+// into Y - r i^k, k = 0..3: four independent negacyclic channels per prime.
+// Same prime family as the general RNS (c 2^18 + 1 < 2^30); as many primes
+// as make P > 4 max|R_j| for N = max_n.
+constexpr uint32_t kQuarticPartBits = 217;
+constexpr uint32_t kQuarticPartWords = 7;   // 32-bit limbs of a component
+constexpr uint32_t kQuarticLimbs30 = 18;    // 30-bit limbs of M_i, K*P and the CRT sums (540 bits)
+constexpr uint32_t kQuarticMaxDelta = 1u << 28;
+constexpr uint32_t kQuarticMaxPrimes = 17;  // 68 rows: enough for every delta < 2^28 at N = 2^17
+
+struct QuarticParams {
+    uint32_t max_logn = 0;
+    uint32_t k = 0;                    // primes; 4k residue rows (row 4i + j: channel j of prime i)
+    uint32_t delta = 0;
+    uint32_t product_bits = 0;
+    std::vector<uint32_t> primes;      // k
+    std::vector<uint32_t> psi_max;     // k, primitive 2^(max_logn+1)-th roots
+    std::vector<uint32_t> mont_neg_inv;
+    std::vector<uint32_t> root4;       // k: r, r^4 = delta mod p
+    std::vector<Twiddle> limb_pow;     // k * 7: 2^(32 l) mod p
+    std::vector<Twiddle> rpow;         // k * 4: r^j (forward mixing)
+    std::vector<Twiddle> rinv;         // k * 4: r^-j (inverse mixing)
+    std::vector<Twiddle> iota;         // k: i, i^2 = -1
+    std::vector<Twiddle> forward;      // k * 2^max_logn, psi^brv(i) (shared by the 4 channels)
+    std::vector<Twiddle> inverse;
+    // post-inverse scale per (logn, prime): N^-1 (M_i mod p_i)^-1 4^-1 2^32 mod p
+    std::vector<Twiddle> scale;        // (max_logn+1) * k
+    std::vector<uint32_t> m30;         // k * 18: M_i = P / p_i in 30-bit limbs
+    std::vector<uint32_t> round;       // k: floor(2^42 / p_i)
+    std::vector<uint32_t> kp30;        // (k+1) * 18: K * P in 30-bit limbs
+    std::vector<uint32_t> negc;        // 28 words: (-k P (1 + B + B^2 + B^3)) mod q
+};
+
+// Empty on success, else why q (28 words) is not a supported special modulus.
+std::string build_quartic_params(uint32_t max_logn, const uint32_t* q, QuarticParams& out);
+
 // Shared helpers (also used by tests through the C-ABI).
 uint32_t pow_mod(uint32_t a, uint64_t e, uint32_t p);
 bool is_prime(uint32_t p);

@@ -209,6 +209,7 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     ctx->d_ra = static_cast<uint32_t*>(ctx->device_alloc(res_bytes));
     ctx->d_rb = static_cast<uint32_t*>(ctx->device_alloc(res_bytes));
     ctx->workspace_bytes = 3 * coef_bytes + 2 * res_bytes;
+    ctx->workspace_rows = k;
     if (!ctx->d_a || !ctx->d_b || !ctx->d_c || !ctx->d_ra || !ctx->d_rb) {
         pm_destroy(ctx);
         return create_failed(PM_ENOMEM, "device workspace allocation failed");
@@ -309,6 +310,25 @@ int pm_plan_rns(uint32_t max_n, const uint32_t* q, uint32_t* k, uint32_t* primes
     return PM_OK;
 }
 
+int pm_plan_quartic(uint32_t max_n, const uint32_t* q, uint32_t* k, uint32_t* primes, uint32_t* roots,
+                    uint32_t* iotas, uint32_t* product_bits) {
+    g_create_error.clear();
+    if (!q || !k) return create_failed(PM_EINVAL, "null argument");
+    if (max_n < 4 || (max_n & (max_n - 1)) || max_n > (1u << pm::kMaxLogN))
+        return create_failed(PM_EUNSUPPORTED, "coverage: power-of-two 4 <= N <= 131072");
+    pm::QuarticParams s;
+    const std::string why = pm::build_quartic_params(uint32_t(__builtin_ctz(max_n)), q, s);
+    if (!why.empty()) return create_failed(PM_EUNSUPPORTED, "coverage: " + why);
+    *k = s.k;
+    for (uint32_t i = 0; i < s.k; ++i) {
+        if (primes) primes[i] = s.primes[i];
+        if (roots) roots[i] = s.root4[i];
+        if (iotas) iotas[i] = s.iota[i].value;
+    }
+    if (product_bits) *product_bits = s.product_bits;
+    return PM_OK;
+}
+
 void pm_destroy(pm_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
@@ -374,15 +394,21 @@ hipError_t pm_workspace_release(pm_context* ctx, hipStream_t stream) {
     return e;
 }
 
+const pmd::Tables& pm_tables(const pm_context* ctx) { return ctx->backend ? ctx->qtables : ctx->tables; }
+
+uint32_t pm_rows(const pm_context* ctx) { return pm_tables(ctx).k; }
+
 int pm_run_plain(pm_context* ctx, const pmd::Job& job_in, uint32_t outer, hipStream_t stream) {
     // the outer stages run inside the conversion and the CRT where they can
+    // (the MFMA kernels of the general backend)
+    const pmd::Tables& tb = pm_tables(ctx);
     pmd::Job job = job_in;
     job.outer = outer;
-    job.fused = pmd::outer_fusable(job.logn, outer) ? 1u : 0u;
+    job.fused = !tb.quartic && pmd::outer_fusable(job.logn, outer) ? 1u : 0u;
     hipError_t e = pm_workspace_acquire(ctx, stream);
-    if (e == hipSuccess) e = pmd::launch_convert(ctx->tables, job, stream);
-    if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, outer, stream);
-    if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, stream);
+    if (e == hipSuccess) e = pmd::launch_convert(tb, job, stream);
+    if (e == hipSuccess) e = pmd::launch_ntt(tb, job, outer, stream);
+    if (e == hipSuccess) e = pmd::launch_crt(tb, job, stream);
     if (e == hipSuccess) e = pm_workspace_release(ctx, stream);
     return pm_hip(ctx, e, "launch plain product");
 }
@@ -406,7 +432,8 @@ int pm_multiply_device_noalloc(pm_context* ctx, uint32_t n, const uint32_t* d_a,
     uint32_t logn = 0;
     int st = pm_check_n(ctx, n, &logn);
     if (st != PM_OK) return st;
-    const uint64_t words = 2ull * ctx->params.k * n;
+    const pmd::Tables& tb = pm_tables(ctx);
+    const uint64_t words = 2ull * tb.k * n;
     if (words > pmd::kScratchWords)
         return pm_fail(ctx, PM_EUNSUPPORTED,
                        "2 x k x N = " + std::to_string(words) + " residues exceed the static scratch of " +
@@ -415,11 +442,11 @@ int pm_multiply_device_noalloc(pm_context* ctx, uint32_t n, const uint32_t* d_a,
     // every residue the kernels read was written by this call's conversion:
     // nothing in the scratch needs clearing
     const uint32_t outer = ctx->outer[logn];
-    pmd::Job job{logn, outer, d_a, d_b, d_c, ctx->scratch, ctx->scratch + size_t(ctx->params.k) * n,
-                 pmd::outer_fusable(logn, outer) ? 1u : 0u};
-    hipError_t e = pmd::launch_convert(ctx->tables, job, stream);
-    if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, outer, stream);
-    if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, stream);
+    pmd::Job job{logn, outer, d_a, d_b, d_c, ctx->scratch, ctx->scratch + size_t(tb.k) * n,
+                 !tb.quartic && pmd::outer_fusable(logn, outer) ? 1u : 0u};
+    hipError_t e = pmd::launch_convert(tb, job, stream);
+    if (e == hipSuccess) e = pmd::launch_ntt(tb, job, outer, stream);
+    if (e == hipSuccess) e = pmd::launch_crt(tb, job, stream);
     return pm_hip(ctx, e, "launch allocation-free product");
 }
 
@@ -488,9 +515,13 @@ int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uin
     pmd::IoArgs io{ctx->hd_flags, ctx->hd_flags + pmd::kMaxIoChunks, ctx->hd_flags + 2 * pmd::kMaxIoChunks,
                    ctx->d_io_counters, ctx->d_io_counters + pmd::kMaxIoChunks, ctx->io_seq, in_chunks,
                    groups_per_chunk, 0, ctx->wait_ticks};
+    const pmd::Tables& tb = pm_tables(ctx);
     pmd::Job job{logn, 0, ctx->hd_in, ctx->hd_in + words, ctx->d_c, ctx->d_ra, ctx->d_rb};
     e = pm_workspace_acquire(ctx, ctx->stream);
-    if (e == hipSuccess) e = pmd::launch_ingest_convert(ctx->tables, job, io, ctx->stream);
+    // the general backend converts straight from page-locked staging as chunks
+    // land; the quartic one converts from device memory after one transfer
+    // of the whole staging (enqueued behind the host copies below)
+    if (e == hipSuccess && !tb.quartic) e = pmd::launch_ingest_convert(tb, job, io, ctx->stream);
     // input chunk j: coefficients [j, j+1) * span of the concatenated a|b.  The
     // other launches come after the copies: a launch costs several us of host
     // time, and the copies are what the device is waiting for
@@ -506,8 +537,17 @@ int pm_multiply_submit(pm_context* ctx, uint32_t n, const uint32_t* a, const uin
         ctx->stall_chunk = -1;
         ctx->copy_pool->run(chunks_of_input, in_chunks);  // returns once a and b are no longer needed
     }
-    if (e == hipSuccess) e = pmd::launch_ntt(ctx->tables, job, ctx->outer[logn], ctx->stream);
-    if (e == hipSuccess) e = pmd::launch_crt(ctx->tables, job, ctx->stream);
+    if (e == hipSuccess && tb.quartic) {
+        // the egress chunk counters, which the general path's ingest kernel clears
+        e = hipMemsetAsync(ctx->d_io_counters, 0, pmd::kMaxIoChunks * sizeof(uint32_t), ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ctx->d_a, ctx->h_in, 2 * words * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream);
+        job.a = ctx->d_a;
+        job.b = ctx->d_a + words;  // the staging's layout: b right behind this product's a
+        if (e == hipSuccess) e = pmd::launch_convert(tb, job, ctx->stream);
+    }
+    if (e == hipSuccess) e = pmd::launch_ntt(tb, job, ctx->outer[logn], ctx->stream);
+    if (e == hipSuccess) e = pmd::launch_crt(tb, job, ctx->stream);
     if (e == hipSuccess) e = pmd::launch_egress(ctx->d_c, ctx->hd_out, words / out_chunks, out_chunks, io, ctx->stream);
     if (e == hipSuccess) e = pm_workspace_release(ctx, ctx->stream);
     if (e != hipSuccess) {
@@ -603,7 +643,7 @@ int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uin
     if (st != PM_OK) return st;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, d_a, d_b, nullptr, d_ra, d_rb};
-    return pm_hip(ctx, pmd::launch_convert(ctx->tables, job, stream), "launch conversion");
+    return pm_hip(ctx, pmd::launch_convert(pm_tables(ctx), job, stream), "launch conversion");
 }
 
 int pm_stage_convert_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
@@ -616,7 +656,7 @@ int pm_stage_convert_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_a, co
     if (st != PM_OK) return st;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, d_a, d_b, nullptr, d_ra, d_rb};
-    return pm_hip(ctx, pmd::launch_convert_scalar(ctx->tables, job, stream), "launch scalar conversion");
+    return pm_hip(ctx, pmd::launch_convert_scalar(pm_tables(ctx), job, stream), "launch scalar conversion");
 }
 
 int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, uint32_t* d_rb, void* hip_stream) {
@@ -628,7 +668,108 @@ int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, ui
         return pm_fail(ctx, PM_EUNSUPPORTED, "outer stages must leave a tile of at most 2^15 (outer <= 2)");
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, nullptr, nullptr, nullptr, d_ra, d_rb};
-    return pm_hip(ctx, pmd::launch_ntt(ctx->tables, job, outer, stream), "launch NTT convolution");
+    return pm_hip(ctx, pmd::launch_ntt(pm_tables(ctx), job, outer, stream), "launch NTT convolution");
+}
+
+// The quartic backend's device tables: per residue row (channel j of prime i
+// at row 4i + j) the prime, Montgomery constant, twiddles (the prime's,
+// repeated for its four channels so the tile kernels index them by row) and
+// post-inverse scale; per prime the conversion and CRT constants.
+static int build_quartic(pm_context* ctx) {
+    const std::string why = pm::build_quartic_params(ctx->max_logn, ctx->params.q.data(), ctx->qparams);
+    if (!why.empty()) return pm_fail(ctx, PM_EUNSUPPORTED, "quartic backend: " + why);
+    const pm::QuarticParams& s = ctx->qparams;
+    const uint32_t k = s.k, rows = 4 * k, max_n = 1u << ctx->max_logn;
+    std::vector<uint32_t> rp(rows), rm(rows);
+    std::vector<pmd::Tw> fwd(size_t(rows) * max_n), inv(size_t(rows) * max_n), scale(size_t(ctx->max_logn + 1) * rows);
+    for (uint32_t row = 0; row < rows; ++row) {
+        const uint32_t i = row / 4;
+        rp[row] = s.primes[i];
+        rm[row] = s.mont_neg_inv[i];
+        for (uint32_t x = 0; x < max_n; ++x) {
+            const pm::Twiddle f = s.forward[size_t(i) * max_n + x], g = s.inverse[size_t(i) * max_n + x];
+            fwd[size_t(row) * max_n + x] = {0u - f.value, f.shoup};  // the butterfly's negated forward twiddle
+            inv[size_t(row) * max_n + x] = {g.value, g.shoup};
+        }
+        for (uint32_t logn = 0; logn <= ctx->max_logn; ++logn) {
+            const pm::Twiddle sc = s.scale[size_t(logn) * k + i];
+            scale[size_t(logn) * rows + row] = {sc.value, sc.shoup};
+        }
+    }
+    auto tw = [](const std::vector<pm::Twiddle>& v) {
+        std::vector<pmd::Tw> out(v.size());
+        for (size_t i = 0; i < v.size(); ++i) out[i] = {v[i].value, v[i].shoup};
+        return out;
+    };
+    int status = PM_OK;
+    pmd::Tables tb = ctx->tables;  // device facts (CUs, max_logn) as the general tables have them
+    tb.primes = upload(ctx, rp, &status);
+    tb.mont = upload(ctx, rm, &status);
+    tb.fwd = upload(ctx, fwd, &status);
+    tb.inv = upload(ctx, inv, &status);
+    tb.scale = upload(ctx, scale, &status);
+    tb.k = rows;
+    tb.quads = 0;
+    tb.conv_w = nullptr;
+    tb.conv_corr = nullptr;
+    tb.conv_sh16 = nullptr;
+    tb.limb_pow = nullptr;
+    tb.crt_m30 = nullptr;
+    tb.crt_round = nullptr;
+    tb.crt_negkp = nullptr;
+    tb.crt_a = nullptr;
+    tb.crt_corr = nullptr;
+    tb.quartic = 1;
+    tb.qk = k;
+    tb.qdelta = s.delta;
+    tb.qprimes = upload(ctx, s.primes, &status);
+    tb.q_limb_pow = upload(ctx, tw(s.limb_pow), &status);
+    tb.q_rpow = upload(ctx, tw(s.rpow), &status);
+    tb.q_rinv = upload(ctx, tw(s.rinv), &status);
+    tb.q_iota = upload(ctx, tw(s.iota), &status);
+    tb.q_m30 = upload(ctx, s.m30, &status);
+    tb.q_round = upload(ctx, s.round, &status);
+    tb.q_kp30 = upload(ctx, s.kp30, &status);
+    tb.q_negc = upload(ctx, s.negc, &status);
+    if (status != PM_OK) return status;
+    ctx->qtables = tb;
+    ctx->quartic_built = true;
+    return PM_OK;
+}
+
+int pm_set_backend(pm_context* ctx, uint32_t backend) {
+    if (!ctx || backend > PM_BACKEND_QUARTIC) return PM_EINVAL;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return pm_hip(ctx, e, "set device");
+    if (backend == PM_BACKEND_QUARTIC && !ctx->quartic_built) {
+        const int st = build_quartic(ctx);
+        if (st != PM_OK) return st;
+    }
+    // products already enqueued keep the residue workspace until they finish
+    if (ctx->workspace_used) e = hipEventSynchronize(ctx->workspace_free);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return pm_hip(ctx, e, "wait for the plain path");
+    const uint32_t rows = backend == PM_BACKEND_QUARTIC ? ctx->qtables.k : ctx->tables.k;
+    if (rows > ctx->workspace_rows) {
+        // a larger plain-path residue workspace (the old one is released with the context)
+        const size_t res_bytes = size_t(rows) << ctx->max_logn << 2;
+        auto* ra = static_cast<uint32_t*>(ctx->device_alloc(res_bytes));
+        auto* rb = static_cast<uint32_t*>(ctx->device_alloc(res_bytes));
+        if (!ra || !rb) return pm_fail(ctx, PM_ENOMEM, "device workspace allocation failed");
+        ctx->d_ra = ra;
+        ctx->d_rb = rb;
+        ctx->workspace_rows = rows;
+        ctx->workspace_bytes += 2 * res_bytes;
+    }
+    ctx->backend = backend;
+    return PM_OK;
+}
+
+int pm_get_backend(const pm_context* ctx, uint32_t* backend, uint32_t* rows) {
+    if (!ctx) return PM_EINVAL;
+    if (backend) *backend = ctx->backend;
+    if (rows) *rows = pm_rows(ctx);
+    return PM_OK;
 }
 
 int pm_set_outer(pm_context* ctx, uint32_t n, uint32_t outer) {
@@ -697,7 +838,7 @@ int pm_stage_crt_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32
     if (st != PM_OK) return st;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, nullptr, nullptr, d_c, const_cast<uint32_t*>(d_t), nullptr};
-    return pm_hip(ctx, pmd::launch_crt_scalar(ctx->tables, job, stream), "launch scalar CRT");
+    return pm_hip(ctx, pmd::launch_crt_scalar(pm_tables(ctx), job, stream), "launch scalar CRT");
 }
 
 int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream) {
@@ -707,7 +848,7 @@ int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c
     if (st != PM_OK) return st;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->stream;
     pmd::Job job{logn, 0, nullptr, nullptr, d_c, const_cast<uint32_t*>(d_t), nullptr};
-    return pm_hip(ctx, pmd::launch_crt(ctx->tables, job, stream), "launch CRT");
+    return pm_hip(ctx, pmd::launch_crt(pm_tables(ctx), job, stream), "launch CRT");
 }
 
 }  // extern "C"

@@ -28,6 +28,14 @@ struct pm_context {
     bool workspace_used = false;
     // this device's static residue scratch (pm_multiply_device_noalloc)
     uint32_t* scratch = nullptr;
+    uint32_t workspace_rows = 0;    // residue rows of the plain-path workspace
+    // quartic backend (q = 2^868 - delta): parameters and device tables, built
+    // by the first pm_set_backend(PM_BACKEND_QUARTIC); backend selects which
+    // tables every path of the context uses
+    pm::QuarticParams qparams;
+    pmd::Tables qtables{};
+    bool quartic_built = false;
+    uint32_t backend = 0;
     // outer radix-2 stages run as separate passes, per log2 N: plain path,
     // and batches / served launches
     uint32_t outer[pm::kMaxLogN + 1] = {};
@@ -65,6 +73,9 @@ int pm_fail(pm_context* ctx, int status, const std::string& why);
 int pm_hip(pm_context* ctx, hipError_t e, const char* what);
 int pm_check_n(pm_context* ctx, uint32_t n, uint32_t* logn);
 int pm_run_plain(pm_context* ctx, const pmd::Job& job, uint32_t outer, hipStream_t stream);
+// the tables of the context's backend, and their residue rows per polynomial
+const pmd::Tables& pm_tables(const pm_context* ctx);
+uint32_t pm_rows(const pm_context* ctx);
 // order a plain-path product on `stream` after the previous one (whose
 // residues share the workspace), and mark the workspace in use until the
 // work enqueued on `stream` so far has run

@@ -128,6 +128,7 @@ struct Launch {
 
 struct pm_server {
     pm_context* ctx = nullptr;
+    pmd::Tables tables{};  // the context's backend when the server was created
     uint32_t max_batch = 0, max_inflight = 0;
     size_t max_items = 0, max_units = 0;
     uint64_t budget = 0;
@@ -149,7 +150,7 @@ struct pm_server {
 
     size_t request_bytes(uint32_t logn) const {
         const size_t n = size_t(1) << logn;
-        return align_up(3 * n * kLimbs * 4) + align_up(2 * size_t(ctx->params.k) * n * 4);
+        return align_up(3 * n * kLimbs * 4) + align_up(2 * size_t(tables.k) * n * 4);
     }
     void run();
     bool start_launch(Launch& L, std::vector<Request*>& reqs, std::string& why);
@@ -157,7 +158,7 @@ struct pm_server {
 };
 
 bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string& why) {
-    const uint32_t k = ctx->params.k;
+    const uint32_t k = tables.k;
     std::vector<pmd::Item> items;
     std::vector<pmd::Unit> units;
     const uint32_t count = uint32_t(reqs.size());
@@ -168,7 +169,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
         pmd::Job& job = L.h_jobs[j];
         job.logn = r->logn;
         job.outer = ctx->batch_outer[r->logn];
-        job.fused = pmd::outer_fusable(job.logn, job.outer) ? 1u : 0u;
+        job.fused = !tables.quartic && pmd::outer_fusable(job.logn, job.outer) ? 1u : 0u;
         job.a = reinterpret_cast<const uint32_t*>(base);
         job.b = job.a + n * kLimbs;
         job.c = reinterpret_cast<uint32_t*>(base) + 2 * n * kLimbs;
@@ -221,7 +222,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
     ba.items = L.d_items;
     ba.units = L.d_units;
     ba.n_items = uint32_t(items.size());
-    if (e == hipSuccess) e = pmd::launch_phased(ctx->tables, ba, ranges, false, false, L.stream);
+    if (e == hipSuccess) e = pmd::launch_phased(tables, ba, ranges, false, false, L.stream);
     // results into page-locked caller buffers go back behind the kernels on
     // the same stream; other buffers are copied on completion
     uint32_t n_out = 0;
@@ -350,6 +351,7 @@ int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_ba
     pm_server* s = new (std::nothrow) pm_server();
     if (!s) return pm_fail(ctx, PM_ENOMEM, "host allocation failed");
     s->ctx = ctx;
+    s->tables = pm_tables(ctx);
     s->budget = hbm_budget_bytes;
     s->max_batch = max_batch;
     s->max_inflight = max_inflight;
@@ -363,7 +365,7 @@ int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_ba
         return pm_hip(ctx, e, "allocate serving arena");
     }
     s->arena.reset(hbm_budget_bytes);
-    const uint32_t k = ctx->params.k, n = 1u << ctx->max_logn;
+    const uint32_t k = s->tables.k, n = 1u << ctx->max_logn;
     const size_t per_req_items = 3 * ((n + pmd::kBlock - 1) / pmd::kBlock) + 3 * k + size_t(k) * 4 + 1;
     s->max_items = per_req_items * max_batch + 64;
     s->max_units = size_t(k) * 4 * max_batch + 1;

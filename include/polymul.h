@@ -48,6 +48,15 @@ typedef struct {
 int pm_plan_rns(uint32_t max_n, const uint32_t* q, uint32_t* k, uint32_t* primes, uint32_t* psi_max,
                 uint32_t* product_bits);
 
+/* Host-only plan of the quartic backend for q = 2^868 - delta, 0 < delta <
+ * 2^28 (role of quartic/host_setup.h:41-110; no device needed): the primes
+ * (c 2^18 + 1 < 2^30 with delta a fourth power mod p, as many as make P >
+ * 4 max_n (2^217-1)^2 (1 + 3 delta)), each prime's fourth root r of delta,
+ * the fourth root of unity i (i^2 = -1) its channels use, and P's bit
+ * length.  primes/roots/iotas may be NULL or hold 32 entries.
+ * PM_EUNSUPPORTED when q is not such a modulus. */
+int pm_plan_quartic(uint32_t max_n, const uint32_t* q, uint32_t* k, uint32_t* primes, uint32_t* roots,
+                    uint32_t* iotas, uint32_t* product_bits);
 /* Replaces the input-independent part of fherma_init (rns/solve.cu:581-718):
  * coverage checks, RNS parameters, resident device tables, workspace for N up
  * to max_n.  w_bits must be 868 and limbs 28 (fherma.h:28-31). */
@@ -117,6 +126,19 @@ int pm_multiply_device(pm_context* ctx, uint32_t n, const uint32_t* d_a, const u
  * contexts, other threads) race and must be serialized by the caller. */
 int pm_multiply_device_noalloc(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b,
                                uint32_t* d_c, void* hip_stream);
+/* Backend of the context (no counterpart as an API in the reference, which
+ * picks its backend at build time, CMakeLists.txt:29-38): PM_BACKEND_GENERAL,
+ * the RNS of any odd q (the default), or PM_BACKEND_QUARTIC for q = 2^868 -
+ * delta, 0 < delta < 2^28 -- four 217-bit components and four channels per
+ * prime (quartic/host_setup.h:41-110, quartic/solve.cu:289-316,906-1040),
+ * whose tables are built by its first selection (PM_EUNSUPPORTED for any
+ * other q).  Every later product of the context, and batches and servers
+ * created afterwards, use it; both give the same words.  get: the backend and
+ * its residue rows per polynomial (k primes, or 4 per quartic prime). */
+#define PM_BACKEND_GENERAL 0u
+#define PM_BACKEND_QUARTIC 1u
+int pm_set_backend(pm_context* ctx, uint32_t backend);
+int pm_get_backend(const pm_context* ctx, uint32_t* backend, uint32_t* rows);
 /* Waits for the context's stream. */
 int pm_synchronize(pm_context* ctx);
 /* Plain-path schedule (no counterpart in the reference): the number of outer

@@ -567,3 +567,266 @@ int orc_rns_multiply(uint32_t n, const uint32_t q[L], uint32_t k, const uint32_t
     free(s); free(ra); free(rb); free(tmp); free(fwd); free(inv); free(scale);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* the special-modulus quartic product (quartic/host_setup.h:41-110,         */
+/* quartic/solve.cu:289-316 and 906-1040, research/QUARTIC_RNS_RU.md:3-47)   */
+/* ------------------------------------------------------------------------ */
+enum { QW = 20 }; /* words of the quartic CRT integers (P < 2^527) */
+
+int orc_quartic_delta(const uint32_t q[L], uint32_t* delta) {
+    /* q = 2^868 - delta, 0 < delta < 2^28: words 1..26 all ones, word 27 = 0xF */
+    for (int i = 1; i < L - 1; ++i)
+        if (q[i] != 0xFFFFFFFFu) return -1;
+    if (q[L - 1] != 0xFu || q[0] == 0) return -1;
+    const uint32_t d = 0u - q[0];
+    if (d == 0 || d >= (1u << 28)) return -1;
+    *delta = d;
+    return 0;
+}
+
+/* bound = n (2^217 - 1)^2 (1 + 3 delta) (quartic/host_setup.h:48-51), as words */
+static void quartic_bound(uint32_t n, uint32_t delta, uint32_t* bound /* QW */) {
+    uint32_t part[7], sq[QW];
+    memset(sq, 0, sizeof(sq));
+    for (int i = 0; i < 7; ++i) part[i] = 0xFFFFFFFFu;
+    part[6] = (1u << 25) - 1;
+    for (int i = 0; i < 7; ++i) mul_small_add(sq + i, QW - i, part, 7, part[i]);
+    memset(bound, 0, QW * sizeof(uint32_t));
+    uint32_t t[QW];
+    memset(t, 0, sizeof(t));
+    mul_small_add(t, QW, sq, QW, n);
+    mul_small_add(bound, QW, t, QW, 1u + 3u * delta);
+}
+
+/* the reference's quartic prime rule (quartic/host_setup.h:52-66): 16 primes
+ * 2^bits - 65535 - 65536 j with delta a fourth power mod p, first with bits =
+ * 30, then 31, until P > 4 bound */
+uint32_t orc_quartic_primes_reference(uint32_t n, uint32_t delta, uint32_t* primes, uint32_t* product_bits,
+                                      uint32_t* bound_bits) {
+    uint32_t bound[QW], four[QW];
+    quartic_bound(n, delta, bound);
+    memset(four, 0, sizeof(four));
+    mul_small_add(four, QW, bound, QW, 4);
+    if (bound_bits) *bound_bits = bits_of(bound, QW);
+    for (uint32_t bits = 30; bits <= 31; ++bits) {
+        uint32_t k = 0, product[QW];
+        memset(product, 0, sizeof(product));
+        product[0] = 1;
+        for (uint32_t cand = (1u << bits) - 65535u; k < 16 && cand > (1u << (bits - 1)); cand -= 65536u) {
+            if (!is_prime32(cand) || orc_pow_mod(delta, (cand - 1) / 4, cand) != 1) continue;
+            primes[k++] = cand;
+            uint32_t t[QW];
+            memset(t, 0, sizeof(t));
+            mul_small_add(t, QW, product, QW, cand);
+            memcpy(product, t, sizeof(t));
+        }
+        if (k == 16 && cmp_words(product, four, QW) > 0) {
+            if (product_bits) *product_bits = bits_of(product, QW);
+            return 16;
+        }
+    }
+    return 0;
+}
+
+/* square root of a quadratic residue mod p (Tonelli-Shanks, as
+ * quartic/host_setup.h:20-38 computes it); 0 if none */
+static uint32_t sqrt_mod_p(uint32_t a, uint32_t p) {
+    a %= p;
+    if (!a || orc_pow_mod(a, (p - 1) / 2, p) != 1) return 0;
+    uint32_t odd = p - 1, order = 0;
+    while (!(odd & 1)) { odd >>= 1; ++order; }
+    uint32_t z = 2;
+    while (orc_pow_mod(z, (p - 1) / 2, p) != p - 1) ++z;
+    uint32_t c = orc_pow_mod(z, odd, p), x = orc_pow_mod(a, (odd + 1) / 2, p), t = orc_pow_mod(a, odd, p);
+    while (t != 1) {
+        uint32_t i = 0, v = t;
+        while (v != 1 && i < order) { v = mulmod(v, v, p); ++i; }
+        if (i >= order) return 0;
+        const uint32_t b = orc_pow_mod(c, 1ull << (order - i - 1), p);
+        x = mulmod(x, b, p);
+        c = mulmod(b, b, p);
+        t = mulmod(t, c, p);
+        order = i;
+    }
+    return x;
+}
+
+/* negacyclic product mod p of xa and xb (length n) into xa, unnormalised by
+ * N (the twisted cyclic transform of orc_rns_multiply) */
+static void negacyclic_mod_p(uint32_t n, uint32_t logn, uint32_t p, uint32_t psi, uint32_t* xa, uint32_t* xb,
+                             uint32_t* tmp, tw_t* fwd, tw_t* inv) {
+    const uint32_t ipsi = orc_pow_mod(psi, p - 2, p);
+    const uint64_t recip = UINT64_MAX / p;
+    uint32_t f = 1, bk = 1;
+    for (uint32_t e = 0; e < n; ++e) {
+        fwd[e] = twiddle(f, p);
+        inv[e] = twiddle(bk, p);
+        f = mulmod(f, psi, p);
+        bk = mulmod(bk, ipsi, p);
+    }
+    for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = shoup(xa[j], fwd[j], p) % p;
+    memcpy(xa, tmp, sizeof(uint32_t) * n);
+    for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = shoup(xb[j], fwd[j], p) % p;
+    memcpy(xb, tmp, sizeof(uint32_t) * n);
+    cyclic_dit(xa, n, p, fwd);
+    cyclic_dit(xb, n, p, fwd);
+    for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = barrett(xa[j], xb[j], p, recip);
+    cyclic_dit(tmp, n, p, inv);
+    for (uint32_t j = 0; j < n; ++j) xa[j] = mulmod(tmp[j], inv[j].value, p); /* untwist by psi^-j */
+}
+
+/* bits [217 j, 217 j + 217) of a 28-word coefficient as 7 words
+ * (quartic/solve.cu:293-302) */
+static void quartic_component(const uint32_t* w, uint32_t j, uint32_t part[7]) {
+    const uint32_t shift = 217u * j, base = shift / 32, s = shift % 32;
+    for (uint32_t l = 0; l < 7; ++l) {
+        uint32_t v = base + l < L ? w[base + l] >> s : 0;
+        if (s && base + l + 1 < L) v |= w[base + l + 1] << (32 - s);
+        part[l] = v;
+    }
+    part[6] &= (1u << 25) - 1;
+}
+
+int orc_quartic_multiply(uint32_t n, const uint32_t q[L], uint32_t k, const uint32_t* primes, const uint32_t* a,
+                         const uint32_t* b, uint32_t* c) {
+    uint32_t logn = 0, delta = 0;
+    while ((1u << logn) < n) ++logn;
+    if (n < 4 || (1u << logn) != n || k == 0 || k > 32 || orc_quartic_delta(q, &delta)) return -1;
+    /* P > 4 bound */
+    uint32_t bound[QW], four[QW], P[QW];
+    quartic_bound(n, delta, bound);
+    memset(four, 0, sizeof(four));
+    mul_small_add(four, QW, bound, QW, 4);
+    memset(P, 0, sizeof(P));
+    P[0] = 1;
+    for (uint32_t i = 0; i < k; ++i) {
+        uint32_t t[QW];
+        memset(t, 0, sizeof(t));
+        mul_small_add(t, QW, P, QW, primes[i]);
+        memcpy(P, t, sizeof(t));
+    }
+    if (cmp_words(P, four, QW) <= 0) return -2;
+    const size_t rows = (size_t)4 * k;
+    uint32_t* ra = (uint32_t*)malloc(sizeof(uint32_t) * n * rows);
+    uint32_t* rb = (uint32_t*)malloc(sizeof(uint32_t) * n * rows);
+    uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    tw_t* fwd = (tw_t*)malloc(sizeof(tw_t) * n);
+    tw_t* inv = (tw_t*)malloc(sizeof(tw_t) * n);
+    uint32_t* comp = (uint32_t*)malloc(sizeof(uint32_t) * n * rows); /* R_j mod p_i, [i][j][n] */
+    if (!ra || !rb || !tmp || !fwd || !inv || !comp) {
+        free(ra); free(rb); free(tmp); free(fwd); free(inv); free(comp);
+        return -3;
+    }
+    for (uint32_t i = 0; i < k; ++i) {
+        const uint32_t p = primes[i];
+        const uint32_t psi = orc_find_psi(p, n);
+        const uint32_t r = sqrt_mod_p(sqrt_mod_p(delta, p), p);
+        const uint32_t io = orc_pow_mod(psi, n / 2, p); /* i^2 = -1 */
+        if (!psi || !r || orc_pow_mod(r, 4, p) != delta % p || mulmod(io, io, p) != p - 1) {
+            free(ra); free(rb); free(tmp); free(fwd); free(inv); free(comp);
+            return -4;
+        }
+        /* channel k of prime i: A(r i^k) = sum_j a_j r^j i^(jk) (quartic/solve.cu:303-316) */
+        for (int poly = 0; poly < 2; ++poly) {
+            const uint32_t* x = poly ? b : a;
+            uint32_t* out = poly ? rb : ra;
+            for (uint32_t m = 0; m < n; ++m) {
+                uint32_t z[4];
+                for (uint32_t j = 0; j < 4; ++j) {
+                    uint32_t part[7];
+                    quartic_component(x + (size_t)m * L, j, part);
+                    uint32_t acc = 0, pw = 1, base = (uint32_t)((1ull << 32) % p);
+                    for (uint32_t l = 0; l < 7; ++l) {
+                        acc = addmod(acc, mulmod(part[l] % p, pw, p), p);
+                        pw = mulmod(pw, base, p);
+                    }
+                    z[j] = mulmod(acc, orc_pow_mod(r, j, p), p);
+                }
+                const uint32_t u = addmod(z[0], z[2], p), v = submod(z[0], z[2], p);
+                const uint32_t s = addmod(z[1], z[3], p), t = mulmod(submod(z[1], z[3], p), io, p);
+                out[(4 * i + 0) * (size_t)n + m] = addmod(u, s, p);
+                out[(4 * i + 1) * (size_t)n + m] = addmod(v, t, p);
+                out[(4 * i + 2) * (size_t)n + m] = submod(u, s, p);
+                out[(4 * i + 3) * (size_t)n + m] = submod(v, t, p);
+            }
+        }
+        for (uint32_t ch = 0; ch < 4; ++ch)
+            negacyclic_mod_p(n, logn, p, psi, ra + (4 * i + ch) * (size_t)n, rb + (4 * i + ch) * (size_t)n, tmp, fwd,
+                             inv);
+        /* inverse mixing and (4 N)^-1 r^-j (quartic/solve.cu:920-935) */
+        const uint32_t inv4n = orc_pow_mod((uint32_t)(((uint64_t)4 * n) % p), p - 2, p);
+        const uint32_t rinv = orc_pow_mod(r, p - 2, p);
+        for (uint32_t m = 0; m < n; ++m) {
+            const uint32_t f0 = ra[(4 * i + 0) * (size_t)n + m], f1 = ra[(4 * i + 1) * (size_t)n + m];
+            const uint32_t f2 = ra[(4 * i + 2) * (size_t)n + m], f3 = ra[(4 * i + 3) * (size_t)n + m];
+            const uint32_t u = addmod(f0, f2, p), v = addmod(f1, f3, p);
+            const uint32_t d0 = submod(f0, f2, p), d1 = mulmod(submod(f1, f3, p), io, p);
+            const uint32_t mixed[4] = {addmod(u, v, p), submod(d0, d1, p), submod(u, v, p), addmod(d0, d1, p)};
+            for (uint32_t j = 0; j < 4; ++j)
+                comp[(4 * i + j) * (size_t)n + m] = mulmod(mulmod(mixed[j], inv4n, p), orc_pow_mod(rinv, j, p), p);
+        }
+    }
+    /* signed CRT per component, then sum_j R_j 2^(217 j) mod q
+     * (quartic/solve.cu:936-1040: magnitude, shift, fold, sign, add) */
+    uint32_t M[32][QW], Minv[32], half[QW];
+    for (uint32_t i = 0; i < k; ++i) {
+        uint64_t rem = 0;
+        for (int w = QW - 1; w >= 0; --w) {
+            const uint64_t v = (rem << 32) | P[w];
+            M[i][w] = (uint32_t)(v / primes[i]);
+            rem = v % primes[i];
+        }
+        uint64_t mm = 0;
+        for (int w = QW - 1; w >= 0; --w) mm = ((mm << 32) | M[i][w]) % primes[i];
+        Minv[i] = orc_pow_mod((uint32_t)mm, primes[i] - 2, primes[i]);
+    }
+    for (int w = 0; w < QW; ++w) half[w] = (P[w] >> 1) | (w + 1 < QW ? P[w + 1] << 31 : 0);
+    for (uint32_t m = 0; m < n; ++m) {
+        uint32_t total[L];
+        memset(total, 0, sizeof(total));
+        for (uint32_t j = 0; j < 4; ++j) {
+            uint32_t x[QW + 1];
+            memset(x, 0, sizeof(x));
+            for (uint32_t i = 0; i < k; ++i) {
+                const uint32_t t = mulmod(comp[(4 * i + j) * (size_t)n + m], Minv[i], primes[i]);
+                mul_small_add(x, QW + 1, M[i], QW, t);
+            }
+            /* x mod P (x < k P), then the signed value: negative above P/2 */
+            while (x[QW] || cmp_words(x, P, QW) >= 0) {
+                uint64_t borrow = 0;
+                for (int w = 0; w <= QW; ++w) {
+                    const uint64_t d = (uint64_t)x[w] - (w < QW ? P[w] : 0) - borrow;
+                    x[w] = (uint32_t)d;
+                    borrow = (d >> 63) & 1;
+                }
+            }
+            int negative = cmp_words(x, half, QW) > 0;
+            if (negative) { /* |R| = P - x */
+                uint64_t borrow = 0;
+                for (int w = 0; w < QW; ++w) {
+                    const uint64_t d = (uint64_t)P[w] - x[w] - borrow;
+                    x[w] = (uint32_t)d;
+                    borrow = (d >> 63) & 1;
+                }
+            }
+            /* |R_j| 2^(217 j) mod q */
+            uint32_t shifted[QW + 24], red[L];
+            memset(shifted, 0, sizeof(shifted));
+            const uint32_t sh = 217u * j;
+            for (uint32_t bit = 0; bit < 32u * QW; ++bit)
+                if (bit_at(x, bit)) shifted[(bit + sh) / 32] |= 1u << ((bit + sh) % 32);
+            mod_q(shifted, QW + 24, q, red);
+            uint32_t zero[L];
+            memset(zero, 0, sizeof(zero));
+            if (negative) sub_mod_q(zero, red, q, red);
+            /* total = (total + red) mod q */
+            uint32_t nred[L];
+            sub_mod_q(zero, red, q, nred);
+            sub_mod_q(total, nred, q, total);
+        }
+        memcpy(c + (size_t)m * L, total, sizeof(total));
+    }
+    free(ra); free(rb); free(tmp); free(fwd); free(inv); free(comp);
+    return 0;
+}

@@ -97,6 +97,22 @@ int orc_rns_multiply(uint32_t n, const uint32_t q[ORC_LIMBS], uint32_t k,
                      const uint32_t* primes, const uint32_t* a,
                      const uint32_t* b, uint32_t* c);
 
+/* special modulus q = 2^868 - delta, 0 < delta < 2^28: the quartic RNS
+ * product (quartic/host_setup.h:41-110, quartic/solve.cu:289-316,906-1040).
+ * orc_quartic_delta: 0 and delta when q has that form.
+ * orc_quartic_primes_reference: the reference's prime rule (16 primes
+ *   2^bits - 65535 - 65536 j, bits 30 then 31, delta a fourth power, P > 4
+ *   bound); returns 16 or 0; product and bound bit lengths.
+ * orc_quartic_multiply: c = a*b mod (X^N+1, q) through four 217-bit
+ *   components, four channels A(r i^k) per prime, negacyclic products,
+ *   inverse mixing, signed CRT per component and sum_j R_j 2^(217 j) mod q;
+ *   any primes with delta a fourth power, 2N | p - 1 and P > 4 bound. */
+int orc_quartic_delta(const uint32_t q[ORC_LIMBS], uint32_t* delta);
+uint32_t orc_quartic_primes_reference(uint32_t n, uint32_t delta, uint32_t* primes, uint32_t* product_bits,
+                                      uint32_t* bound_bits);
+int orc_quartic_multiply(uint32_t n, const uint32_t q[ORC_LIMBS], uint32_t k, const uint32_t* primes,
+                         const uint32_t* a, const uint32_t* b, uint32_t* c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,10 @@ class Oracle:
         lib = ctypes.CDLL(path)
         u32, u64 = ctypes.c_uint32, ctypes.c_uint64
         lib.orc_gen_modulus.argtypes = [u64, U32]
+        lib.orc_quartic_delta.argtypes = [U32, ctypes.POINTER(u32)]
+        lib.orc_quartic_primes_reference.argtypes = [u32, u32, U32, ctypes.POINTER(u32), ctypes.POINTER(u32)]
+        lib.orc_quartic_primes_reference.restype = u32
+        lib.orc_quartic_multiply.argtypes = [u32, U32, u32, U32, U32, U32, U32]
         lib.orc_special_modulus.argtypes = [u32, U32]
         lib.orc_gen_poly.argtypes = [u64, u32, u32, U32, U32]
         lib.orc_schoolbook.argtypes = [u32, U32, U32, U32, U32]
@@ -109,6 +113,25 @@ class Oracle:
         pmq = np.zeros(LIMBS, np.uint32)
         assert self.lib.orc_crt_constants(self._u32(q), k, self._u32(primes), bases, inv, pmq) == 0
         return bases.reshape(k, LIMBS), inv, pmq
+
+    def quartic_delta(self, q):
+        d = ctypes.c_uint32()
+        return d.value if self.lib.orc_quartic_delta(self._u32(q), ctypes.byref(d)) == 0 else None
+
+    def quartic_primes_reference(self, n, delta):
+        """(primes, product bits, bound bits) by the reference's quartic rule."""
+        out = np.zeros(32, np.uint32)
+        pb, bb = ctypes.c_uint32(), ctypes.c_uint32()
+        k = self.lib.orc_quartic_primes_reference(n, delta, out, ctypes.byref(pb), ctypes.byref(bb))
+        return out[:k].copy(), pb.value, bb.value
+
+    def quartic_multiply(self, q, a, b, primes):
+        n = a.shape[0]
+        c = np.zeros(n * LIMBS, np.uint32)
+        rc = self.lib.orc_quartic_multiply(n, self._u32(q), len(primes), self._u32(primes),
+                                           self._u32(a).ravel(), self._u32(b).ravel(), c)
+        assert rc == 0, rc
+        return c.reshape(n, LIMBS)
 
     def rns_multiply(self, q, a, b, primes):
         n = a.shape[0]

@@ -194,3 +194,24 @@ def test_planted_mistake_is_caught(orc, workload):
         wrong.append(s % Q)
     assert wrong != [v % Q for v in y]
     assert [to_int(x) for x in orc.schoolbook(q, a, b)] == [v % Q for v in y]
+
+
+def test_residues_against_python_modulo(orc, workload):
+    """orc_residues (the restatement of rns/solve.cu:138-157: even/odd Shoup
+    sums of the 28 limbs against 2^(32j) mod p) against Python's exact
+    int % p, on values at every limb boundary: 2^(32j) - 1, 2^(32j),
+    2^(32j) + 1, all-ones limbs, q - 1, and random coefficients."""
+    import random
+    q = workload.modulus(5)
+    qi = workload.to_int(q)
+    vals = [0, 1, qi - 1, (1 << 867) + 12345]
+    for j in range(1, 28):
+        vals += [(1 << (32 * j)) - 1, 1 << (32 * j), (1 << (32 * j)) + 1]
+    vals.append(sum(0xFFFFFFFF << (32 * j) for j in range(27)))  # 27 all-ones limbs
+    rng = random.Random(7)
+    vals += [rng.randrange(qi) for _ in range(64)]
+    x = np.stack([workload.from_int(v) for v in vals])
+    for primes in (orc.primes_reference(), orc.primes_general(1 << 17, q)):
+        got = orc.residues(x, primes)
+        want = np.array([[v % int(p) for v in vals] for p in primes], dtype=np.uint64)
+        assert (got.astype(np.uint64) == want).all()
