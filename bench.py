@@ -171,6 +171,9 @@ def run_headline(args):
     ref_ctx.close()
     ref_ms, ref_exact = dropin_harness(pm, q_ref, n, ref_inputs * (args.harness_cases // len(ref_inputs) or 1),
                                        ref_first)
+    # the same through the quartic backend (PM_BACKEND=quartic in fherma_init)
+    ref_q_ms, ref_q_exact = dropin_harness(pm, q_ref, n, ref_inputs * (args.harness_cases // len(ref_inputs) or 1),
+                                           ref_first, backend="quartic")
     print(f"[bench] plain: {plain_rate:.0f} products/s, p50 {percentile(plain_ms, 50):.3f} ms, "
           f"host drop-in p50 {percentile(host_ms, 50):.3f} ms, fherma_run p50 {percentile(harness_ms, 50):.3f} ms",
           file=sys.stderr, flush=True)
@@ -277,7 +280,9 @@ def run_headline(args):
             "dropin_fherma_run_special_q_p50_ms": round(percentile(ref_ms, 50), 4),
             "dropin_fherma_run_special_q_p99_ms": round(percentile(ref_ms, 99), 4),
             "special_q": "2^868 - 11993087 (the reference's published configuration)",
-            "gpu_paths_agree": gpu_consistent and harness_exact and ref_exact,
+            "dropin_fherma_run_special_q_quartic_p50_ms": round(percentile(ref_q_ms, 50), 4),
+            "dropin_fherma_run_special_q_quartic_p99_ms": round(percentile(ref_q_ms, 99), 4),
+            "gpu_paths_agree": gpu_consistent and harness_exact and ref_exact and ref_q_exact,
         },
         "roofline": roofline_record(dominant, phase_ms, phase_gbs, per_phase, phase_key, B, args.mode, traffic,
                                     valu_instructions, valu_frac, per_product, launch_ms),
@@ -305,12 +310,52 @@ def run_headline(args):
                                               "bytes_written": words * 4, "launches": 3}
     batch.close()
     ctx.close()
+    line["config"]["special_q_backends"] = special_q_backends(pm, n, B, q_ref, sptr, stream)
+    line["bit_exact"] = bool(line["bit_exact"] and line["config"]["special_q_backends"]["bit_exact"])
     if not args.no_extra:
         line["config"].update(extra_workloads(args))
         line["bit_exact"] = bool(line["bit_exact"] and all(
             line["config"][key].get("bit_exact", True) for key in ("uneven_batch", "large_n", "serving")))
     print(json.dumps(line), flush=True)
     return 0
+
+
+def special_q_backends(pm, n, B, q_ref, sptr, stream):
+    """At the reference's q = 2^868 - 11993087: the batched workload (B
+    products of N = n, one graph launch per step) with the general and the
+    quartic backend, their outputs compared, and the backend autotuner's
+    choice."""
+    import torch
+    wl = pm.workload
+    ctx = pm.Context(q_ref, n)
+    inputs = [(wl.poly(7000 + i, 0, n, q_ref), wl.poly(7000 + i, 1, n, q_ref)) for i in range(B)]
+    rec, outs = {}, {}
+    for backend in ("general", "quartic"):
+        ctx.set_backend(backend)
+        batch = pm.Batch(ctx, [n] * B)
+        for i, (a, b) in enumerate(inputs):
+            batch.upload(i, a, b)
+        for _ in range(3):
+            batch.run(sptr)
+        batch.wait(sptr)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            batch.run(sptr)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 10
+        rec[backend] = {"products_per_s": round(B / dt, 1), "ms_per_launch": round(dt * 1e3, 4),
+                        "rows_per_polynomial": ctx.backend()[1],
+                        "phases_ms": {k_: round(v, 4) for k_, v in batch.phase_times(sptr)}}
+        outs[backend] = [batch.download(i) for i in (0, B // 2, B - 1)]
+        batch.close()
+    tuned = pm.autotune_backend(ctx, n, count=16, reps=3, stream=stream)
+    ctx.close()
+    print(f"[bench] special q: general {rec['general']['products_per_s']}/s, quartic "
+          f"{rec['quartic']['products_per_s']}/s, autotuner keeps {tuned['backend']}", file=sys.stderr, flush=True)
+    return {"modulus": "2^868 - 11993087", "workload": f"batched, {B} products of N={n} per graph launch",
+            "general": rec["general"], "quartic": rec["quartic"], "autotuned_backend": tuned["backend"],
+            "bit_exact": all(np.array_equal(x, y) for x, y in zip(outs["general"], outs["quartic"]))}
 
 
 def extra_workloads(args):
@@ -396,14 +441,14 @@ def cpu_model():
     return None
 
 
-def dropin_harness(pm, q, n, cases, first_c):
+def dropin_harness(pm, q, n, cases, first_c, backend=None):
     """Runs the `solution` harness (fherma_init once, each fherma_run timed
     alone, as main.cpp:181-186 does) over a point directory of these cases;
     returns the per-case milliseconds and whether case 0 equals first_c."""
     import tempfile
     with tempfile.TemporaryDirectory(prefix="polymul_point_") as d:
         pm.pointdir.write_point(d, q, n, cases)
-        report = pm.pointdir.run_solution(d, timeout=300)
+        report = pm.pointdir.run_solution(d, timeout=300, backend=backend)
         ms = [c["seconds"] * 1e3 for c in report["cases"] if c["status"] == "ok"]
         c0 = pm.pointdir.read_output(d, 0, n)
     return ms, bool(len(ms) == len(cases) and c0 is not None and np.array_equal(c0, first_c))

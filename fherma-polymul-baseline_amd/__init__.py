@@ -358,14 +358,22 @@ class Context:
         return {1 << lg: self.get_outer(1 << lg) for lg in range(1, self.info()["max_n"].bit_length())}
 
     def load_schedule(self, path):
-        """Apply a schedule written by `autotune_plain` / `autotune_batch`
-        (sizes this context cannot hold are skipped); returns
-        {"outer": {n: u}, "batch_outer": {n: u}, "batch_tiles_per_block":
-        {tile_n: tiles}} as applied."""
+        """Apply a schedule written by `autotune_plain` / `autotune_batch` /
+        `autotune_backend` (sizes this context cannot hold, and a quartic
+        backend its modulus does not allow, are skipped); returns {"outer":
+        {n: u}, "batch_outer": {n: u}, "batch_tiles_per_block": {tile_n:
+        tiles}} and "backend" when one was set, as applied."""
         import json
         with open(path) as f:
             doc = json.load(f)
         applied = {"outer": {}, "batch_outer": {}, "batch_tiles_per_block": {}}
+        if "backend" in doc:
+            try:
+                self.set_backend(doc["backend"])
+                applied["backend"] = doc["backend"]
+            except PolymulError as err:
+                if err.status != PM_EUNSUPPORTED:
+                    raise
         for key, setter in (("outer", self.set_outer), ("batch_outer", self.set_batch_outer),
                             ("batch_tiles_per_block", self.set_batch_tiles_per_block)):
             for n_str, outer in doc.get(key, {}).items():
@@ -679,6 +687,39 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
         _save_schedule(path, "batch_tiles_per_block", {t: r["tiles"] for t, r in tile_result.items()})
     result["tiles_per_block"] = tile_result
     return result
+
+
+def autotune_backend(ctx, n, count=16, reps=5, path=None, stream=None):
+    """The backend the batched path runs fastest for products of size n:
+    times a graph launch of `count` products with each backend the context's
+    q allows (the quartic one only for q = 2^868 - delta, delta < 2^28), keeps
+    the faster in the context and returns {"backend": name, "ms": {name:
+    median ms per launch}}; with `path` the choice is written as "backend" in
+    the schedule file `load_schedule` reads."""
+    import json
+    import torch
+    from . import workload as wl
+    s = stream if stream is not None else torch.cuda.Stream()
+    a, b = wl.poly(n, 0, n, ctx.q), wl.poly(n, 1, n, ctx.q)
+    times = {}
+    for name in ("general", "quartic"):
+        try:
+            ctx.set_backend(name)
+        except PolymulError:
+            continue  # not a special modulus
+        times[name] = _time_batch(ctx, [n] * count, [(a, b)] * count, reps, s)
+    best = min(times, key=times.get)
+    ctx.set_backend(best)
+    if path is not None:
+        doc = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                doc = json.load(f)
+        doc["device"] = torch.cuda.get_device_name(0)
+        doc["backend"] = best
+        with open(path, "w") as f:
+            json.dump(doc, f, indent=1)
+    return {"backend": best, "ms": times}
 
 
 class Solver:

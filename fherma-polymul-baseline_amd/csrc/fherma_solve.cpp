@@ -8,6 +8,8 @@
 // fherma_free  -- releases the state (rns/solve.cu:854).
 // Failures throw std::runtime_error naming the operation, as the reference
 // does (rns/solve.cu:421-423); the harness records them per case.
+#include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
@@ -32,8 +34,20 @@ void* fherma_init(const fherma::Point& p) {
     // reference's init does (host_affinity.h:14-46); unknown topology: as is
     if (pm_placement_enabled()) (void)pm_place_near_device(0, nullptr, nullptr);
     pm_context* ctx = nullptr;
-    const int status = pm_create(p.N, p.W, p.L, p.q.data.data(), 0, &ctx);
+    int status = pm_create(p.N, p.W, p.L, p.q.data.data(), 0, &ctx);
     if (status != PM_OK) raise("fherma_init", nullptr, status);
+    // PM_BACKEND=quartic selects the special-modulus backend (q = 2^868 -
+    // delta, delta < 2^28), as the reference's build picks quartic/solve.cu
+    // (CMakeLists.txt:29-38); any other q keeps the general one
+    const char* backend = std::getenv("PM_BACKEND");
+    if (backend && std::strcmp(backend, "quartic") == 0) {
+        status = pm_set_backend(ctx, PM_BACKEND_QUARTIC);
+        if (status != PM_OK && status != PM_EUNSUPPORTED) {
+            pm_destroy(ctx);
+            throw std::runtime_error(std::string("fherma_init failed: quartic backend (status ") +
+                                     std::to_string(status) + ")");
+        }
+    }
     return ctx;
 }
 

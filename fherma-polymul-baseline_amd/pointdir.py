@@ -87,9 +87,13 @@ def solution_binary():
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "solution")
 
 
-def run_solution(path, binary=None, timeout=600, quiet=True):
+def run_solution(path, binary=None, timeout=600, quiet=True, backend=None):
     """Runs `solution <path>` (fherma_init once, each fherma_run timed alone)
-    and returns the parsed out/results.json."""
+    and returns the parsed out/results.json; backend="quartic" sets
+    PM_BACKEND for the process (the special-modulus backend)."""
+    env = None
+    if backend is not None:
+        env = dict(os.environ, PM_BACKEND=backend)
     subprocess.run([binary or solution_binary(), str(path)], check=True, timeout=timeout,
-                   stdout=subprocess.DEVNULL if quiet else None)
+                   stdout=subprocess.DEVNULL if quiet else None, env=env)
     return read_results(path)

@@ -140,3 +140,29 @@ def test_quartic_serving(pm, workload):
         assert (c == w).all()
     server.close()
     ctx.close()
+
+
+def test_autotune_backend(pm, workload, tmp_path):
+    """The backend autotuner times both backends at the special modulus,
+    keeps one, writes it to the schedule file, and a fresh context that loads
+    the file runs that backend with unchanged products; at a general modulus
+    it keeps the general backend."""
+    q = workload.special_modulus(DELTA)
+    n = 1 << 12
+    ctx = pm.Context(q, n)
+    a, b = workload.family("dense-random-1", n, q, 3)
+    want = ctx.multiply(a, b)
+    path = str(tmp_path / "schedule.json")
+    got = pm.autotune_backend(ctx, n, count=4, reps=2, path=path)
+    say("autotune", got)
+    assert set(got["ms"]) == {"general", "quartic"} and got["backend"] in got["ms"]
+    assert ctx.backend()[0] == got["backend"]
+    ctx.close()
+    ctx = pm.Context(q, n)
+    assert ctx.load_schedule(path)["backend"] == got["backend"]
+    assert ctx.backend()[0] == got["backend"]
+    assert (ctx.multiply(a, b) == want).all()
+    ctx.close()
+    ctx = pm.Context(workload.modulus(1), n)
+    assert pm.autotune_backend(ctx, n, count=2, reps=1)["backend"] == "general"
+    ctx.close()

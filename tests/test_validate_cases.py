@@ -97,10 +97,13 @@ def test_verify_flags_a_wrong_output(tmp_path, pm):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,delta", [(32, None), (1024, None), (4096, 11993087)])
-def test_solution_passes_validation(tmp_path, n, delta):
-    """The drop-in `solution` over the validator's cases: every case exact."""
-    print("validate", n, delta, flush=True)
-    assert vc.main(["--n", str(n), "--dir", str(tmp_path)] + (["--delta", str(delta)] if delta else [])) == 0
+@pytest.mark.parametrize("n,delta,backend", [(32, None, None), (1024, None, None), (4096, 11993087, None),
+                                             (32, 11993087, "quartic"), (4096, 11993087, "quartic")])
+def test_solution_passes_validation(tmp_path, n, delta, backend):
+    """The drop-in `solution` over the validator's cases: every case exact,
+    also with the quartic backend chosen through PM_BACKEND."""
+    print("validate", n, delta, backend, flush=True)
+    assert vc.main(["--n", str(n), "--dir", str(tmp_path)] + (["--delta", str(delta)] if delta else [])
+                   + (["--backend", backend] if backend else [])) == 0
     rep = json.loads((tmp_path / "validation.json").read_text())
     assert rep["passed"] and len(rep["verdicts"]) == 9

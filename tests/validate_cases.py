@@ -172,6 +172,8 @@ def main(argv=None):
     ap.add_argument("--dir", help="point directory (default: local/n<N>-w868[-c<delta>])")
     ap.add_argument("--binary", help="harness to run over the directory (default: the package's solution)")
     ap.add_argument("--existing", action="store_true", help="reuse the cases already in the directory")
+    ap.add_argument("--backend", choices=("general", "quartic"),
+                    help="product backend the harness's fherma_init selects (PM_BACKEND)")
     ap.add_argument("--verify-only", action="store_true")
     args = ap.parse_args(argv)
     if args.delta is not None and not 0 < args.delta < (1 << 28):
@@ -184,7 +186,7 @@ def main(argv=None):
     if not args.verify_only:
         if not args.existing:
             make(path, args.n, args.delta)
-        pm.pointdir.run_solution(path, binary=binary)
+        pm.pointdir.run_solution(path, binary=binary, backend=args.backend)
     report = verify(path, binary)
     for v in report["verdicts"]:
         print(v["case"], "PASS" if v["passed"] else "FAIL", v["seconds"])
