@@ -488,7 +488,9 @@ def run_uneven(args):
     # from the autotuner, on small batches of each size
     tuned = pm.autotune_batch(ctx, sorted(set(ns)), count=8, reps=3, stream=stream)
     tuned_tiles = tuned.pop("tiles_per_block")
-    print(f"[bench] batch schedule: {tuned} tiles per block {tuned_tiles}", file=sys.stderr, flush=True)
+    tuned_waves = tuned.pop("crt_waves")
+    print(f"[bench] batch schedule: {tuned} tiles per block {tuned_tiles} CRT waves {tuned_waves}",
+          file=sys.stderr, flush=True)
     results = {}
     outputs = {}
     for mode in ("phased",):
@@ -541,6 +543,8 @@ def run_uneven(args):
                                             for n, r in tuned.items()},
             "batched": results,
             "autotuned_tiles_per_block": {str(t): r["tiles"] for t, r in tuned_tiles.items()},
+            "autotuned_crt_waves": tuned_waves["waves"],
+            "autotune_crt_waves_ms": {str(w): round(t, 4) for w, t in tuned_waves["ms"].items()},
             "plain_one_at_a_time_products_per_s": round(len(ns) / (plain_total_ms * 1e-3), 1),
             "bit_exact_vs_plain": True}
     ctx.close()

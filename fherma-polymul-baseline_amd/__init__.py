@@ -89,6 +89,10 @@ def library():
     lib.pm_get_batch_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_set_batch_tiles_per_block.argtypes = [P, u32, u32]
     lib.pm_get_batch_tiles_per_block.argtypes = [P, u32, ctypes.POINTER(u32)]
+    lib.pm_set_batch_crt_waves.argtypes = [P, u32]
+    lib.pm_get_batch_crt_waves.argtypes = [P, ctypes.POINTER(u32)]
+    lib.pm_set_conversion_slices.argtypes = [P, u32, u32]
+    lib.pm_get_conversion_slices.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_stage_convert.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_convert_scalar.argtypes = [P, u32, P, P, P, P, P]
     lib.pm_stage_ntt.argtypes = [P, u32, u32, P, P, P]
@@ -122,7 +126,8 @@ EXPORTED = ["pm_plan_rns", "pm_plan_quartic", "pm_create", "pm_destroy", "pm_las
             "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_set_io_wait_limit",
             "pm_inject_io_stall", "pm_place_near_device", "pm_multiply_device", "pm_multiply_device_noalloc",
             "pm_synchronize", "pm_set_backend", "pm_get_backend", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
-            "pm_set_batch_tiles_per_block", "pm_get_batch_tiles_per_block",
+            "pm_set_batch_tiles_per_block", "pm_get_batch_tiles_per_block", "pm_set_batch_crt_waves",
+            "pm_get_batch_crt_waves", "pm_set_conversion_slices", "pm_get_conversion_slices",
             "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
@@ -333,7 +338,8 @@ class Context:
 
     def set_batch_outer(self, n, outer):
         """Batch schedule: outer radix-2 passes for products of size n in
-        batches and served launches created afterwards."""
+        batches and served launches created afterwards, on the context's
+        current backend (each backend keeps its own)."""
         self._check("pm_set_batch_outer", self._lib.pm_set_batch_outer(self._h, int(n), int(outer)))
 
     def get_batch_outer(self, n):
@@ -342,8 +348,9 @@ class Context:
         return v.value
 
     def set_batch_tiles_per_block(self, tile_n, tiles):
-        """Batch schedule: tiles of size tile_n per block (0 = the default,
-        a 1024-thread block) in batches and served launches created afterwards."""
+        """Batch schedule: tiles of size tile_n per block (0 = the default:
+        one tile from 2^13 up, a 1024-thread block of smaller tiles) in
+        batches and served launches created afterwards."""
         self._check("pm_set_batch_tiles_per_block",
                     self._lib.pm_set_batch_tiles_per_block(self._h, int(tile_n), int(tiles)))
 
@@ -351,6 +358,27 @@ class Context:
         v = ctypes.c_uint32()
         self._check("pm_get_batch_tiles_per_block",
                     self._lib.pm_get_batch_tiles_per_block(self._h, int(tile_n), ctypes.byref(v)))
+        return v.value
+
+    def set_batch_crt_waves(self, waves):
+        """Batch schedule: waves per block of the MFMA CRT (2, 4 or 8; 0 =
+        the default 8) in batches and served launches created afterwards."""
+        self._check("pm_set_batch_crt_waves", self._lib.pm_set_batch_crt_waves(self._h, int(waves)))
+
+    def get_batch_crt_waves(self):
+        v = ctypes.c_uint32()
+        self._check("pm_get_batch_crt_waves", self._lib.pm_get_batch_crt_waves(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_conversion_slices(self, n, slices):
+        """Plain-path schedule: slices of the primes the conversion of a
+        product of size n spreads over (1, 2 or 4; 0 = chosen at launch)."""
+        self._check("pm_set_conversion_slices", self._lib.pm_set_conversion_slices(self._h, int(n), int(slices)))
+
+    def get_conversion_slices(self, n):
+        v = ctypes.c_uint32()
+        self._check("pm_get_conversion_slices",
+                    self._lib.pm_get_conversion_slices(self._h, int(n), ctypes.byref(v)))
         return v.value
 
     def schedule(self):
@@ -362,11 +390,12 @@ class Context:
         `autotune_backend` (sizes this context cannot hold, and a quartic
         backend its modulus does not allow, are skipped); returns {"outer":
         {n: u}, "batch_outer": {n: u}, "batch_tiles_per_block": {tile_n:
-        tiles}} and "backend" when one was set, as applied."""
+        tiles}, "conversion_slices": {n: s}} and "backend" and
+        "batch_crt_waves" when they were set, as applied."""
         import json
         with open(path) as f:
             doc = json.load(f)
-        applied = {"outer": {}, "batch_outer": {}, "batch_tiles_per_block": {}}
+        applied = {"outer": {}, "batch_outer": {}, "batch_tiles_per_block": {}, "conversion_slices": {}}
         if "backend" in doc:
             try:
                 self.set_backend(doc["backend"])
@@ -374,8 +403,12 @@ class Context:
             except PolymulError as err:
                 if err.status != PM_EUNSUPPORTED:
                     raise
+        if "batch_crt_waves" in doc:
+            self.set_batch_crt_waves(doc["batch_crt_waves"])
+            applied["batch_crt_waves"] = int(doc["batch_crt_waves"])
         for key, setter in (("outer", self.set_outer), ("batch_outer", self.set_batch_outer),
-                            ("batch_tiles_per_block", self.set_batch_tiles_per_block)):
+                            ("batch_tiles_per_block", self.set_batch_tiles_per_block),
+                            ("conversion_slices", self.set_conversion_slices)):
             for n_str, outer in doc.get(key, {}).items():
                 n = int(n_str)
                 if n <= self.info()["max_n"]:
@@ -533,15 +566,32 @@ class Server:
 
 
 def autotune_plain(ctx, ns, reps=20, path=None, stream=None):
-    """Per-N schedule of the plain path: times one product of each size with
-    0, 1 and 2 outer radix-2 passes (where the tile stays within 2..2^15) on
-    the GPU, keeps the fastest in the context and returns
-    {n: {"outer": u, "ms": {u: median ms}}}; with `path` the choice is also
-    written as JSON ({"device": ..., "outer": {n: u}}) for `load_schedule`."""
-    import json
+    """Per-N schedule of the plain path, timed on the GPU one product at a
+    time: first the outer radix-2 passes (0, 1, 2, where the tile stays
+    within 2..2^15), then, with the best of those, the slices of the primes
+    the conversion spreads over (1, 2, 4).  Keeps the fastest in the context
+    and returns {n: {"outer": u, "ms": {u: median ms}, "slices": s,
+    "slices_ms": {s: median ms}}}; with `path` the choice is also written as
+    JSON ({"device": ..., "outer": {n: u}, "conversion_slices": {n: s}}) for
+    `load_schedule`."""
     import torch
     from . import workload as wl
     s = stream if stream is not None else torch.cuda.Stream()
+
+    def timed(n, a, b, c):
+        with torch.cuda.stream(s):
+            for _ in range(3):
+                ctx.multiply_device(n, a, b, c, s.cuda_stream)
+            ms = []
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(s)
+                ctx.multiply_device(n, a, b, c, s.cuda_stream)
+                e1.record(s)
+                e1.synchronize()
+                ms.append(e0.elapsed_time(e1))
+        return float(np.median(ms))
+
     result = {}
     for n in sorted(set(int(x) for x in ns)):
         logn = n.bit_length() - 1
@@ -554,23 +604,19 @@ def autotune_plain(ctx, ns, reps=20, path=None, stream=None):
             if outer > logn or logn - outer > 15 or logn - outer < 1:
                 continue
             ctx.set_outer(n, outer)
-            with torch.cuda.stream(s):
-                for _ in range(3):
-                    ctx.multiply_device(n, a, b, c, s.cuda_stream)
-                ms = []
-                for _ in range(reps):
-                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    e0.record(s)
-                    ctx.multiply_device(n, a, b, c, s.cuda_stream)
-                    e1.record(s)
-                    e1.synchronize()
-                    ms.append(e0.elapsed_time(e1))
-            times[outer] = float(np.median(ms))
+            times[outer] = timed(n, a, b, c)
         best = min(times, key=times.get)
         ctx.set_outer(n, best)
-        result[n] = {"outer": best, "ms": times}
+        slice_ms = {}
+        for sl in (1, 2, 4):
+            ctx.set_conversion_slices(n, sl)
+            slice_ms[sl] = timed(n, a, b, c)
+        best_sl = min(slice_ms, key=slice_ms.get)
+        ctx.set_conversion_slices(n, best_sl)
+        result[n] = {"outer": best, "ms": times, "slices": best_sl, "slices_ms": slice_ms}
     if path is not None:
         _save_schedule(path, "outer", {n: r["outer"] for n, r in result.items()})
+        _save_schedule(path, "conversion_slices", {n: r["slices"] for n, r in result.items()})
     return result
 
 
@@ -634,13 +680,16 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
        every valid tiles-per-block count and the fastest kept;
     2. outer stages (0..2: the tile size, and radix-2 or radix-4 outer
        stages in the conversion and the CRT) for each N, timed the same way
-       with the tiles per block of step 1.
+       with the tiles per block of step 1;
+    3. waves per block of the MFMA CRT (2, 4, 8): the sum over the sizes of
+       a launch of `count` products each, with the schedule of steps 1-2.
 
-    Both are kept in the context (for batches and served launches created
-    afterwards).  Returns {n: {"outer": u, "ms": {u: median ms per launch}},
-    "tiles_per_block": {tile_n: {"tiles": t, "ms": {t: ms}}}}; with `path`
-    the choice is also written to the schedule file `load_schedule` reads
-    ("batch_outer" and "batch_tiles_per_block")."""
+    All are kept in the context (for batches and served launches created
+    afterwards, on its current backend).  Returns {n: {"outer": u, "ms": {u:
+    median ms per launch}}, "tiles_per_block": {tile_n: {"tiles": t, "ms":
+    {t: ms}}}, "crt_waves": {"waves": w, "ms": {w: ms}}}; with `path` the
+    choice is also written to the schedule file `load_schedule` reads
+    ("batch_outer", "batch_tiles_per_block" and "batch_crt_waves")."""
     import torch
     from . import workload as wl
     s = stream if stream is not None else torch.cuda.Stream()
@@ -682,10 +731,24 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
         best = min(times, key=times.get)
         ctx.set_batch_outer(n, best)
         result[n] = {"outer": best, "ms": times}
+    wave_ms = {}
+    for w in (2, 4, 8):
+        ctx.set_batch_crt_waves(w)
+        wave_ms[w] = sum(_time_batch(ctx, [n] * count, [(wl.poly(n, 0, n, ctx.q), wl.poly(n, 1, n, ctx.q))] * count,
+                                     reps, s) for n in sizes)
+    best_w = min(wave_ms, key=wave_ms.get)
+    ctx.set_batch_crt_waves(best_w)
     if path is not None:
+        import json
         _save_schedule(path, "batch_outer", {n: r["outer"] for n, r in result.items()})
         _save_schedule(path, "batch_tiles_per_block", {t: r["tiles"] for t, r in tile_result.items()})
+        with open(path) as f:
+            doc = json.load(f)
+        doc["batch_crt_waves"] = best_w
+        with open(path, "w") as f:
+            json.dump(doc, f, indent=1)
     result["tiles_per_block"] = tile_result
+    result["crt_waves"] = {"waves": best_w, "ms": wave_ms}
     return result
 
 

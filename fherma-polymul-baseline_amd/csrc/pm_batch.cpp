@@ -80,6 +80,7 @@ static int upload_worklist(pm_context* ctx, pm_batch* b, const pmd::Job* d_jobs,
 int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint32_t* const* d_a,
                    const uint32_t* const* d_b, uint32_t* const* d_c) {
     b->tables = pm_tables(ctx);  // the backend at creation stays the batch's
+    b->crt_waves = ctx->batch_crt_waves;
     const uint32_t k = b->tables.k, count = b->count;
     b->logn.resize(count);
     for (uint32_t i = 0; i < count; ++i) {
@@ -94,7 +95,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
         const size_t n = size_t(1) << b->logn[i];
         pmd::Job& j = jobs[i];
         j.logn = b->logn[i];
-        j.outer = ctx->batch_outer[b->logn[i]];
+        j.outer = ctx->batch_outer[b->tables.quartic ? PM_BACKEND_QUARTIC : PM_BACKEND_GENERAL][b->logn[i]];
         j.fused = !b->separate_outer && !b->tables.quartic && pmd::outer_fusable(j.logn, j.outer) ? 1u : 0u;
         const bool caller_io = d_a && d_b && d_c;
         if (caller_io) {
@@ -156,7 +157,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
         hipError_t le = hipSuccess;
         for (uint32_t q = 0; q < parts && le == hipSuccess; ++q) {
             if (q > 0) le = hipStreamWaitEvent(streams[q], first_done[q - 1], 0);  // fork
-            const auto phases = pmd::phase_list(part_ranges[q], b->scalar_conversion, b->scalar_crt);
+            const auto phases = pmd::phase_list(part_ranges[q], b->scalar_conversion, b->scalar_crt, b->crt_waves);
             for (size_t i = 0; i < phases.size() && le == hipSuccess; ++i) {
                 le = pmd::launch_phase(b->tables, part_args[q], phases[i].kind, phases[i].first, phases[i].count,
                                        streams[q]);
@@ -372,7 +373,8 @@ int pm_batch_phase_times(pm_batch* b, void* hip_stream, uint32_t max, uint32_t* 
                          uint32_t* count) {
     if (!b || !count) return PM_EINVAL;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->ctx->stream;
-    const std::vector<pmd::Phase> phases = pmd::phase_list(b->ranges, b->scalar_conversion, b->scalar_crt);
+    const std::vector<pmd::Phase> phases =
+        pmd::phase_list(b->ranges, b->scalar_conversion, b->scalar_crt, b->crt_waves);
     *count = uint32_t(phases.size());
     if (phases.size() > max || !kinds || !ms) return PM_OK;
     std::vector<hipEvent_t> ev(phases.size() + 1, nullptr);

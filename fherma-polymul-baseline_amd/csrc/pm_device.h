@@ -22,11 +22,12 @@ constexpr uint32_t kCrtPerBlock = 256;
 // MFMA CRT: groups of 64 coefficients per wave, 8 waves per block; per wave
 // the LDS holds 31 words (low 32 bits + signed high 16 bits) per coefficient
 constexpr uint32_t kCrtWaveCoefs = 64;
-constexpr uint32_t kCrtMfmaWaves = 8;   // waves per block of the batched CRT
+constexpr uint32_t kCrtMfmaWaves = 8;   // default waves per block of the batched CRT (2, 4 or 8)
 // waves per block of the single-product CRT: one product has few groups (512
 // at N = 2^15), and blocks of 4 spread them over twice as many CUs
 constexpr uint32_t kCrtPlainWaves = 4;
 constexpr uint32_t kCrtWords = 31;
+constexpr size_t kLdsPerCu = 160u * 1024u;  // gfx950
 // words per row: a partial-sum store writes words 4r..4r+3 of 16 coefficients,
 // lanes (g, col) at g*stride + col; stride = 16 mod 64 puts the 64 lanes on
 // 64 distinct banks (and each 32-lane half on distinct banks of 32)
@@ -93,6 +94,9 @@ struct Job {
     // coefficients a quarter of n apart, so each lane holds a whole radix-4
     // column of its prime
     uint32_t fused;
+    // plain-path conversion: slices of the primes over blockIdx.z (1, 2 or
+    // 4), 0 for the launch's own choice
+    uint32_t conv_slices;
 };
 
 // whether a product of 2^logn with `outer` outer stages can fuse them

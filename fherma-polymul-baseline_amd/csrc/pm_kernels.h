@@ -58,12 +58,15 @@ enum PhaseKind : uint32_t {
 struct Phase {
     uint32_t kind, first, count;
 };
-std::vector<Phase> phase_list(const PhaseRanges& ranges, bool scalar_conversion, bool scalar_crt);
+// crt_waves: waves per block of the MFMA CRT (2, 4 or 8; anything else: 8),
+// carried in the CRT phase's kind as kPhaseCrt | log2(waves) << 16
+std::vector<Phase> phase_list(const PhaseRanges& ranges, bool scalar_conversion, bool scalar_crt,
+                              uint32_t crt_waves = 0);
 hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, uint32_t first, uint32_t count,
                         hipStream_t stream);
 // one launch per phase (and per tile size class) over the same work list
 hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& ranges, bool scalar_conversion,
-                         bool scalar_crt, hipStream_t stream);
+                         bool scalar_crt, uint32_t crt_waves, hipStream_t stream);
 // device address of the static residue scratch (kScratchWords words) on
 // `device`, which must be the current device; looked up once per device
 hipError_t scratch_address(int device, uint32_t** out);

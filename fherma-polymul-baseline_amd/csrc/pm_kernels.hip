@@ -1525,11 +1525,14 @@ __global__ void __launch_bounds__(kCrtPlainWaves * 64) k_crt_mfma(Tables tb, Job
 }
 
 // CRT items cover kBlock coefficients = kBlock / 64 groups each; groups past a
-// product's n (small N) load clamped data and store nothing
-__global__ void __launch_bounds__(kCrtMfmaWaves * 64) k_items_crt_mfma(Tables tb, BatchArgs ba, uint32_t first,
-                                                                       uint32_t count) {
+// product's n (small N) load clamped data and store nothing.  W waves per
+// block: 8 fill a CU's LDS (about 154 KB) and its VGPRs (224 per lane, two
+// waves per SIMD) with one block; 4 or 2 spread a small batch over more CUs.
+template <int W>
+__global__ void __launch_bounds__(W * 64) k_items_crt_mfma(Tables tb, BatchArgs ba, uint32_t first,
+                                                           uint32_t count) {
     constexpr uint32_t per_item = kBlock / kCrtWaveCoefs;
-    crt_group_loop<kCrtMfmaWaves>(tb, count * per_item, [&](uint32_t gi, CrtGroup& w) {
+    crt_group_loop<W>(tb, count * per_item, [&](uint32_t gi, CrtGroup& w) {
         const Item it = ba.items[first + gi / per_item];
         const Job job = ba.jobs[it.job];
         const uint32_t n = 1u << job.logn, c0 = it.a0 * kBlock + (gi % per_item) * kCrtWaveCoefs;
@@ -1761,6 +1764,18 @@ hipError_t launch_egress(const uint32_t* src, uint32_t* dst_host, size_t chunk_w
     return hipSuccess;
 }
 
+// Slices of the primes (over blockIdx.z) of a plain-path conversion of
+// `waves` waves: the job's choice (1, 2 or 4), else up to 4 while that leaves
+// fewer than ~4 waves per CU, so a single product (or one chunk of it) still
+// spreads over the whole device
+static uint32_t conv_slices(const Tables& tb, const Job& job, uint32_t waves) {
+    if (job.conv_slices == 1 || job.conv_slices == 2 || job.conv_slices == 4)
+        return job.conv_slices < tb.quads ? job.conv_slices : 1u;
+    uint32_t slices = 1;
+    while (slices < 4 && slices < tb.quads && waves * slices < 4 * tb.cus) slices *= 2;
+    return slices;
+}
+
 hipError_t launch_convert_range(const Tables& tb, const Job& job, uint32_t c_first, uint32_t c_end,
                                 uint32_t poly_first, uint32_t polys, hipStream_t stream) {
     const uint32_t n = 1u << job.logn;
@@ -1773,26 +1788,22 @@ hipError_t launch_convert_range(const Tables& tb, const Job& job, uint32_t c_fir
     if (job.fused && (c_first != 0 || c_end != n || !outer_fusable(job.logn, job.outer))) return hipErrorInvalidValue;
     if (job.fused) {
         if (poly_first != 0 || polys != 2) return hipErrorInvalidValue;
-        // 4-wave blocks of one slot (128 coefficients) per wave; primes split
-        // over up to 4 slices while that leaves fewer than ~4 waves per CU
+        // 4-wave blocks of one slot (128 coefficients) per wave, the primes
+        // split over conv_slices slices
         const uint32_t slots = n / kConvSlotCoefs, threads = kConvSmallBlock;
         const uint32_t per_block = threads / kWaveSize, blocks = (slots + per_block - 1) / per_block;
         const uint32_t waves = blocks * polys * per_block;
-        uint32_t slices = 1;
-        while (slices < 4 && slices < tb.quads && waves * slices < 4 * tb.cus) slices *= 2;
+        const uint32_t slices = conv_slices(tb, job, waves);
         hipLaunchKernelGGL(k_convert_fused, dim3(blocks, 2, slices), dim3(threads), 0, stream, tb, job);
         return hipGetLastError();
     }
     const uint32_t count = c_end - c_first;
-    // 4-wave blocks, and the primes split over up to 4 slices while that
-    // leaves fewer than ~4 waves per CU: a single product (or one chunk of it)
-    // then still spreads over the whole device
+    // 4-wave blocks, the primes split over conv_slices slices
     uint32_t threads = kConvSmallBlock;
     if (count < threads) threads = ((count + kConvPerWave - 1) / kConvPerWave) * kWaveSize;
     const uint32_t blocks = (count + threads - 1) / threads;
     const uint32_t waves = blocks * polys * (threads / kWaveSize);
-    uint32_t slices = 1;
-    while (slices < 4 && slices < tb.quads && waves * slices < 4 * tb.cus) slices *= 2;
+    const uint32_t slices = conv_slices(tb, job, waves);
     hipLaunchKernelGGL(k_convert, dim3(blocks, polys, slices), dim3(threads), 0, stream, tb, job, c_first, c_end,
                        poly_first);
     return hipGetLastError();
@@ -1840,6 +1851,18 @@ static hipError_t launch_items_tile(const Tables& tb, const BatchArgs& ba, uint3
     return hipSuccess;
 }
 
+// W-wave CRT blocks, as many as the LDS lets share a CU (two of 2 waves,
+// one of 4 or 8), looping over the groups
+template <int W>
+static void launch_items_crt(const Tables& tb, const BatchArgs& ba, uint32_t first, uint32_t count,
+                             uint32_t groups, hipStream_t stream) {
+    constexpr uint32_t per_cu = crt_block_lds<W>() * 2 <= kLdsPerCu ? 2u : 1u;
+    uint32_t grid = (groups + W - 1) / W;
+    grid = grid < per_cu * tb.cus ? grid : per_cu * tb.cus;
+    hipLaunchKernelGGL(k_items_crt_mfma<W>, dim3(grid), dim3(W * 64), crt_block_lds<W>(), stream, tb, ba, first,
+                       count);
+}
+
 hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, uint32_t first, uint32_t count,
                         hipStream_t stream) {
     if (!count) return hipSuccess;
@@ -1874,10 +1897,13 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
             break;
         case kPhaseCrt: {
             const uint32_t groups = count * (kBlock / kCrtWaveCoefs);
-            uint32_t grid = (groups + kCrtMfmaWaves - 1) / kCrtMfmaWaves;
-            grid = grid < tb.cus ? grid : tb.cus;
-            hipLaunchKernelGGL(k_items_crt_mfma, dim3(grid), dim3(kCrtMfmaWaves * 64), crt_block_lds<kCrtMfmaWaves>(),
-                               stream, tb, ba, first, count);
+            switch ((kind >> 16) & 0xFu) {
+                case 1: launch_items_crt<2>(tb, ba, first, count, groups, stream); break;
+                case 2: launch_items_crt<4>(tb, ba, first, count, groups, stream); break;
+                case 0:  // the default
+                case 3: launch_items_crt<8>(tb, ba, first, count, groups, stream); break;
+                default: return hipErrorInvalidValue;
+            }
             break;
         }
         case kPhaseCrtScalar:
@@ -1909,7 +1935,9 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
     return hipGetLastError();
 }
 
-std::vector<Phase> phase_list(const PhaseRanges& r, bool scalar_conversion, bool scalar_crt) {
+std::vector<Phase> phase_list(const PhaseRanges& r, bool scalar_conversion, bool scalar_crt, uint32_t crt_waves) {
+    uint32_t crt_lg = 0;  // log2 of the CRT's waves per block (0: the default)
+    if (crt_waves == 2 || crt_waves == 4 || crt_waves == 8) crt_lg = uint32_t(__builtin_ctz(crt_waves));
     std::vector<Phase> out;
     if (r.conv_count) out.push_back({scalar_conversion ? kPhaseConvScalar : kPhaseConv, r.conv_first, r.conv_count});
     if (r.fwd_count) out.push_back({kPhaseOuterFwd, r.fwd_first, r.fwd_count});
@@ -1920,13 +1948,14 @@ std::vector<Phase> phase_list(const PhaseRanges& r, bool scalar_conversion, bool
         out.push_back({kPhaseTile | (tlog << 8) | (lg << 16), r.tile_first[tlog], r.tile_count[tlog]});
     }
     if (r.inv_count) out.push_back({kPhaseOuterInv, r.inv_first, r.inv_count});
-    if (r.crt_count) out.push_back({scalar_crt ? kPhaseCrtScalar : kPhaseCrt, r.crt_first, r.crt_count});
+    if (r.crt_count)
+        out.push_back({scalar_crt ? kPhaseCrtScalar : kPhaseCrt | (crt_lg << 16), r.crt_first, r.crt_count});
     return out;
 }
 
 hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& r, bool scalar_conversion,
-                         bool scalar_crt, hipStream_t stream) {
-    for (const Phase& ph : phase_list(r, scalar_conversion, scalar_crt)) {
+                         bool scalar_crt, uint32_t crt_waves, hipStream_t stream) {
+    for (const Phase& ph : phase_list(r, scalar_conversion, scalar_crt, crt_waves)) {
         const hipError_t e = launch_phase(tb, ba, ph.kind, ph.first, ph.count, stream);
         if (e != hipSuccess) return e;
     }
@@ -1978,7 +2007,8 @@ namespace pmd {
 // words) or one tile of 2^15
 constexpr uint32_t kTileLdsMaxWords =
     uint32_t(4 * Geo<13>::STRIDE) > kLdsWords ? uint32_t(4 * Geo<13>::STRIDE) : kLdsWords;
-static_assert(kTileLdsMaxWords * 4 <= 160 * 1024, "tile blocks fit the LDS of a CU");
+static_assert(kTileLdsMaxWords * 4 <= kLdsPerCu, "tile blocks fit the LDS of a CU");
+static_assert(crt_block_lds<8>() <= kLdsPerCu, "an 8-wave CRT block fits the LDS of a CU");
 template <int TLOG>
 static hipError_t allow_lds() {
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv_tile<TLOG>),
@@ -1998,9 +2028,14 @@ hipError_t configure_kernels() {
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  int(crt_block_lds<kCrtPlainWaves>()))) != hipSuccess)
         return e;
-    if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_crt_mfma),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 int(crt_block_lds<kCrtMfmaWaves>()))) != hipSuccess)
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_crt_mfma<2>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(crt_block_lds<2>()))) != hipSuccess)
+        return e;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_crt_mfma<4>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(crt_block_lds<4>()))) != hipSuccess)
+        return e;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_crt_mfma<8>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(crt_block_lds<8>()))) != hipSuccess)
         return e;
     if ((e = allow_lds<1>()) != hipSuccess) return e;
     if ((e = allow_lds<2>()) != hipSuccess) return e;

@@ -77,7 +77,12 @@ uint32_t pm_default_outer(uint32_t logn) { return logn <= 13 ? 0u : (logn == 14 
 // the others' arithmetic.  Measured at 128 x N = 2^15 (tile phase, same
 // residues): 2^15 tiles in 1024-thread blocks 1.68 ms, 2^14 tiles one per
 // block 1.45 ms, 2^13 tiles one per block 1.24 ms.
-uint32_t pm_default_batch_outer(uint32_t logn) {
+// The quartic backend runs outer stages as passes of their own (two HBM
+// round trips), so its batches keep the largest tile: measured at 128 x N =
+// 2^15 and 64 rows, 2^13 tiles plus two outer passes 2.52 ms, one 2^15 tile
+// about 1.8 ms.
+uint32_t pm_default_batch_outer(uint32_t logn, bool quartic) {
+    if (quartic) return logn > 15 ? logn - 15 : 0u;
     return logn <= 13 ? 0u : (logn - 13 < 2 ? logn - 13 : 2u);
 }
 
@@ -274,7 +279,8 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     }
     for (uint32_t logn = 0; logn <= pm::kMaxLogN; ++logn) {
         ctx->outer[logn] = pm_default_outer(logn);
-        ctx->batch_outer[logn] = pm_default_batch_outer(logn);
+        ctx->batch_outer[PM_BACKEND_GENERAL][logn] = pm_default_batch_outer(logn, false);
+        ctx->batch_outer[PM_BACKEND_QUARTIC][logn] = pm_default_batch_outer(logn, true);
     }
     for (uint32_t tlog = 0; tlog < 16; ++tlog) ctx->batch_tiles[tlog] = pm_default_batch_tiles(tlog);
     // one product of size max_n on zero inputs: the runtime loads each kernel
@@ -405,6 +411,7 @@ int pm_run_plain(pm_context* ctx, const pmd::Job& job_in, uint32_t outer, hipStr
     pmd::Job job = job_in;
     job.outer = outer;
     job.fused = !tb.quartic && pmd::outer_fusable(job.logn, outer) ? 1u : 0u;
+    job.conv_slices = ctx->conv_slices[job.logn];
     hipError_t e = pm_workspace_acquire(ctx, stream);
     if (e == hipSuccess) e = pmd::launch_convert(tb, job, stream);
     if (e == hipSuccess) e = pmd::launch_ntt(tb, job, outer, stream);
@@ -443,7 +450,7 @@ int pm_multiply_device_noalloc(pm_context* ctx, uint32_t n, const uint32_t* d_a,
     // nothing in the scratch needs clearing
     const uint32_t outer = ctx->outer[logn];
     pmd::Job job{logn, outer, d_a, d_b, d_c, ctx->scratch, ctx->scratch + size_t(tb.k) * n,
-                 !tb.quartic && pmd::outer_fusable(logn, outer) ? 1u : 0u};
+                 !tb.quartic && pmd::outer_fusable(logn, outer) ? 1u : 0u, ctx->conv_slices[logn]};
     hipError_t e = pmd::launch_convert(tb, job, stream);
     if (e == hipSuccess) e = pmd::launch_ntt(tb, job, outer, stream);
     if (e == hipSuccess) e = pmd::launch_crt(tb, job, stream);
@@ -790,7 +797,7 @@ int pm_set_batch_outer(pm_context* ctx, uint32_t n, uint32_t outer) {
     if (st != PM_OK) return st;
     if (!pm_outer_valid(logn, outer, 5))
         return pm_fail(ctx, PM_EUNSUPPORTED, "batch outer stages must leave a tile of 2^5..2^15 (outer <= 2)");
-    ctx->batch_outer[logn] = outer;
+    ctx->batch_outer[ctx->backend][logn] = outer;
     return PM_OK;
 }
 
@@ -807,6 +814,40 @@ int pm_set_batch_tiles_per_block(pm_context* ctx, uint32_t tile_n, uint32_t tile
     return PM_OK;
 }
 
+int pm_set_batch_crt_waves(pm_context* ctx, uint32_t waves) {
+    if (!ctx) return PM_EINVAL;
+    if (waves == 0) waves = pmd::kCrtMfmaWaves;
+    if (waves != 2 && waves != 4 && waves != 8) return pm_fail(ctx, PM_EINVAL, "CRT waves per block: 2, 4 or 8");
+    ctx->batch_crt_waves = waves;
+    return PM_OK;
+}
+
+int pm_get_batch_crt_waves(pm_context* ctx, uint32_t* waves) {
+    if (!ctx || !waves) return PM_EINVAL;
+    *waves = ctx->batch_crt_waves;
+    return PM_OK;
+}
+
+int pm_set_conversion_slices(pm_context* ctx, uint32_t n, uint32_t slices) {
+    if (!ctx) return PM_EINVAL;
+    uint32_t logn = 0;
+    const int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    if (slices != 0 && slices != 1 && slices != 2 && slices != 4)
+        return pm_fail(ctx, PM_EINVAL, "conversion slices: 1, 2 or 4 (0: chosen at launch)");
+    ctx->conv_slices[logn] = slices;
+    return PM_OK;
+}
+
+int pm_get_conversion_slices(pm_context* ctx, uint32_t n, uint32_t* slices) {
+    if (!ctx || !slices) return PM_EINVAL;
+    uint32_t logn = 0;
+    const int st = pm_check_n(ctx, n, &logn);
+    if (st != PM_OK) return st;
+    *slices = ctx->conv_slices[logn];
+    return PM_OK;
+}
+
 int pm_get_batch_tiles_per_block(pm_context* ctx, uint32_t tile_n, uint32_t* tiles) {
     if (!ctx || !tiles || tile_n < 32 || tile_n > 32768 || (tile_n & (tile_n - 1))) return PM_EINVAL;
     *tiles = ctx->batch_tiles[__builtin_ctz(tile_n)];
@@ -818,7 +859,7 @@ int pm_get_batch_outer(pm_context* ctx, uint32_t n, uint32_t* outer) {
     uint32_t logn = 0;
     int st = pm_check_n(ctx, n, &logn);
     if (st != PM_OK) return st;
-    *outer = ctx->batch_outer[logn];
+    *outer = ctx->batch_outer[ctx->backend][logn];
     return PM_OK;
 }
 

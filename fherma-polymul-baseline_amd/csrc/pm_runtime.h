@@ -39,10 +39,17 @@ struct pm_context {
     // outer radix-2 stages run as separate passes, per log2 N: plain path,
     // and batches / served launches
     uint32_t outer[pm::kMaxLogN + 1] = {};
-    uint32_t batch_outer[pm::kMaxLogN + 1] = {};
+    // batches: one table per backend (index PM_BACKEND_*), since only the
+    // general backend runs its outer stages inside conversion and CRT
+    uint32_t batch_outer[2][pm::kMaxLogN + 1] = {};
     // tiles per block of each tile size class 2^tlog in batches and served
     // launches (blocks of tiles << (tlog - 5) threads)
     uint32_t batch_tiles[16] = {};
+    // waves per block of the batches' MFMA CRT (2, 4 or 8)
+    uint32_t batch_crt_waves = pmd::kCrtMfmaWaves;
+    // plain path: slices of the primes in the conversion, per log2 N (0:
+    // chosen at launch)
+    uint32_t conv_slices[pm::kMaxLogN + 1] = {};
     // host-buffer path: page-locked staging (inputs a|b and the output) with
     // its device views, output chunk events, and the product between submit
     // and collect
@@ -82,7 +89,7 @@ uint32_t pm_rows(const pm_context* ctx);
 hipError_t pm_workspace_acquire(pm_context* ctx, hipStream_t stream);
 hipError_t pm_workspace_release(pm_context* ctx, hipStream_t stream);
 uint32_t pm_default_outer(uint32_t logn);
-uint32_t pm_default_batch_outer(uint32_t logn);
+uint32_t pm_default_batch_outer(uint32_t logn, bool quartic = false);
 // tiles per block that fill 1024 threads: 2^(15 - tlog)
 uint32_t pm_default_batch_tiles(uint32_t tlog);
 // whether `outer` passes leave a tile of size 2^lo..2^15 for products of 2^logn

@@ -129,6 +129,7 @@ struct Launch {
 struct pm_server {
     pm_context* ctx = nullptr;
     pmd::Tables tables{};  // the context's backend when the server was created
+    uint32_t crt_waves = 0;  // waves per block of the MFMA CRT (the context's at creation)
     uint32_t max_batch = 0, max_inflight = 0;
     size_t max_items = 0, max_units = 0;
     uint64_t budget = 0;
@@ -168,7 +169,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
         uint8_t* base = arena_base + r->offset;
         pmd::Job& job = L.h_jobs[j];
         job.logn = r->logn;
-        job.outer = ctx->batch_outer[r->logn];
+        job.outer = ctx->batch_outer[tables.quartic ? PM_BACKEND_QUARTIC : PM_BACKEND_GENERAL][r->logn];
         job.fused = !tables.quartic && pmd::outer_fusable(job.logn, job.outer) ? 1u : 0u;
         job.a = reinterpret_cast<const uint32_t*>(base);
         job.b = job.a + n * kLimbs;
@@ -222,7 +223,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
     ba.items = L.d_items;
     ba.units = L.d_units;
     ba.n_items = uint32_t(items.size());
-    if (e == hipSuccess) e = pmd::launch_phased(tables, ba, ranges, false, false, L.stream);
+    if (e == hipSuccess) e = pmd::launch_phased(tables, ba, ranges, false, false, crt_waves, L.stream);
     // results into page-locked caller buffers go back behind the kernels on
     // the same stream; other buffers are copied on completion
     uint32_t n_out = 0;
@@ -352,6 +353,7 @@ int pm_server_create(pm_context* ctx, uint64_t hbm_budget_bytes, uint32_t max_ba
     if (!s) return pm_fail(ctx, PM_ENOMEM, "host allocation failed");
     s->ctx = ctx;
     s->tables = pm_tables(ctx);
+    s->crt_waves = ctx->batch_crt_waves;
     s->budget = hbm_budget_bytes;
     s->max_batch = max_batch;
     s->max_inflight = max_inflight;

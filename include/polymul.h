@@ -143,17 +143,39 @@ int pm_get_backend(const pm_context* ctx, uint32_t* backend, uint32_t* rows);
 int pm_synchronize(pm_context* ctx);
 /* Plain-path schedule (no counterpart in the reference): the number of outer
  * radix-2 stages (0..2) run as separate passes before/after the fused tiles
- * for products of size n; the default is max(0, log2 n - 15).  More outer
- * stages give more, smaller tiles -- more blocks for a single product.  Set by
- * the autotuner in the Python mirror (`autotune_plain`). */
+ * for products of size n; the default is 0 up to N = 2^13, 1 at 2^14 and 2
+ * above.  More outer stages give more, smaller tiles -- more blocks for a
+ * single product.  Set by the autotuner in the Python mirror
+ * (`autotune_plain`). */
 int pm_set_outer(pm_context* ctx, uint32_t n, uint32_t outer);
 int pm_get_outer(pm_context* ctx, uint32_t n, uint32_t* outer);
-/* Batch schedule (no counterpart in the reference): outer passes (0..2,
- * leaving a tile of 2^5..2^15) for products of size n in batches and served
- * launches created afterwards; default max(0, log2 n - 15) (the largest
- * tile).  Set by `autotune_batch` in the Python mirror. */
+/* Batch schedule (no counterpart in the reference): outer radix-2 stages
+ * (0..2, leaving a tile of 2^5..2^15) for products of size n in batches and
+ * served launches created afterwards, kept per backend (set/get act on the
+ * context's current one).  The general backend runs one or two outer stages
+ * inside its conversion and CRT; its default leaves a 2^13 tile (0 stages up
+ * to 2^13, 1 at 2^14, 2 above).  The quartic backend runs them as passes of
+ * their own; its default is max(0, log2 n - 15), the largest tile.  Set by
+ * `autotune_batch` in the Python mirror. */
 int pm_set_batch_outer(pm_context* ctx, uint32_t n, uint32_t outer);
 int pm_get_batch_outer(pm_context* ctx, uint32_t n, uint32_t* outer);
+/* Batch schedule (no counterpart in the reference, whose CRT launches fixed
+ * 128-thread blocks, quartic/solve.cu:1280,1299): waves per block of the
+ * MFMA CRT in batches and served launches created afterwards -- 8 (the
+ * default; 0 selects it) fill a compute unit with one block, 4 and 2 spread
+ * a small batch over more compute units.  PM_EINVAL for other values.  Set by
+ * `autotune_batch` in the Python mirror. */
+int pm_set_batch_crt_waves(pm_context* ctx, uint32_t waves);
+int pm_get_batch_crt_waves(pm_context* ctx, uint32_t* waves);
+/* Plain-path schedule (no counterpart in the reference; the role of its
+ * primes-per-conversion-block constant, quartic/solve.cu:325): the slices of
+ * the primes that the conversion of a product of size n spreads over
+ * separate blocks (1, 2 or 4; 0, the default, chooses at launch: up to 4
+ * while that leaves fewer than about 4 waves per compute unit).  The
+ * host-buffer path (pm_multiply, pm_submit) converts as it ingests and does
+ * not use it.  Set by `autotune_plain` in the Python mirror. */
+int pm_set_conversion_slices(pm_context* ctx, uint32_t n, uint32_t slices);
+int pm_get_conversion_slices(pm_context* ctx, uint32_t n, uint32_t* slices);
 /* Batch schedule (no counterpart in the reference; the role of the
  * reference's compile-time tile and primes-per-block knobs,
  * quartic/host_setup.h:11-12, quartic/solve.cu:325): tiles of size tile_n

@@ -141,8 +141,31 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
                     assert sha(batch.download(i)) == case["sha256"]["c"], (n, outer, tiles, i)
                 batch.close()
             ctx.set_batch_tiles_per_block(tile_n, 0)
+        # waves per block of the MFMA CRT, on the default tiles
+        ctx.set_batch_outer(n, outers[-1])
+        for waves in (2, 4, 8):
+            say("batch CRT waves", n, waves)
+            ctx.set_batch_crt_waves(waves)
+            assert ctx.get_batch_crt_waves() == waves
+            batch = pm.Batch(ctx, [n] * 3 + [1024])
+            for i in range(3):
+                batch.upload(i, a, b)
+            batch.upload(3, a[:1024], b[:1024])
+            batch.run()
+            batch.wait()
+            for i in range(3):
+                assert sha(batch.download(i)) == case["sha256"]["c"], (n, waves, i)
+            small = batch.download(3)
+            batch.close()
+            if waves == 2:
+                small_want = small
+            assert (small == small_want).all(), (n, waves)
+        ctx.set_batch_crt_waves(0)
+        assert ctx.get_batch_crt_waves() == 8
         ctx.close()
     ctx = pm.Context(workload.modulus(1), 1 << 16)
+    with pytest.raises(pm.PolymulError):
+        ctx.set_batch_crt_waves(16)
     with pytest.raises(pm.PolymulError):
         ctx.set_batch_outer(32, 1)  # would leave a tile of 2^4
     with pytest.raises(pm.PolymulError):
@@ -151,6 +174,8 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
     say("batch autotune")
     res = pm.autotune_batch(ctx, [4096, 32768], count=4, reps=2, path=str(path))
     tiles = res.pop("tiles_per_block")
+    waves = res.pop("crt_waves")
+    assert waves["waves"] == ctx.get_batch_crt_waves() and set(waves["ms"]) == {2, 4, 8}
     assert set(res) == {4096, 32768}
     for n, r in res.items():
         assert r["outer"] == ctx.get_batch_outer(n) and r["outer"] in r["ms"]
@@ -159,7 +184,9 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
         assert r["tiles"] == ctx.get_batch_tiles_per_block(tile_n) and r["tiles"] in r["ms"]
     ctx.set_batch_outer(32768, 2 if res[32768]["outer"] != 2 else 0)
     ctx.set_batch_tiles_per_block(8192, 4 if tiles[8192]["tiles"] != 4 else 1)
+    ctx.set_batch_crt_waves(2 if waves["waves"] != 2 else 4)
     applied = ctx.load_schedule(str(path))
+    assert applied["batch_crt_waves"] == waves["waves"] == ctx.get_batch_crt_waves()
     assert applied["batch_outer"][32768] == res[32768]["outer"] == ctx.get_batch_outer(32768)
     assert applied["batch_tiles_per_block"][8192] == tiles[8192]["tiles"] == ctx.get_batch_tiles_per_block(8192)
     ctx.close()

@@ -302,8 +302,9 @@ def test_fherma_run_input_size():
 
 
 def test_plain_schedule_and_autotune(pm, workload, tmp_path):
-    """Every plain-path schedule (outer passes 0..2) gives the same product;
-    the autotuner keeps the fastest per size and its JSON loads back."""
+    """Every plain-path schedule (outer passes 0..2, and on the device path
+    conversion slices 1, 2, 4) gives the same product; the autotuner keeps
+    the fastest per size and its JSON loads back."""
     q = workload.modulus(23)
     ctx = pm.Context(q, 1 << 15)
     say("schedules")
@@ -316,6 +317,16 @@ def test_plain_schedule_and_autotune(pm, workload, tmp_path):
             c = ctx.multiply(a, b)
             want = c if want is None else want
             assert (c == want).all(), (n, outer)
+            da, db = to_dev(torch, a), to_dev(torch, b)
+            dc = torch.empty_like(da)
+            for sl in (1, 2, 4, 0):
+                ctx.set_conversion_slices(n, sl)
+                assert ctx.get_conversion_slices(n) == sl
+                ctx.multiply_device(n, da, db, dc)
+                ctx.synchronize()
+                assert (dc.cpu().numpy().view(np.uint32).reshape(n, 28) == want).all(), (n, outer, sl)
+    with pytest.raises(pm.PolymulError):
+        ctx.set_conversion_slices(4096, 3)
     with pytest.raises(pm.PolymulError):
         ctx.set_outer(32768, 3)
     with pytest.raises(pm.PolymulError):
@@ -325,9 +336,12 @@ def test_plain_schedule_and_autotune(pm, workload, tmp_path):
     assert set(res) == {1024, 32768}
     for n, r in res.items():
         assert r["outer"] == ctx.get_outer(n) and r["outer"] in r["ms"]
+        assert r["slices"] == ctx.get_conversion_slices(n) and set(r["slices_ms"]) == {1, 2, 4}
     ctx.set_outer(32768, 0 if res[32768]["outer"] else 1)
+    ctx.set_conversion_slices(32768, 0)
     applied = ctx.load_schedule(str(path))
     assert applied["outer"][32768] == res[32768]["outer"] == ctx.get_outer(32768)
+    assert applied["conversion_slices"][32768] == res[32768]["slices"] == ctx.get_conversion_slices(32768)
     ctx.close()
 
 

@@ -124,6 +124,23 @@ def test_quartic_batch_matches_general(pm, workload):
             batch.close()
         for i, n in enumerate(ns):
             assert (outs["quartic"][i] == outs["general"][i]).all(), (delta, n)
+        # the quartic schedule kept apart from the general one: 2^13 tiles
+        # and two outer passes at 2^15 on the quartic backend only
+        ctx.set_backend("quartic")
+        ctx.set_batch_outer(32768, 2)
+        assert ctx.get_batch_outer(32768) == 2
+        ctx.set_backend("general")
+        assert ctx.get_batch_outer(32768) == 2  # the general default, a 2^13 tile too
+        ctx.set_batch_outer(32768, 1)
+        ctx.set_backend("quartic")
+        assert ctx.get_batch_outer(32768) == 2
+        batch = pm.Batch(ctx, ns)
+        for i, (a, b) in enumerate(ops):
+            batch.upload(i, a, b)
+        batch.run()
+        batch.wait()
+        assert all((batch.download(i) == outs["general"][i]).all() for i in range(len(ns))), "quartic outer 2"
+        batch.close()
         ctx.close()
 
 
