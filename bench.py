@@ -364,7 +364,7 @@ def extra_workloads(args):
     import copy
     t0 = time.perf_counter()
     a = copy.copy(args)
-    a.count, a.steps, a.warmup = 0, 5, 1
+    a.count, a.steps, a.warmup = 0, 15, 3
     u = run_uneven(a)
     print(f"[bench] uneven: {u['batched']['phased']['products_per_s']} products/s", file=sys.stderr, flush=True)
     lg = run_large(a)
@@ -380,7 +380,9 @@ def extra_workloads(args):
             "plain_one_at_a_time_products_per_s": u["plain_one_at_a_time_products_per_s"],
             "batched_over_plain": round(ph["products_per_s"] / u["plain_one_at_a_time_products_per_s"], 3),
             "autotuned_batch_outer": u["autotuned_batch_outer"],
-            "autotuned_tiles_per_block": u["autotuned_tiles_per_block"], "bit_exact": u["bit_exact_vs_plain"]},
+            "autotuned_tiles_per_block": u["autotuned_tiles_per_block"],
+            "autotuned_crt_waves": u["autotuned_crt_waves"], "autotuned_parts": u["autotuned_parts"],
+            "bit_exact": u["bit_exact_vs_plain"]},
         "large_n": {
             "workload": lg["workload"], "k_primes": lg["k_primes"],
             "mfma_conversion": {"products_per_s": m["products_per_s"], "conversion_ms": m["phases_ms"]["conversion"],
@@ -484,13 +486,15 @@ def run_uneven(args):
     sptr = stream.cuda_stream
     inputs = make_inputs(pm, q, ns, args.seed)
     print(f"[bench] uneven inputs ready: {len(ns)} products", file=sys.stderr, flush=True)
-    # per-N batch schedule (outer passes, hence tile size and tiles per block)
-    # from the autotuner, on small batches of each size
-    tuned = pm.autotune_batch(ctx, sorted(set(ns)), count=8, reps=3, stream=stream)
+    # batch schedule from the autotuner: per N (outer passes, hence tile size;
+    # tiles per block) on batches of each size that fill the device, then CRT
+    # waves per block, then the graph's parts on this mix
+    tuned = pm.autotune_batch(ctx, ns, count=8, reps=3, stream=stream)
     tuned_tiles = tuned.pop("tiles_per_block")
     tuned_waves = tuned.pop("crt_waves")
-    print(f"[bench] batch schedule: {tuned} tiles per block {tuned_tiles} CRT waves {tuned_waves}",
-          file=sys.stderr, flush=True)
+    tuned_parts = tuned.pop("parts")
+    print(f"[bench] batch schedule: {tuned} tiles per block {tuned_tiles} CRT waves {tuned_waves} "
+          f"parts {tuned_parts}", file=sys.stderr, flush=True)
     results = {}
     outputs = {}
     for mode in ("phased",):
@@ -539,12 +543,14 @@ def run_uneven(args):
     line = {"workload": "batched uneven N (2^12..2^16 log-uniform), one graph launch",
             "products": len(ns), "sizes": {str(n): c for n, c in sorted(counts.items())}, "k_primes": k,
             "autotuned_batch_outer": {str(n): r["outer"] for n, r in tuned.items()},
-            "autotune_ms_per_launch_of_8": {str(n): {str(u): round(t, 4) for u, t in r["ms"].items()}
+            "autotune_outer_ms_per_launch": {str(n): {str(u): round(t, 4) for u, t in r["ms"].items()}
                                             for n, r in tuned.items()},
             "batched": results,
             "autotuned_tiles_per_block": {str(t): r["tiles"] for t, r in tuned_tiles.items()},
             "autotuned_crt_waves": tuned_waves["waves"],
             "autotune_crt_waves_ms": {str(w): round(t, 4) for w, t in tuned_waves["ms"].items()},
+            "autotuned_parts": tuned_parts["parts"],
+            "autotune_parts_ms": {str(p_): round(t, 4) for p_, t in tuned_parts["ms"].items()},
             "plain_one_at_a_time_products_per_s": round(len(ns) / (plain_total_ms * 1e-3), 1),
             "bit_exact_vs_plain": True}
     ctx.close()

@@ -90,6 +90,8 @@ def library():
     lib.pm_set_batch_tiles_per_block.argtypes = [P, u32, u32]
     lib.pm_get_batch_tiles_per_block.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_set_batch_crt_waves.argtypes = [P, u32]
+    lib.pm_set_batch_parts.argtypes = [P, u32]
+    lib.pm_get_batch_parts.argtypes = [P, ctypes.POINTER(u32)]
     lib.pm_get_batch_crt_waves.argtypes = [P, ctypes.POINTER(u32)]
     lib.pm_set_conversion_slices.argtypes = [P, u32, u32]
     lib.pm_get_conversion_slices.argtypes = [P, u32, ctypes.POINTER(u32)]
@@ -127,7 +129,7 @@ EXPORTED = ["pm_plan_rns", "pm_plan_quartic", "pm_create", "pm_destroy", "pm_las
             "pm_inject_io_stall", "pm_place_near_device", "pm_multiply_device", "pm_multiply_device_noalloc",
             "pm_synchronize", "pm_set_backend", "pm_get_backend", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
             "pm_set_batch_tiles_per_block", "pm_get_batch_tiles_per_block", "pm_set_batch_crt_waves",
-            "pm_get_batch_crt_waves", "pm_set_conversion_slices", "pm_get_conversion_slices",
+            "pm_get_batch_crt_waves", "pm_set_batch_parts", "pm_get_batch_parts", "pm_set_conversion_slices", "pm_get_conversion_slices",
             "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
             "pm_batch_create", "pm_batch_destroy", "pm_batch_io", "pm_batch_upload", "pm_batch_download",
@@ -370,6 +372,17 @@ class Context:
         self._check("pm_get_batch_crt_waves", self._lib.pm_get_batch_crt_waves(self._h, ctypes.byref(v)))
         return v.value
 
+    def set_batch_parts(self, parts):
+        """Batch schedule: staggered parts (1-4; 0 = chosen per batch: 3
+        where a product runs 2^15 tiles, else 1) of batches created
+        afterwards."""
+        self._check("pm_set_batch_parts", self._lib.pm_set_batch_parts(self._h, int(parts)))
+
+    def get_batch_parts(self):
+        v = ctypes.c_uint32()
+        self._check("pm_get_batch_parts", self._lib.pm_get_batch_parts(self._h, ctypes.byref(v)))
+        return v.value
+
     def set_conversion_slices(self, n, slices):
         """Plain-path schedule: slices of the primes the conversion of a
         product of size n spreads over (1, 2 or 4; 0 = chosen at launch)."""
@@ -390,8 +403,8 @@ class Context:
         `autotune_backend` (sizes this context cannot hold, and a quartic
         backend its modulus does not allow, are skipped); returns {"outer":
         {n: u}, "batch_outer": {n: u}, "batch_tiles_per_block": {tile_n:
-        tiles}, "conversion_slices": {n: s}} and "backend" and
-        "batch_crt_waves" when they were set, as applied."""
+        tiles}, "conversion_slices": {n: s}} and "backend",
+        "batch_crt_waves" and "batch_parts" when they were set, as applied."""
         import json
         with open(path) as f:
             doc = json.load(f)
@@ -406,6 +419,9 @@ class Context:
         if "batch_crt_waves" in doc:
             self.set_batch_crt_waves(doc["batch_crt_waves"])
             applied["batch_crt_waves"] = int(doc["batch_crt_waves"])
+        if "batch_parts" in doc:
+            self.set_batch_parts(doc["batch_parts"])
+            applied["batch_parts"] = int(doc["batch_parts"])
         for key, setter in (("outer", self.set_outer), ("batch_outer", self.set_batch_outer),
                             ("batch_tiles_per_block", self.set_batch_tiles_per_block),
                             ("conversion_slices", self.set_conversion_slices)):
@@ -502,10 +518,10 @@ class Batch:
         return out
 
     def info(self):
-        items, blocks, nbytes = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
-        self._ctx._check("pm_batch_info", self._lib.pm_batch_info(self._h, ctypes.byref(items), ctypes.byref(blocks),
+        items, parts, nbytes = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        self._ctx._check("pm_batch_info", self._lib.pm_batch_info(self._h, ctypes.byref(items), ctypes.byref(parts),
                                                                    ctypes.byref(nbytes)))
-        return {"items": items.value, "blocks": blocks.value, "device_bytes": nbytes.value}
+        return {"items": items.value, "parts": parts.value, "device_bytes": nbytes.value}
 
 
 class Server:
@@ -670,31 +686,49 @@ def tiles_per_block_choices(tile_n):
     return out
 
 
-def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
-    """Per-N batch schedule, the role of the reference's compile-time tile,
-    radix and primes-per-block knobs (quartic/host_setup.h:11-12,
-    quartic/solve.cu:325,576), chosen on the device:
+def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None, coefficients=1 << 20):
+    """Batch schedule, the role of the reference's compile-time tile, radix
+    and primes-per-block knobs (quartic/host_setup.h:11-12,
+    quartic/solve.cu:325,576), chosen on the device.  Each trial is a graph
+    launch of a batch of one size n holding max(count, coefficients / n)
+    products, at most 512 (a batch too small to fill the device would favour
+    whatever spreads a few products widest):
 
-    1. tiles per block of each tile size 2^t the sizes can use: a graph
-       launch of `count` products of N = 2^t (no outer stages) is timed with
-       every valid tiles-per-block count and the fastest kept;
+    1. tiles per block of each tile size 2^t the sizes can use, on products
+       of N = 2^t (no outer stages);
     2. outer stages (0..2: the tile size, and radix-2 or radix-4 outer
-       stages in the conversion and the CRT) for each N, timed the same way
-       with the tiles per block of step 1;
-    3. waves per block of the MFMA CRT (2, 4, 8): the sum over the sizes of
-       a launch of `count` products each, with the schedule of steps 1-2.
+       stages in the conversion and the CRT) for each N, with the tiles per
+       block of step 1;
+    3. waves per block of the MFMA CRT (2, 4, 8): the sum over the sizes;
+    4. staggered parts of the graph (1-4): the batch of `ns` as given (the
+       mix the caller runs; each part holds at least 4 products), compared
+       with the per-batch default (0).
 
     All are kept in the context (for batches and served launches created
     afterwards, on its current backend).  Returns {n: {"outer": u, "ms": {u:
     median ms per launch}}, "tiles_per_block": {tile_n: {"tiles": t, "ms":
-    {t: ms}}}, "crt_waves": {"waves": w, "ms": {w: ms}}}; with `path` the
-    choice is also written to the schedule file `load_schedule` reads
-    ("batch_outer", "batch_tiles_per_block" and "batch_crt_waves")."""
+    {t: ms}}}, "crt_waves": {"waves": w, "ms": {w: ms}}, "parts": {"parts":
+    p, "ms": {p: ms}}}; with `path` the choice is also written to the
+    schedule file `load_schedule` reads ("batch_outer",
+    "batch_tiles_per_block", "batch_crt_waves", "batch_parts")."""
+    import json
     import torch
     from . import workload as wl
     s = stream if stream is not None else torch.cuda.Stream()
-    sizes = sorted(set(int(x) for x in ns))
+    ns = [int(x) for x in ns]
+    sizes = sorted(set(ns))
     max_n = ctx.info()["max_n"]
+    polys = {}
+
+    def operands(n):
+        if n not in polys:
+            polys[n] = (wl.poly(n, 0, n, ctx.q), wl.poly(n, 1, n, ctx.q))
+        return polys[n]
+
+    def trial(n):
+        m = min(512, max(count, coefficients // n))
+        return _time_batch(ctx, [n] * m, [operands(n)] * m, reps, s)
+
     tiles = {}
     for n in sizes:
         lg = n.bit_length() - 1
@@ -706,13 +740,12 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
         choices = tiles_per_block_choices(tile_n)
         if tile_n > max_n or len(choices) < 2:
             continue
-        a, b = wl.poly(tile_n, 0, tile_n, ctx.q), wl.poly(tile_n, 1, tile_n, ctx.q)
         saved = ctx.get_batch_outer(tile_n)
         ctx.set_batch_outer(tile_n, 0)
         times = {}
         for t in choices:
             ctx.set_batch_tiles_per_block(tile_n, t)
-            times[t] = _time_batch(ctx, [tile_n] * count, [(a, b)] * count, reps, s)
+            times[t] = trial(tile_n)
         best = min(times, key=times.get)
         ctx.set_batch_tiles_per_block(tile_n, best)
         ctx.set_batch_outer(tile_n, saved)
@@ -720,35 +753,41 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None):
     result = {}
     for n in sizes:
         logn = n.bit_length() - 1
-        a = wl.poly(n, 0, n, ctx.q)
-        b = wl.poly(n, 1, n, ctx.q)
         times = {}
         for outer in (0, 1, 2):
             if outer > logn or logn - outer > 15 or logn - outer < 5:
                 continue
             ctx.set_batch_outer(n, outer)
-            times[outer] = _time_batch(ctx, [n] * count, [(a, b)] * count, reps, s)
+            times[outer] = trial(n)
         best = min(times, key=times.get)
         ctx.set_batch_outer(n, best)
         result[n] = {"outer": best, "ms": times}
     wave_ms = {}
     for w in (2, 4, 8):
         ctx.set_batch_crt_waves(w)
-        wave_ms[w] = sum(_time_batch(ctx, [n] * count, [(wl.poly(n, 0, n, ctx.q), wl.poly(n, 1, n, ctx.q))] * count,
-                                     reps, s) for n in sizes)
+        wave_ms[w] = sum(trial(n) for n in sizes)
     best_w = min(wave_ms, key=wave_ms.get)
     ctx.set_batch_crt_waves(best_w)
+    part_ms = {}
+    for p in (0, 1, 2, 3, 4):
+        if p and len(ns) < 4 * p:
+            continue
+        ctx.set_batch_parts(p)
+        part_ms[p] = _time_batch(ctx, ns, [operands(n) for n in ns], reps, s)
+    best_p = min(part_ms, key=part_ms.get)
+    ctx.set_batch_parts(best_p)
     if path is not None:
-        import json
         _save_schedule(path, "batch_outer", {n: r["outer"] for n, r in result.items()})
         _save_schedule(path, "batch_tiles_per_block", {t: r["tiles"] for t, r in tile_result.items()})
         with open(path) as f:
             doc = json.load(f)
         doc["batch_crt_waves"] = best_w
+        doc["batch_parts"] = best_p
         with open(path, "w") as f:
             json.dump(doc, f, indent=1)
     result["tiles_per_block"] = tile_result
     result["crt_waves"] = {"waves": best_w, "ms": wave_ms}
+    result["parts"] = {"parts": best_p, "ms": part_ms}
     return result
 
 

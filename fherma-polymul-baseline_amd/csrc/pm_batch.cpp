@@ -30,14 +30,29 @@ namespace {
 
 uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-// A batch of at least kSplitMin products runs in kSplitParts staggered parts
-// of 40 %, 35 % and 25 % of its products: the last part's tiles and CRT run
-// alone at the end, so it is the smallest.  Measured at 128 x N = 2^15 over
-// one part: 2 equal parts +3.0 %, 3 equal parts +3.5-4.8 %, 4 equal parts
-// +2.8 %; 40/35/25 a further +0.6 %, 45/35/20 -0.9 %.
-constexpr uint32_t kSplitMin = 16;
-constexpr uint32_t kSplitParts = 3;
-constexpr double kPartCut[kSplitParts + 1] = {0.0, 0.40, 0.75, 1.0};  // cumulative fractions
+// A batch may run as 2-4 staggered parts on as many graph branches, each
+// starting when the one before has converted its residues, so a part's
+// HBM-bound conversion runs beside the previous part's tile blocks; later
+// parts are smaller, since the last one's tiles and CRT run alone at the
+// end.  Each part holds at least kPartMinProducts products.  Cumulative
+// fractions of the products per part count:
+constexpr uint32_t kMaxParts = 4;
+constexpr uint32_t kPartMinProducts = 4;
+constexpr double kPartCut[kMaxParts + 1][kMaxParts + 1] = {
+    {0.0, 1.0}, {0.0, 1.0}, {0.0, 0.55, 1.0}, {0.0, 0.40, 0.75, 1.0}, {0.0, 0.30, 0.57, 0.80, 1.0}};
+
+// Parts when the context leaves the choice open: 3 where some product runs
+// 2^15 tiles (a 1024-thread block fills a CU, and only blocks of another
+// part's conversion or CRT fill the CUs its tail leaves), else 1 (smaller
+// tiles share a CU four to a CU already).  Measured in A/B runs of one
+// build: 32 x N = 2^17 (2^15 tiles) 3 parts 11.97-12.05k against
+// 11.50-11.59k products/s in one; 128 x N = 2^15 (2^13 tiles) one part
+// 53.8-54.2k against 53.2k; 256 uneven products of 2^12..2^16 alike.
+uint32_t auto_parts(const std::vector<pmd::Job>& jobs) {
+    for (const pmd::Job& j : jobs)
+        if (j.logn - j.outer == 15) return 3;
+    return 1;
+}
 
 }  // namespace
 
@@ -120,24 +135,22 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     if (e != hipSuccess) return pm_hip(ctx, e, "upload batch tables");
     b->jobs = jobs;
 
-    // the whole batch as one work list (per-phase timing; graphs of small batches)
+    // the whole batch as one work list (per-phase timing; graphs of one part)
     int st = upload_worklist(ctx, b, d_jobs, 0, count, b->args, b->ranges);
     if (st != PM_OK) return st;
     b->n_items = b->args.n_items;
 
-    // capture the phase launches once; runs replay the graph.  From
-    // kSplitMin products on (each part then holds at least 4), the graph holds
-    // kSplitParts parts of the batch on
-    // as many branches, each starting when the one before has converted its
-    // residues: a part's HBM-bound conversion then runs beside the previous
-    // part's VALU-bound tile blocks, and its tiles beside that part's CRT.
-    const uint32_t parts = count >= kSplitMin ? kSplitParts : 1u;
+    // capture the phase launches once; runs replay the graph, in one part or
+    // in staggered parts on as many branches (kPartCut)
+    uint32_t parts = ctx->batch_parts ? ctx->batch_parts : auto_parts(b->jobs);
+    while (parts > 1 && count < parts * kPartMinProducts) --parts;
+    b->parts = parts;
     std::vector<pmd::BatchArgs> part_args(parts);
     std::vector<pmd::PhaseRanges> part_ranges(parts);
     if (parts > 1) {
         for (uint32_t q = 0; q < parts; ++q) {
-            const uint32_t lo = uint32_t(count * kPartCut[q]);
-            const uint32_t hi = q + 1 == parts ? count : uint32_t(count * kPartCut[q + 1]);
+            const uint32_t lo = uint32_t(count * kPartCut[parts][q]);
+            const uint32_t hi = q + 1 == parts ? count : uint32_t(count * kPartCut[parts][q + 1]);
             st = upload_worklist(ctx, b, d_jobs, lo, hi - lo, part_args[q], part_ranges[q]);
             if (st != PM_OK) return st;
         }
@@ -399,10 +412,10 @@ int pm_batch_phase_times(pm_batch* b, void* hip_stream, uint32_t max, uint32_t* 
     return pm_hip(b->ctx, e, "time batch phases");
 }
 
-int pm_batch_info(const pm_batch* b, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes) {
+int pm_batch_info(const pm_batch* b, uint32_t* items, uint32_t* parts, uint64_t* device_bytes) {
     if (!b) return PM_EINVAL;
     if (items) *items = b->n_items;
-    if (blocks) *blocks = 0;  // one launch per phase, each sized to its items
+    if (parts) *parts = b->parts;
     if (device_bytes) *device_bytes = b->device_bytes;
     return PM_OK;
 }

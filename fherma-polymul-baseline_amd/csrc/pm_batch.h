@@ -31,6 +31,7 @@ struct pm_batch {
     bool scalar_crt = false;        // phased only: vector-ALU CRT sums instead of MFMA
     bool separate_outer = false;    // outer NTT stages as passes of their own, not inside conversion / CRT
     uint32_t crt_waves = 0;         // waves per block of the MFMA CRT (the context's at creation)
+    uint32_t parts = 1;             // staggered parts of the graph (1: one part)
     std::vector<void*> allocations;
     uint64_t device_bytes = 0;
     hipGraph_t graph = nullptr;

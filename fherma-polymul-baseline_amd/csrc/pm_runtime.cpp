@@ -822,6 +822,19 @@ int pm_set_batch_crt_waves(pm_context* ctx, uint32_t waves) {
     return PM_OK;
 }
 
+int pm_set_batch_parts(pm_context* ctx, uint32_t parts) {
+    if (!ctx) return PM_EINVAL;
+    if (parts > 4) return pm_fail(ctx, PM_EINVAL, "batch parts: 1 to 4, or 0 to choose per batch");
+    ctx->batch_parts = parts;
+    return PM_OK;
+}
+
+int pm_get_batch_parts(pm_context* ctx, uint32_t* parts) {
+    if (!ctx || !parts) return PM_EINVAL;
+    *parts = ctx->batch_parts;
+    return PM_OK;
+}
+
 int pm_get_batch_crt_waves(pm_context* ctx, uint32_t* waves) {
     if (!ctx || !waves) return PM_EINVAL;
     *waves = ctx->batch_crt_waves;

@@ -47,6 +47,8 @@ struct pm_context {
     uint32_t batch_tiles[16] = {};
     // waves per block of the batches' MFMA CRT (2, 4 or 8)
     uint32_t batch_crt_waves = pmd::kCrtMfmaWaves;
+    // staggered parts of a batch's graph (1-4; 0: chosen per batch)
+    uint32_t batch_parts = 0;
     // plain path: slices of the primes in the conversion, per log2 N (0:
     // chosen at launch)
     uint32_t conv_slices[pm::kMaxLogN + 1] = {};

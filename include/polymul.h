@@ -167,6 +167,15 @@ int pm_get_batch_outer(pm_context* ctx, uint32_t n, uint32_t* outer);
  * `autotune_batch` in the Python mirror. */
 int pm_set_batch_crt_waves(pm_context* ctx, uint32_t waves);
 int pm_get_batch_crt_waves(pm_context* ctx, uint32_t* waves);
+/* Batch schedule (no counterpart in the reference): the staggered parts
+ * (1-4) a batch created afterwards splits into, each part on a graph branch
+ * of its own that starts when the part before has converted its residues;
+ * a part holds at least 4 products.  0 (the default) chooses per batch: 3
+ * where some product runs tiles of 2^15 residues, else 1.  PM_EINVAL above
+ * 4.  pm_batch_info reports a batch's count.  Set by `autotune_batch` in
+ * the Python mirror. */
+int pm_set_batch_parts(pm_context* ctx, uint32_t parts);
+int pm_get_batch_parts(pm_context* ctx, uint32_t* parts);
 /* Plain-path schedule (no counterpart in the reference; the role of its
  * primes-per-conversion-block constant, quartic/solve.cu:325): the slices of
  * the primes that the conversion of a product of size n spreads over
@@ -239,7 +248,9 @@ int pm_batch_download(pm_batch* batch, uint32_t i, uint32_t* c);
  * replay, on whichever stream it was enqueued. */
 int pm_batch_run(pm_batch* batch, void* hip_stream);
 int pm_batch_wait(pm_batch* batch, void* hip_stream);
-int pm_batch_info(const pm_batch* batch, uint32_t* items, uint32_t* blocks, uint64_t* device_bytes);
+/* The batch's work-list items, the staggered parts of its graph and the
+ * device memory it holds (any pointer may be null). */
+int pm_batch_info(const pm_batch* batch, uint32_t* items, uint32_t* parts, uint64_t* device_bytes);
 /* Runs the batch's phases once as direct launches on a stream with HIP events
  * between them and returns each phase's kind (0 conversion, 1 outer forward,
  * 2 | tlog<<8 tile class, 3 outer inverse, 4 CRT) and milliseconds.  On entry

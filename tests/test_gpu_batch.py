@@ -172,10 +172,12 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
         ctx.set_batch_outer(65536, 0)  # tile of 2^16
     path = tmp_path / "schedule.json"
     say("batch autotune")
-    res = pm.autotune_batch(ctx, [4096, 32768], count=4, reps=2, path=str(path))
+    res = pm.autotune_batch(ctx, [4096, 32768] * 4, count=4, reps=2, path=str(path), coefficients=1 << 17)
     tiles = res.pop("tiles_per_block")
     waves = res.pop("crt_waves")
+    parts = res.pop("parts")
     assert waves["waves"] == ctx.get_batch_crt_waves() and set(waves["ms"]) == {2, 4, 8}
+    assert parts["parts"] == ctx.get_batch_parts() and set(parts["ms"]) == {0, 1, 2}  # 8 products: parts of >= 4
     assert set(res) == {4096, 32768}
     for n, r in res.items():
         assert r["outer"] == ctx.get_batch_outer(n) and r["outer"] in r["ms"]
@@ -185,33 +187,47 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
     ctx.set_batch_outer(32768, 2 if res[32768]["outer"] != 2 else 0)
     ctx.set_batch_tiles_per_block(8192, 4 if tiles[8192]["tiles"] != 4 else 1)
     ctx.set_batch_crt_waves(2 if waves["waves"] != 2 else 4)
+    ctx.set_batch_parts(4)
     applied = ctx.load_schedule(str(path))
     assert applied["batch_crt_waves"] == waves["waves"] == ctx.get_batch_crt_waves()
+    assert applied["batch_parts"] == parts["parts"] == ctx.get_batch_parts()
     assert applied["batch_outer"][32768] == res[32768]["outer"] == ctx.get_batch_outer(32768)
     assert applied["batch_tiles_per_block"][8192] == tiles[8192]["tiles"] == ctx.get_batch_tiles_per_block(8192)
     ctx.close()
 
 
 def test_batch_in_staggered_parts(pm, workload):
-    """A batch of 16 or more products runs as three staggered parts on three
-    graph branches: every product equals the same product on the plain path,
-    on two replays in a row (the parts' work lists and branches)."""
+    """A batch split into 1-4 staggered parts on as many graph branches:
+    every product equals the same product on the plain path, on two replays
+    in a row (the parts' work lists and branches); the per-batch default
+    splits where a product runs 2^15 tiles, and a part keeps 4 products."""
     q = workload.modulus(6)
-    ctx = pm.Context(q, 1 << 16)
+    ctx = pm.Context(q, 1 << 17)
     ns = [1024, 4096, 32768, 256, 8192, 65536, 2048, 16384] * 3
-    say("batch in parts", len(ns))
-    batch = pm.Batch(ctx, ns)
-    inputs = []
-    for i, n in enumerate(ns):
-        a, b = workload.family("dense-random-1", n, q, 300 + i)
-        batch.upload(i, a, b)
-        inputs.append((a, b))
-    for replay in range(2):
-        batch.run()
-        batch.wait()
-        for i, (n, (a, b)) in enumerate(zip(ns, inputs)):
-            assert (batch.download(i) == ctx.multiply(a, b)).all(), (replay, i, n)
-    batch.close()
+    inputs = [workload.family("dense-random-1", n, q, 300 + i) for i, n in enumerate(ns)]
+    want = [ctx.multiply(a, b) for a, b in inputs]
+    for parts in (0, 1, 2, 3, 4):
+        say("batch in parts", len(ns), parts)
+        ctx.set_batch_parts(parts)
+        batch = pm.Batch(ctx, ns)
+        assert batch.info()["parts"] == (parts or 1)  # no 2^15 tile here: 2^16 runs two outer stages
+        for i, (a, b) in enumerate(inputs):
+            batch.upload(i, a, b)
+        for replay in range(2):
+            batch.run()
+            batch.wait()
+            for i, n in enumerate(ns):
+                assert (batch.download(i) == want[i]).all(), (parts, replay, i, n)
+        batch.close()
+    ctx.set_batch_parts(0)
+    big = pm.Batch(ctx, [1 << 17] * 12)
+    assert big.info()["parts"] == 3
+    big.close()
+    small = pm.Batch(ctx, [1 << 17] * 8)
+    assert small.info()["parts"] == 2  # 3 parts would hold fewer than 4 products each
+    small.close()
+    with pytest.raises(pm.PolymulError):
+        ctx.set_batch_parts(5)
     ctx.close()
 
 
