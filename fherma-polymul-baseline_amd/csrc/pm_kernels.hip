@@ -401,15 +401,26 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
     const Tw s = tb.scale[logn_full * tb.k + pi];
     const __amdgpu_buffer_rsrc_t rd =
         __builtin_amdgcn_make_buffer_rsrc(wave_uniform(dst), short(0), int(G::T * 4u), int(kRawBufferWord3));
+    // A tile that is a sub-block of a larger transform (bm > 1: outer inverse
+    // stages follow, in the CRT or a pass of their own) stores its scaled
+    // values in [0,2p), as those Gentleman-Sande stages take them; they end in
+    // [0,p).  A whole transform stores canonical residues for the CRT.
+    const auto store = [&](auto canonical) {
 #pragma unroll
-    for (int r = 0; r < G::E; ++r) {
-        const uint32_t t = csub(shoup(a[r], s, p), p);  // [0,2p) -> [0,p)
-        if (!valid) continue;
-        if constexpr (kWaveTile)
-            __builtin_amdgcn_raw_buffer_store_b32(t, rd, x0 * 4u, G::reg_scatter(0, r) * 4u, kTileStreamPolicy);
-        else
-            dst[x0 + G::reg_scatter(0, r)] = t;
-    }
+        for (int r = 0; r < G::E; ++r) {
+            uint32_t t = shoup(a[r], s, p);  // [0,2p)
+            if constexpr (decltype(canonical)::value) t = csub(t, p);
+            if (!valid) continue;
+            if constexpr (kWaveTile)
+                __builtin_amdgcn_raw_buffer_store_b32(t, rd, x0 * 4u, G::reg_scatter(0, r) * 4u, kTileStreamPolicy);
+            else
+                dst[x0 + G::reg_scatter(0, r)] = t;
+        }
+    };
+    if (bm > 1u)
+        store(std::false_type{});
+    else
+        store(std::true_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -425,11 +436,13 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
 // corrections -- read with LDS instructions so that no global load is issued
 // between a group's prefetch and its use (the vector memory counter drains in
 // order).
+// fold: the digit-sum corrections of the recombination's two halves with
+// their offsets, {128 (C0 + 256 C1) + p, 128 (C2 + 256 C3) + 2p} mod 2^32.
 struct ConvLds {
     uint4 w[16 * 2 * 64];  // [quad][step][lane]
     uint32_t p[64];
     uint32_t sh16[64];
-    int4 corr[64];
+    uint2 fold[64];
     Tw otw[64][3];  // forward twiddles 1..3 of each prime: the fused outer stages
 };
 
@@ -442,7 +455,8 @@ __device__ __forceinline__ void conv_tables_to_lds(const Tables& tb, ConvLds& L,
         const uint32_t pi = i < tb.k ? i : 0u;
         L.p[i] = tb.primes[pi];
         L.sh16[i] = tb.conv_sh16[pi];
-        L.corr[i] = i < tb.quads * 4u ? reinterpret_cast<const int4*>(tb.conv_corr)[i] : make_int4(0, 0, 0, 0);
+        const int4 corr = i < tb.quads * 4u ? reinterpret_cast<const int4*>(tb.conv_corr)[i] : make_int4(0, 0, 0, 0);
+        L.fold[i] = make_uint2(uint32_t(corr.x + 256 * corr.y) + L.p[i], uint32_t(corr.z + 256 * corr.w) + 2u * L.p[i]);
         for (uint32_t j = 0; j < 3; ++j)  // twiddle 1 + j of the 2^max_logn table (every N shares its prefix)
             L.otw[i][j] = (1u + j) < (1u << tb.max_logn) ? tb.fwd[(size_t(pi) << tb.max_logn) + 1u + j] : Tw{0u, 0u};
     }
@@ -493,6 +507,27 @@ __device__ __forceinline__ void conv_load_host(const uint32_t* __restrict__ coef
     }
 }
 
+// Residue of one column tile from its two MFMA steps, not yet reduced.  With
+// the exact digit sums T_d = sum_j a_j * digit_d(256^j mod p) (accumulator
+// plus its correction), u = T0 + 256 T1 and w = T2 + 256 T3, the value
+//     v = u + p + ((w + 2p) * 2^16 mod p, one Shoup step in [0,2p))
+// is congruent to the coefficient mod p, lies in [0, 2^32), and v - 2p < 4p
+// when v >= 2p: the exact per-prime ranges of u and w over any 112 coefficient
+// bytes are checked when the tables are built (pm_params.cpp).  So v is a
+// valid multiplied (upper) operand of the forward NTT's first butterflies as
+// it is, and one conditional subtraction of 2p makes it a lower operand
+// (below 4p).  The corrections and offsets are folded into two constants
+// (ConvLds::fold): 8 VALU instructions per residue.
+__device__ __forceinline__ uint32_t conv_residue(const v4i& w0, const v4i& w1, const v4i& b0, const v4i& b1,
+                                                 uint2 fold, uint32_t p, Tw c16) {
+    v4i acc = v4i{0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w0, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w1, b1, acc, 0, 0, 0);
+    const uint32_t u = uint32_t(acc[0]) + (uint32_t(acc[1]) << 8);
+    const uint32_t w = uint32_t(acc[2]) + (uint32_t(acc[3]) << 8) + fold.y;
+    return u + fold.x + shoup(w, c16, p);
+}
+
 // Residues of the group for quads [quad_begin, quad_end) of primes; n is the
 // row stride of the residue array, coefficients at or past c_end are not
 // stored.  With outer = 1 or 2 (cstride = n/4) the lane's four residues are
@@ -523,23 +558,10 @@ __device__ __forceinline__ void conv_compute(const Tables& tb, const v4u (&v0)[4
         const uint32_t prime = quad * 4u + kg;
         const bool live = prime < tb.k;
         const uint32_t p = L.p[prime], p2 = 2u * p;
-        const int4 corr = L.corr[prime];
         const Tw c16{65536u, L.sh16[prime]};
         uint32_t r[4];
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-            v4i acc = v4i{0, 0, 0, 0};
-            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w0, b0[ct], acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w1, b1[ct], acc, 0, 0, 0);
-            // exact digit sums T_d = sum_j a_j * digit_d(256^j mod p)
-            const int32_t t0 = acc[0] + corr.x, t1 = acc[1] + corr.y;
-            const int32_t t2 = acc[2] + corr.z, t3 = acc[3] + corr.w;
-            const int32_t u = t0 + t1 * 256;  // |u| < 2^30
-            const int32_t w = t2 + t3 * 256;  // |w| < 2^30
-            uint32_t x = csub(shoup(uint32_t(w) + p2, c16, p), p);   // w * 2^16 mod p, [0,p)
-            uint32_t v = uint32_t(u) + p2 + x;                          // < 2^32, > 0
-            r[ct] = csub(csub(v, p2), p2);                              // [0,2p)
-        }
+        for (int ct = 0; ct < 4; ++ct) r[ct] = csub(conv_residue(w0, w1, b0[ct], b1[ct], L.fold[prime], p, c16), p2);
         if (outer >= 1u) {
             // bit log2(n)-1 pairs x with x + n/2 (twiddle 1); bit log2(n)-2
             // pairs x with x + n/4 in each half (twiddles 2 and 3): the
@@ -559,21 +581,6 @@ __device__ __forceinline__ void conv_compute(const Tables& tb, const v4u (&v0)[4
             if (live && c < c_end) res[size_t(prime) * n + c] = r[ct];
         }
     }
-}
-
-// Residue (in [0,2p)) of one column tile from its two MFMA steps: the exact
-// digit sums T_d = sum_j a_j * digit_d(256^j mod p) recombined as
-// (T0 + 256 T1) + 2^16 (T2 + 256 T3) with one Shoup step.
-__device__ __forceinline__ uint32_t conv_residue(const v4i& w0, const v4i& w1, const v4i& b0, const v4i& b1,
-                                                 const int4& corr, uint32_t p, uint32_t p2, Tw c16) {
-    v4i acc = v4i{0, 0, 0, 0};
-    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w0, b0, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w1, b1, acc, 0, 0, 0);
-    const int32_t u = (acc[0] + corr.x) + (acc[1] + corr.y) * 256;  // |u| < 2^30
-    const int32_t w = (acc[2] + corr.z) + (acc[3] + corr.w) * 256;  // |w| < 2^30
-    const uint32_t x = csub(shoup(uint32_t(w) + p2, c16, p), p);   // w * 2^16 mod p, [0,p)
-    const uint32_t v = uint32_t(u) + p2 + x;                       // < 2^32, > 0
-    return csub(csub(v, p2), p2);
 }
 
 // A slot of 128 coefficients as two halves A and B, each four column tiles of
@@ -607,22 +614,30 @@ __device__ __forceinline__ void conv_slot_compute(const Tables& tb, const v4u (&
         const v4i w1 = __builtin_bit_cast(v4i, L.w[(quad * 2 + 1) * 64 + lane]);
         const uint32_t prime = quad * 4u + kg;
         const uint32_t p = L.p[prime], p2 = 2u * p;
-        const int4 corr = L.corr[prime];
+        const uint2 fold = L.fold[prime];
         const Tw c16{65536u, L.sh16[prime]};
         uint32_t ra[4], rb[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
-            ra[ct] = conv_residue(w0, w1, a0[ct], a1[ct], corr, p, p2, c16);
-            rb[ct] = conv_residue(w0, w1, b0[ct], b1[ct], corr, p, p2, c16);
+            ra[ct] = conv_residue(w0, w1, a0[ct], a1[ct], fold, p, c16);
+            rb[ct] = conv_residue(w0, w1, b0[ct], b1[ct], fold, p, c16);
         }
         if (!fused) {
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
                 const uint32_t c = cA + 16u * ct + col;
-                if (prime < tb.k && c < n) res[size_t(prime) * n + c] = ra[ct];
-                if (prime < tb.k && c + 64u < n) res[size_t(prime) * n + c + 64u] = rb[ct];
+                if (prime < tb.k && c < n) res[size_t(prime) * n + c] = csub(ra[ct], p2);
+                if (prime < tb.k && c + 64u < n) res[size_t(prime) * n + c + 64u] = csub(rb[ct], p2);
             }
             continue;
+        }
+        // fused layout: column tiles 0, 1 sit in the lower half of the
+        // polynomial (lower operands of the first outer stage: below 4p),
+        // tiles 2, 3 in the upper half (multiplied operands: any 32-bit value)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            ra[ct] = csub(ra[ct], p2);
+            rb[ct] = csub(rb[ct], p2);
         }
         if (outer >= 1u) {  // the outer stages, as conv_compute applies them
             const Tw t1 = L.otw[prime][0];
@@ -854,7 +869,7 @@ __host__ __device__ constexpr uint32_t crt_ot_slot(uint32_t i) { return 2u * i +
 
 // The outer inverse stages of the fused layout on a loaded group: for each of
 // the lane's 16 primes its four values (positions x, x + n/4, x + n/2,
-// x + 3n/4 of the tile's scaled output, in [0,p)) go through the radix-2 or
+// x + 3n/4 of the tile's scaled output, in [0,2p)) go through the radix-2 or
 // radix-4 Gentleman-Sande stages of the separate outer pass, in the same
 // order, and back to [0,p) (the CRT needs canonical residues).  ot holds per
 // prime {p, w1, w1', w2} {w2', w3, w3', 0}: the inverse twiddles 1..3.

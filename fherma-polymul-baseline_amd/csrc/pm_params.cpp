@@ -334,6 +334,22 @@ std::string build_params(uint32_t max_logn, const uint32_t* q_words, uint32_t li
             r = mulmod(r, 256u, p);
         }
         s.conv_shoup16[i] = uint32_t((uint64_t(1) << 48) / p);
+        // The conversion keeps v = u + p + ((w + 2p) 2^16 mod p, in [0,2p)) with
+        // u = T0 + 256 T1, w = T2 + 256 T3 (T_d the byte-weighted digit sums)
+        // and reduces it at most once by 2p: exact ranges of u and w over any
+        // 112 bytes must give w + 2p in [0, 2^32), v in [0, 2^32) and v - 2p
+        // below 4p.  (Wide margins for every prime of this form: u >= -0.31 p,
+        // u + 3p <= 0.81 * 2^32.)
+        int64_t u_lo = 0, u_hi = 0, w_lo = 0, w_hi = 0;
+        for (uint32_t j = 0; j < kConvBytes; ++j) {
+            const int8_t* d = &s.conv_digits[(size_t(i) * kConvBytes + j) * 4];
+            const int64_t e = int64_t(d[0]) + 256 * int64_t(d[1]), f = int64_t(d[2]) + 256 * int64_t(d[3]);
+            (e < 0 ? u_lo : u_hi) += 255 * e;
+            (f < 0 ? w_lo : w_hi) += 255 * f;
+        }
+        const int64_t P = p, two32 = int64_t(1) << 32;
+        if (u_lo < -P || u_hi + 3 * P > two32 || u_hi >= 3 * P || w_lo < -2 * P || w_hi + 2 * P >= two32)
+            return "residue conversion bounds do not hold for a prime";
     }
     s.limb_pow.resize(size_t(k) * kLimbs);
     for (uint32_t i = 0; i < k; ++i) {

@@ -199,12 +199,13 @@ int pm_get_batch_tiles_per_block(pm_context* ctx, uint32_t tile_n, uint32_t* til
 /* ---- stage entry points (device buffers, context stream) ------------------
  * The three kernels of the plain path, exposed so each can be checked alone.
  * Residue buffers are [k][n] uint32, 16-byte aligned (PM_EINVAL otherwise). */
-/* residues of a and b: ra[i*n+j] = a_j mod p_i (value below 2*p_i) -- rns/solve.cu:138-157 */
+/* residues of a and b: ra[i*n+j] = a_j mod p_i (a representative below 4*p_i, the
+ * NTT's lazy input range) -- rns/solve.cu:138-157 */
 int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
                      uint32_t* d_rb, void* hip_stream);
 /* same contract by scalar limb reduction (sum of 28 Shoup products with
- * 2^(32j) mod p_i, rns/solve.cu:138-157) -- the comparison path of the MFMA
- * conversion */
+ * 2^(32j) mod p_i, rns/solve.cu:138-157; values below 2*p_i) -- the comparison
+ * path of the MFMA conversion */
 int pm_stage_convert_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
                             uint32_t* d_rb, void* hip_stream);
 /* forward NTTs, pointwise, inverse, scale: ra <- N^-1 (M_i mod p_i)^-1 * (a*b mod p_i) --
