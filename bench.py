@@ -126,6 +126,12 @@ def run_headline(args):
     batch2.close()
     print(f"[bench] double-buffered: {double_rate:.0f} products/s, equal {double_exact}", file=sys.stderr, flush=True)
 
+    # ---- the same batch as one persistent launch (grid barriers between the
+    # phases instead of one kernel per phase; reported in config only)
+    persistent = persistent_record(pm, ctx, [n] * B, batch, sptr, args.steps, args.warmup, args.seed)
+    print(f"[bench] persistent single launch: {persistent['products_per_s']:.0f} products/s, "
+          f"equal {persistent['equal_to_phased']}", file=sys.stderr, flush=True)
+
     # ---- plain path (one product per launch sequence, inputs in HBM) --------
     da = torch.from_numpy(inputs[0][0].view(np.int32)).cuda()
     db = torch.from_numpy(inputs[0][1].view(np.int32)).cuda()
@@ -272,6 +278,7 @@ def run_headline(args):
             "batched_over_plain": round(batched_rate / plain_rate, 3),
             "double_buffered_products_per_s": round(double_rate, 1),
             "double_buffered_equal": bool(double_exact),
+            "persistent_single_launch": persistent,
             "dropin_host_buffers_p50_ms": round(percentile(host_ms, 50), 4),
             "dropin_host_buffers_p99_ms": round(percentile(host_ms, 99), 4),
             "dropin_fherma_run_p50_ms": round(percentile(harness_ms, 50), 4),
@@ -295,7 +302,7 @@ def run_headline(args):
             "single_thread_product_p99_ms": round(percentile(cpu_ms, 99), 2) if cpu_ms else None,
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
         },
-        "bit_exact": bool(exact and gpu_consistent and done > 0),
+        "bit_exact": bool(exact and gpu_consistent and done > 0 and persistent["equal_to_phased"]),
     }
     if args.copy_mib:
         # a plain streaming kernel of known size (int32 negation: reads and
@@ -318,6 +325,34 @@ def run_headline(args):
             line["config"][key].get("bit_exact", True) for key in ("uneven_batch", "large_n", "serving")))
     print(json.dumps(line), flush=True)
     return 0
+
+
+def persistent_record(pm, ctx, ns, phased, sptr, steps, warmup, seed):
+    """The batch of `phased` (same sizes, same seeded inputs) as one
+    persistent launch: products/s over `steps` launches and whether its words
+    equal the phased graph's."""
+    import torch
+    wl = pm.workload
+    q = ctx.q
+    pers = pm.Batch(ctx, ns, mode="persistent")
+    for i, n in enumerate(ns):
+        s_ = wl.mix64(np.uint64(seed * 1000003 + i)).item()
+        pers.upload(i, wl.poly(s_, 0, n, q), wl.poly(s_, 1, n, q))
+    for _ in range(max(1, warmup)):
+        pers.run(sptr)
+    pers.wait(sptr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pers.run(sptr)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    pers.wait(sptr)
+    idx = sorted(set([0, len(ns) // 2, len(ns) - 1]))
+    equal = all(np.array_equal(pers.download(i), phased.download(i)) for i in idx)
+    pers.close()
+    return {"products_per_s": round(len(ns) / dt, 1), "ms_per_launch": round(dt * 1e3, 4),
+            "equal_to_phased": bool(equal)}
 
 
 def special_q_backends(pm, n, B, q_ref, sptr, stream):
@@ -382,7 +417,10 @@ def extra_workloads(args):
             "autotuned_batch_outer": u["autotuned_batch_outer"],
             "autotuned_tiles_per_block": u["autotuned_tiles_per_block"],
             "autotuned_crt_waves": u["autotuned_crt_waves"], "autotuned_parts": u["autotuned_parts"],
-            "bit_exact": u["bit_exact_vs_plain"]},
+            "persistent_single_launch": {"products_per_s": u["batched"]["persistent"]["products_per_s"],
+                                         "ms_per_launch": u["batched"]["persistent"]["ms_per_launch"],
+                                         "equal_to_phased": u["persistent_equal_to_phased"]},
+            "bit_exact": u["bit_exact_vs_plain"] and u["persistent_equal_to_phased"]},
         "large_n": {
             "workload": lg["workload"], "k_primes": lg["k_primes"],
             "mfma_conversion": {"products_per_s": m["products_per_s"], "conversion_ms": m["phases_ms"]["conversion"],
@@ -497,7 +535,8 @@ def run_uneven(args):
           f"parts {tuned_parts}", file=sys.stderr, flush=True)
     results = {}
     outputs = {}
-    for mode in ("phased",):
+    # the phased graph, and the same work list as one persistent launch
+    for mode in ("phased", "persistent"):
         batch = pm.Batch(ctx, ns, mode=mode)
         for i, (a, b) in enumerate(inputs):
             batch.upload(i, a, b)
@@ -533,6 +572,7 @@ def run_uneven(args):
         c = dc.cpu().numpy().view(np.uint32).reshape(ns[i], 28)
         j = i // 17
         assert np.array_equal(c, outputs["phased"][j])
+    persistent_equal = all(np.array_equal(x, y) for x, y in zip(outputs["phased"], outputs["persistent"]))
     counts = {}
     for n in ns:
         counts[n] = counts.get(n, 0) + 1
@@ -552,6 +592,7 @@ def run_uneven(args):
             "autotuned_parts": tuned_parts["parts"],
             "autotune_parts_ms": {str(p_): round(t, 4) for p_, t in tuned_parts["ms"].items()},
             "plain_one_at_a_time_products_per_s": round(len(ns) / (plain_total_ms * 1e-3), 1),
+            "persistent_equal_to_phased": bool(persistent_equal),
             "bit_exact_vs_plain": True}
     ctx.close()
     return line

@@ -6,9 +6,10 @@
 // conversion blocks, optional outer NTT stages, fused convolution tiles
 // grouped by tile size and ordered prime-major, CRT blocks -- in which every
 // phase is one contiguous run of items, and launches one kernel per phase over
-// its run.  (A persistent single-kernel mode whose blocks claimed items and
-// waited on per-product counters was removed after it hung the GPU; the flag
-// is refused.)
+// its run.  PM_BATCH_PERSISTENT runs the same work list in one launch instead:
+// one block per compute unit, grid barriers between the phases (no block waits
+// on another block's item; an earlier design whose blocks claimed items and
+// waited on per-product counters was removed after it hung the GPU).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -92,10 +93,52 @@ static int upload_worklist(pm_context* ctx, pm_batch* b, const pmd::Job* d_jobs,
     return PM_OK;
 }
 
+// The persistent launch (one graph node after the barrier reset): the whole
+// work list, built for its 512-thread tile blocks.
+static int build_persistent(pm_context* ctx, pm_batch* b, const pmd::Job* d_jobs) {
+    const uint32_t k = b->tables.k;
+    uint32_t tpb[16] = {};
+    for (uint32_t t = 5; t <= uint32_t(pmd::kPersistentMaxTileLog); ++t) tpb[t] = (pmd::kPersistentThreads * 32u) >> t;
+    std::vector<pmd::Item> items;
+    std::vector<pmd::Unit> units;
+    pm_build_worklist(k, b->jobs.data(), b->count, true, items, units, tpb);
+    if (!pm_phase_ranges(items, b->ranges)) return pm_fail(ctx, PM_EINVAL, "work list phases are not contiguous");
+    if (units.empty()) units.push_back({0, 0});
+    auto* d_items = static_cast<pmd::Item*>(b->alloc(items.size() * sizeof(pmd::Item)));
+    auto* d_units = static_cast<pmd::Unit*>(b->alloc(units.size() * sizeof(pmd::Unit)));
+    b->sync = static_cast<uint32_t*>(b->alloc(4 * sizeof(uint32_t)));
+    if (!d_items || !d_units || !b->sync) return pm_fail(ctx, PM_ENOMEM, "batch table allocation failed");
+    hipError_t e = hipMemcpy(d_items, items.data(), items.size() * sizeof(pmd::Item), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_units, units.data(), units.size() * sizeof(pmd::Unit), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(b->sync, 0, 4 * sizeof(uint32_t));
+    if (e == hipSuccess) e = pmd::persistent_blocks(b->tables.cus, &b->persistent_blocks);
+    if (e != hipSuccess) return pm_hip(ctx, e, "persistent batch set-up");
+    if (!b->persistent_blocks) return pm_fail(ctx, PM_EUNSUPPORTED, "persistent batch: a block does not fit a CU");
+    b->args = pmd::BatchArgs{d_jobs, d_items, d_units, uint32_t(items.size())};
+    b->n_items = b->args.n_items;
+    b->parts = 1;
+    const pmd::GridSync gs{b->sync, b->sync + pmd::kPersistentBarriers, ctx->wait_ticks};
+    hipStream_t stream = nullptr;
+    e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const hipError_t le = pmd::launch_persistent(b->tables, b->args, b->ranges, gs, b->persistent_blocks, stream);
+        const hipError_t ce = hipStreamEndCapture(stream, &b->graph);
+        e = le != hipSuccess ? le : ce;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->last_run, hipEventDisableTiming);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (e != hipSuccess) return pm_hip(ctx, e, "capture persistent batch graph");
+    return PM_OK;
+}
+
 int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint32_t* const* d_a,
                    const uint32_t* const* d_b, uint32_t* const* d_c) {
     b->tables = pm_tables(ctx);  // the backend at creation stays the batch's
     b->crt_waves = ctx->batch_crt_waves;
+    if (b->persistent && b->tables.quartic)
+        return pm_fail(ctx, PM_EUNSUPPORTED, "persistent batch: general backend only");
     const uint32_t k = b->tables.k, count = b->count;
     b->logn.resize(count);
     for (uint32_t i = 0; i < count; ++i) {
@@ -111,6 +154,13 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
         pmd::Job& j = jobs[i];
         j.logn = b->logn[i];
         j.outer = ctx->batch_outer[b->tables.quartic ? PM_BACKEND_QUARTIC : PM_BACKEND_GENERAL][b->logn[i]];
+        if (b->persistent) {
+            // tiles of at most 2^14 (a 512-thread block), outer stages fused
+            const uint32_t least = j.logn > uint32_t(pmd::kPersistentMaxTileLog) ? j.logn - pmd::kPersistentMaxTileLog : 0u;
+            if (j.outer < least) j.outer = least;
+            if (j.outer && !pmd::outer_fusable(j.logn, j.outer))
+                return pm_fail(ctx, PM_EUNSUPPORTED, "persistent batch: N up to 2^16");
+        }
         j.fused = !b->separate_outer && !b->tables.quartic && pmd::outer_fusable(j.logn, j.outer) ? 1u : 0u;
         const bool caller_io = d_a && d_b && d_c;
         if (caller_io) {
@@ -134,6 +184,7 @@ int pm_batch_build(pm_context* ctx, pm_batch* b, const uint32_t* ns, const uint3
     hipError_t e = hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(pmd::Job), hipMemcpyHostToDevice);
     if (e != hipSuccess) return pm_hip(ctx, e, "upload batch tables");
     b->jobs = jobs;
+    if (b->persistent) return build_persistent(ctx, b, d_jobs);
 
     // the whole batch as one work list (per-phase timing; graphs of one part)
     int st = upload_worklist(ctx, b, d_jobs, 0, count, b->args, b->ranges);
@@ -282,17 +333,13 @@ int pm_batch_create(pm_context* ctx, uint32_t count, const uint32_t* ns, const u
     if (flags & ~uint32_t(PM_BATCH_PERSISTENT | kPhasedFlags)) return PM_EINVAL;
     if ((flags & PM_BATCH_PERSISTENT) && (flags & kPhasedFlags)) return PM_EINVAL;
     *out = nullptr;
-    // The persistent kernel hung the GPU on a uniform batch after a change to
-    // the tile code it shares; no cause was found, so the mode was removed.
-    if (flags & PM_BATCH_PERSISTENT)
-        return pm_fail(ctx, PM_EUNSUPPORTED, "persistent batch mode is disabled (it hung the GPU in testing); "
-                                             "use the phased mode");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return pm_hip(ctx, e, "set device");
     pm_batch* b = new (std::nothrow) pm_batch();
     if (!b) return pm_fail(ctx, PM_ENOMEM, "host allocation failed");
     b->ctx = ctx;
     b->count = count;
+    b->persistent = (flags & PM_BATCH_PERSISTENT) != 0;
     b->scalar_conversion = (flags & PM_BATCH_SCALAR_CONVERSION) != 0;
     b->scalar_crt = (flags & PM_BATCH_SCALAR_CRT) != 0;
     // the scalar conversion and CRT keep consecutive coefficients: outer stages as passes of their own
@@ -353,9 +400,16 @@ int pm_batch_run(pm_batch* b, void* hip_stream) {
 int pm_batch_wait(pm_batch* b, void* hip_stream) {
     if (!b) return PM_EINVAL;
     hipStream_t stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->ctx->stream;
-    // phased launches have no in-kernel waits: a device error is all there is to report
-    const hipError_t e = hipStreamSynchronize(stream);
+    // phased launches have no in-kernel waits: a device error is all there is
+    // to report; the persistent launch also reports a barrier wait that gave up
+    hipError_t e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return pm_hip(b->ctx, e, "batch wait");
+    if (b->persistent) {
+        uint32_t err = 0;
+        e = hipMemcpy(&err, b->sync + pmd::kPersistentBarriers, sizeof(err), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return pm_hip(b->ctx, e, "batch wait");
+        if (err) return pm_fail(b->ctx, PM_ETIMEOUT, "persistent batch: a grid barrier wait ran out (blocks not resident)");
+    }
     return PM_OK;
 }
 

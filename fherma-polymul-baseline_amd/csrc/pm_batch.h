@@ -32,6 +32,9 @@ struct pm_batch {
     bool separate_outer = false;    // outer NTT stages as passes of their own, not inside conversion / CRT
     uint32_t crt_waves = 0;         // waves per block of the MFMA CRT (the context's at creation)
     uint32_t parts = 1;             // staggered parts of the graph (1: one part)
+    bool persistent = false;        // one launch for the whole work list (grid barriers between phases)
+    uint32_t* sync = nullptr;       // persistent: [2] barrier counters, [2] error word (device memory)
+    uint32_t persistent_blocks = 0;
     std::vector<void*> allocations;
     uint64_t device_bytes = 0;
     hipGraph_t graph = nullptr;

@@ -127,6 +127,19 @@ struct BatchArgs {
 
 constexpr uint32_t kErrTimeout = 2;
 
+// Persistent batch (one launch for the whole work list): one block of 512
+// threads per compute unit; tiles up to 2^14 residues (the block's 16384
+// residues); two grid barriers, their arrival counters and a shared error
+// word in device memory, cleared by a kernel ahead of each launch.
+constexpr uint32_t kPersistentThreads = 512;
+constexpr int kPersistentMaxTileLog = 14;
+constexpr uint32_t kPersistentBarriers = 2;
+struct GridSync {
+    uint32_t* counters;   // [kPersistentBarriers]
+    uint32_t* error;      // kErrTimeout when a barrier wait gave up
+    uint64_t wait_ticks;  // bound of one barrier wait (wall-clock ticks)
+};
+
 // Host-buffer path: flags in page-locked host memory (fine-grained) that the
 // host sets as input chunks land in staging (ready) and the device sets as
 // result chunks land in staging (done); counters (device memory) count the

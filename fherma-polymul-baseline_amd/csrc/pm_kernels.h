@@ -67,6 +67,15 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
 // one launch per phase (and per tile size class) over the same work list
 hipError_t launch_phased(const Tables& tb, const BatchArgs& ba, const PhaseRanges& ranges, bool scalar_conversion,
                          bool scalar_crt, uint32_t crt_waves, hipStream_t stream);
+// The persistent batch: clears gs, then one launch of `blocks` blocks runs the
+// conversion, tile and CRT phases of the work list with grid barriers between
+// them (the work list built for kPersistentThreads-thread tile blocks, tiles
+// of at most 2^kPersistentMaxTileLog, outer stages fused, general backend).
+hipError_t launch_persistent(const Tables& tb, const BatchArgs& ba, const PhaseRanges& ranges, const GridSync& gs,
+                             uint32_t blocks, hipStream_t stream);
+// blocks of the persistent kernel that are resident at once on `cus` compute
+// units (0 when none fits)
+hipError_t persistent_blocks(uint32_t cus, uint32_t* blocks);
 // device address of the static residue scratch (kScratchWords words) on
 // `device`, which must be the current device; looked up once per device
 hipError_t scratch_address(int device, uint32_t** out);

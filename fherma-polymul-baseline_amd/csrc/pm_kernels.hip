@@ -1411,9 +1411,10 @@ __global__ void __launch_bounds__(kCrtPerBlock) k_crt(Tables tb, Job job) {
 // slots of 128) run by resident blocks: wave w of block b takes slots b*W + w,
 // + grid*W, ...; the weights go to LDS once per block.  A slot's two halves
 // are loaded together (64 b128 loads per lane in flight), then converted.
-__global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, uint32_t first, uint32_t count) {
-    __shared__ ConvLds L;
-    conv_tables_to_lds(tb, L);
+// The conversion items [first, first + count) by the waves of the grid, the
+// tables already in L (each wave independent: no block barrier inside).
+__device__ __forceinline__ void conv_items_loop(const Tables& tb, const BatchArgs& ba, uint32_t first,
+                                                uint32_t count, const ConvLds& L) {
     constexpr uint32_t per_item = kConvItemCoefs / kConvSlotCoefs;
     const uint32_t waves = blockDim.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t total = count * per_item, stride = gridDim.x * waves;
@@ -1434,6 +1435,12 @@ __global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, 
         conv_load(coef, n, cB, lane, b0, b1, cstride);
         conv_slot_compute(tb, a0, a1, b0, b1, n, cA, cstride, fused, job.outer, res, L, lane);
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_items_conv(Tables tb, BatchArgs ba, uint32_t first, uint32_t count) {
+    __shared__ ConvLds L;
+    conv_tables_to_lds(tb, L);
+    conv_items_loop(tb, ba, first, count, L);
 }
 
 __global__ void __launch_bounds__(kBlock) k_items_conv_scalar(Tables tb, BatchArgs ba, uint32_t first) {
@@ -1544,8 +1551,8 @@ __global__ void __launch_bounds__(kCrtPlainWaves * 64) k_crt_mfma(Tables tb, Job
 // block: 8 fill a CU's LDS (about 154 KB) and its VGPRs (224 per lane, two
 // waves per SIMD) with one block; 4 or 2 spread a small batch over more CUs.
 template <int W>
-__global__ void __launch_bounds__(W * 64) k_items_crt_mfma(Tables tb, BatchArgs ba, uint32_t first,
-                                                           uint32_t count) {
+__device__ __forceinline__ void crt_items_loop(const Tables& tb, const BatchArgs& ba, uint32_t first,
+                                               uint32_t count) {
     constexpr uint32_t per_item = kBlock / kCrtWaveCoefs;
     crt_group_loop<W>(tb, count * per_item, [&](uint32_t gi, CrtGroup& w) {
         const Item it = ba.items[first + gi / per_item];
@@ -1557,12 +1564,84 @@ __global__ void __launch_bounds__(W * 64) k_items_crt_mfma(Tables tb, BatchArgs 
     });
 }
 
+template <int W>
+__global__ void __launch_bounds__(W * 64) k_items_crt_mfma(Tables tb, BatchArgs ba, uint32_t first,
+                                                           uint32_t count) {
+    crt_items_loop<W>(tb, ba, first, count);
+}
+
 __global__ void __launch_bounds__(kBlock) k_items_crt(Tables tb, BatchArgs ba, uint32_t first) {
     const Item it = ba.items[first + blockIdx.x];
     const Job job = ba.jobs[it.job];
     const uint32_t n = 1u << job.logn;
     const uint32_t c = it.a0 * kBlock + threadIdx.x;
     if (c < n) crt_one(tb, job.ra, n, c, job.c);
+}
+
+// ---------------------------------------------------------------------------
+// persistent batch: the whole work list in one launch
+// ---------------------------------------------------------------------------
+// One block of kPersistentThreads per compute unit runs the phases of the
+// phased graph in turn -- conversion, fused tiles (largest size class first),
+// CRT -- as grid-stride loops over their items, with a grid barrier between
+// phases.  Every block runs the same sequence (no early exit before the last
+// phase), so each barrier is reached by all blocks; the grid is one block per
+// compute unit and a block holds most of a CU's LDS, so all blocks are
+// resident at once.  A barrier wait is bounded by wall-clock ticks and a spin
+// count: running out latches kErrTimeout in the error word and the block goes
+// on (the results are then wrong and the host reports the timeout), so no
+// wait can last.  One kernel is register-allocated for its heaviest phase
+// (the MFMA CRT), which is what this mode costs against the phased graph.
+constexpr uint32_t kPersistentMaxSpins = 1u << 26;
+
+__device__ __forceinline__ void grid_barrier(uint32_t* counter, const GridSync& gs) {
+    __threadfence();  // this thread's stores visible at device scope (L2 written back)
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t start = wall_clock64();
+        for (uint32_t spins = 0;; ++spins) {
+            if (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= gridDim.x) break;
+            if (__hip_atomic_load(gs.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+            if (wall_clock64() - start > gs.wait_ticks || spins >= kPersistentMaxSpins) {
+                __hip_atomic_fetch_or(gs.error, kErrTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+        }
+    }
+    __syncthreads();
+    __threadfence();  // and the other blocks' stores visible to this thread's loads (caches invalidated)
+}
+
+// Tile items of class TLOG and then of the smaller classes, block-strided;
+// the block's threads are an item's tiles x threads per tile (the work list
+// is built for it).
+template <int TLOG>
+__device__ __forceinline__ void tile_items_loop(const Tables& tb, const BatchArgs& ba, const PhaseRanges& r,
+                                                uint32_t* lds) {
+    for (uint32_t i = blockIdx.x; i < r.tile_count[TLOG]; i += gridDim.x) {
+        __syncthreads();  // the previous item's last exchange may still be read by other waves
+        tile_item<TLOG>(tb, ba, ba.items[r.tile_first[TLOG] + i], lds);
+    }
+    if constexpr (TLOG > 5) tile_items_loop<TLOG - 1>(tb, ba, r, lds);
+}
+
+__global__ void __launch_bounds__(kPersistentThreads) k_batch_persistent(Tables tb, BatchArgs ba, PhaseRanges r,
+                                                                        GridSync gs) {
+    extern __shared__ uint32_t lds[];
+    ConvLds& L = *reinterpret_cast<ConvLds*>(lds);
+    conv_tables_to_lds(tb, L);
+    conv_items_loop(tb, ba, r.conv_first, r.conv_count, L);
+    grid_barrier(gs.counters + 0, gs);
+    tile_items_loop<kPersistentMaxTileLog>(tb, ba, r, lds);
+    grid_barrier(gs.counters + 1, gs);
+    crt_items_loop<kPersistentThreads / 64>(tb, ba, r.crt_first, r.crt_count);
+}
+
+__global__ void __launch_bounds__(64) k_clear_sync(GridSync gs) {
+    if (threadIdx.x < kPersistentBarriers) gs.counters[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) *gs.error = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -2015,6 +2094,51 @@ hipError_t launch_crt(const Tables& tb, const Job& job, hipStream_t stream) {
     return launch_crt_range(tb, job, 0, 1u << job.logn, stream);
 }
 
+// LDS of a persistent block: the largest of the conversion tables, a tile
+// phase's tiles (kPersistentThreads x 32 residues, padded) and the 8-wave CRT
+template <int TLOG>
+constexpr uint32_t persistent_tile_words() {
+    using G = Geo<TLOG>;
+    const uint32_t here = (kPersistentThreads >> G::TPTLOG) * uint32_t(G::STRIDE);
+    if constexpr (TLOG > 5) {
+        const uint32_t below = persistent_tile_words<TLOG - 1>();
+        return here > below ? here : below;
+    } else {
+        return here;
+    }
+}
+constexpr size_t persistent_lds() {
+    size_t m = sizeof(ConvLds);
+    const size_t t = size_t(persistent_tile_words<kPersistentMaxTileLog>()) * sizeof(uint32_t);
+    m = t > m ? t : m;
+    const size_t c = crt_block_lds<kPersistentThreads / 64>();
+    return c > m ? c : m;
+}
+static_assert(persistent_lds() <= kLdsPerCu, "a persistent block fits the LDS of a CU");
+static_assert(kPersistentThreads / 64 == 8, "the persistent CRT phase runs 8-wave groups");
+
+hipError_t persistent_blocks(uint32_t cus, uint32_t* blocks) {
+    int per_cu = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&k_batch_persistent), int(kPersistentThreads), persistent_lds());
+    if (e != hipSuccess) return e;
+    // one block per CU: the LDS holds no second one, and more blocks would
+    // only wait at the barriers
+    *blocks = per_cu >= 1 ? cus : 0u;
+    return hipSuccess;
+}
+
+hipError_t launch_persistent(const Tables& tb, const BatchArgs& ba, const PhaseRanges& r, const GridSync& gs,
+                             uint32_t blocks, hipStream_t stream) {
+    if (!blocks || tb.quartic || r.fwd_count || r.inv_count || r.tile_count[15]) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_clear_sync, dim3(1), dim3(64), 0, stream, gs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch_persistent, dim3(blocks), dim3(kPersistentThreads), persistent_lds(), stream, tb, ba,
+                       r, gs);
+    return hipGetLastError();
+}
+
 }  // namespace pmd
 
 namespace pmd {
@@ -2051,6 +2175,9 @@ hipError_t configure_kernels() {
         return e;
     if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_crt_mfma<8>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(crt_block_lds<8>()))) != hipSuccess)
+        return e;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_batch_persistent),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(persistent_lds()))) != hipSuccess)
         return e;
     if ((e = allow_lds<1>()) != hipSuccess) return e;
     if ((e = allow_lds<2>()) != hipSuccess) return e;

@@ -225,9 +225,14 @@ int pm_stage_crt_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32
 typedef struct pm_batch pm_batch;
 /* flags: PM_BATCH_PHASED (default, 0 also selects it) runs one launch per
  * phase -- conversion, outer stages, one per tile size class, CRT -- inside
- * the captured graph; PM_BATCH_PERSISTENT (one persistent kernel whose blocks
- * claim items in order and wait on per-product completion counters) is
- * currently disabled and returns PM_EUNSUPPORTED: it hung the GPU in testing. */
+ * the captured graph, each kernel register-allocated for its own phase.
+ * PM_BATCH_PERSISTENT runs the same work list in one persistent launch: one
+ * 512-thread block per compute unit loops over the conversion items, then the
+ * tiles, then the CRT groups, with a grid barrier between phases (bounded
+ * waits: a barrier that cannot complete makes pm_batch_wait return
+ * PM_ETIMEOUT instead of hanging).  General backend, N up to 2^16 (tiles of at
+ * most 2^14, outer stages inside conversion and CRT); PM_EUNSUPPORTED
+ * otherwise.  Same words as the phased graph. */
 #define PM_BATCH_PERSISTENT 1u
 #define PM_BATCH_PHASED 2u
 #define PM_BATCH_SCALAR_CONVERSION 4u /* phased: scalar limb reduction instead of int8 MFMA */

@@ -103,6 +103,46 @@ def test_batch_golden_and_identical_products(pm, workload, mode):
     ctx.close()
 
 
+@pytest.mark.parametrize("ns", [[32768] * 24, [65536, 4096, 32768, 1024, 8192, 16384, 2048, 256, 65536, 512]],
+                         ids=["uniform-2^15", "uneven-2^8..2^16"])
+def test_persistent_launch_matches_phased(pm, orc, workload, ns):
+    """One persistent launch (grid barriers between the phases) gives the
+    phased graph's words, on two replays; the first product against the
+    oracle as well."""
+    q = workload.modulus(3)
+    ctx = pm.Context(q, max(ns))
+    say("persistent vs phased", len(ns), sorted(set(ns)))
+    inputs = [workload.family("dense-random-1", n, q, 500 + i) for i, n in enumerate(ns)]
+    outs = {}
+    for mode in ("phased", "persistent"):
+        batch = pm.Batch(ctx, ns, mode=mode)
+        for i, (a, b) in enumerate(inputs):
+            batch.upload(i, a, b)
+        batch.run()
+        batch.wait()
+        first = [batch.download(i) for i in range(len(ns))]
+        batch.run()
+        batch.wait()
+        assert all((batch.download(i) == first[i]).all() for i in range(len(ns))), mode + ": replay differs"
+        outs[mode] = first
+        batch.close()
+    for i in range(len(ns)):
+        assert (outs["persistent"][i] == outs["phased"][i]).all(), (i, ns[i])
+    a, b = inputs[0]
+    n = ns[0]
+    primes = orc.reference_primes_for(n, q) if n <= 32768 else orc.primes_general(n, q)
+    assert (outs["persistent"][0] == orc.rns_multiply(q, a, b, primes)).all()
+    ctx.close()
+
+
+def test_persistent_launch_refuses_what_it_does_not_cover(pm, workload):
+    q = workload.modulus(3)
+    ctx = pm.Context(q, 1 << 17)
+    with pytest.raises(pm.PolymulError):
+        pm.Batch(ctx, [131072, 1024], mode="persistent")  # tiles above 2^14 would need 2^17 outer stages
+    ctx.close()
+
+
 def test_batch_rejects_small_n(pm, workload):
     q = workload.modulus(1)
     ctx = pm.Context(q, 1 << 10)
