@@ -219,10 +219,13 @@ def run_headline(args):
             valu_instructions = tphase.get("valu_instructions_per_launch")
     # The tile is bound by the vector ALUs: busy fraction = wave64 VALU
     # instructions per launch (SQ_INSTS_VALU, deterministic for this workload)
-    # x 4 SIMD cycles each / (1024 SIMDs x 2.4 GHz x the phase time)
+    # x 4 SIMD cycles each / (1024 SIMDs x 2.4 GHz x the phase time); and the
+    # same at the clock the chip holds while the batch runs (power-capped)
     valu_frac = None
     if valu_instructions:
         valu_frac = valu_instructions * 4 / (VALU_SIMDS * VALU_CLOCK_HZ * phase_ms[dominant] * 1e-3)
+    clock = clock_and_power(lambda: batch.run(sptr))
+    print(f"[bench] under load: {clock}", file=sys.stderr, flush=True)
 
     # ---- CPU baseline: the oracle port on a bounded sample -------------------
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -292,7 +295,7 @@ def run_headline(args):
             "gpu_paths_agree": gpu_consistent and harness_exact and ref_exact and ref_q_exact,
         },
         "roofline": roofline_record(dominant, phase_ms, phase_gbs, per_phase, phase_key, B, args.mode, traffic,
-                                    valu_instructions, valu_frac, per_product, launch_ms),
+                                    valu_instructions, valu_frac, per_product, launch_ms, clock),
         "cpu_baseline": {
             "value": round(par_rate, 4), "unit": "products/s", "cores": threads, "kind": "port",
             "sample": f"{sample} products of the same workload (N={n}), oracle RNS port, one product per thread on "
@@ -325,6 +328,59 @@ def run_headline(args):
             line["config"][key].get("bit_exact", True) for key in ("uneven_batch", "large_n", "serving")))
     print(json.dumps(line), flush=True)
     return 0
+
+
+def clock_and_power(launch, seconds=1.0):
+    """Mean GFX clock (MHz, over the XCDs) and socket power (W) sampled with
+    amdsmi every ~10 ms while `launch` is called back to back for about
+    `seconds`; None where amdsmi is not available.  Under a sustained load the
+    chip holds its power cap and lowers the clock below its 2.4 GHz peak."""
+    import threading
+    import torch
+    try:
+        import amdsmi
+        amdsmi.amdsmi_init()
+        handle = amdsmi.amdsmi_get_processor_handles()[0]
+        cap = amdsmi.amdsmi_get_power_cap_info(handle).get("power_cap")
+    except Exception:
+        return None
+    samples, stop = [], [False]
+
+    def sampler():
+        while not stop[0]:
+            try:
+                m = amdsmi.amdsmi_get_gpu_metrics_info(handle)
+                clks = [c for c in m.get("current_gfxclks", []) if isinstance(c, (int, float))]
+                power = m.get("current_socket_power")
+                if clks and isinstance(power, (int, float)):
+                    samples.append((float(np.mean(clks)), float(power)))
+            except Exception:
+                pass
+            time.sleep(0.01)
+
+    for _ in range(3):
+        launch()
+    torch.cuda.synchronize()
+    th = threading.Thread(target=sampler)
+    th.start()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            launch()
+        torch.cuda.synchronize()
+    stop[0] = True
+    th.join()
+    try:
+        amdsmi.amdsmi_shut_down()
+    except Exception:
+        pass
+    if len(samples) < 4:
+        return None
+    busy = samples[len(samples) // 4:]  # after the clock has settled under the load
+    return {"gfx_clock_mhz": round(float(np.mean([c for c, _ in busy])), 1),
+            "socket_power_w": round(float(np.mean([w for _, w in busy])), 1),
+            "power_cap_w": round(cap / 1e6, 1) if isinstance(cap, (int, float)) else None,
+            "samples": len(busy)}
 
 
 def persistent_record(pm, ctx, ns, phased, sptr, steps, warmup, seed):
@@ -443,7 +499,7 @@ def extra_workloads(args):
 
 
 def roofline_record(dominant, phase_ms, phase_gbs, per_phase, phase_key, B, mode, traffic, valu_instructions,
-                    valu_frac, per_product, launch_ms):
+                    valu_frac, per_product, launch_ms, clock=None):
     """Roofline of the dominant phase.  The NTT tile is bound by the vector
     ALUs, not HBM: with its VALU instruction count (SQ_INSTS_VALU, from the
     counter profile of this kernel) the binding roof is the VALU issue rate
@@ -467,7 +523,12 @@ def roofline_record(dominant, phase_ms, phase_gbs, per_phase, phase_key, B, mode
         "phases_ms": {k_: round(v, 4) for k_, v in phase_ms.items()},
         "phases_gbs": {k_: round(v, 1) for k_, v in phase_gbs.items()},
         "whole_launch_gbs": round(per_product * B / (launch_ms * 1e-3) / 1e9, 1),
+        "under_load": clock,
     })
+    if valu_frac is not None and clock and clock.get("gfx_clock_mhz"):
+        # the same instructions against the clock sampled while the batch runs
+        rec["valu_frac_at_sampled_clock"] = round(
+            valu_instructions * 4 / (VALU_SIMDS * clock["gfx_clock_mhz"] * 1e6 * phase_ms[dominant] * 1e-3), 4)
     return rec
 
 
