@@ -472,6 +472,7 @@ def extra_workloads(args):
             "batched_over_plain": round(ph["products_per_s"] / u["plain_one_at_a_time_products_per_s"], 3),
             "autotuned_batch_outer": u["autotuned_batch_outer"],
             "autotuned_tiles_per_block": u["autotuned_tiles_per_block"],
+            "autotuned_residues_per_thread": u["autotuned_residues_per_thread"],
             "autotuned_crt_waves": u["autotuned_crt_waves"], "autotuned_parts": u["autotuned_parts"],
             "persistent_single_launch": {"products_per_s": u["batched"]["persistent"]["products_per_s"],
                                          "ms_per_launch": u["batched"]["persistent"]["ms_per_launch"],
@@ -648,6 +649,9 @@ def run_uneven(args):
                                             for n, r in tuned.items()},
             "batched": results,
             "autotuned_tiles_per_block": {str(t): r["tiles"] for t, r in tuned_tiles.items()},
+            "autotuned_residues_per_thread": {str(t): r["residues"] for t, r in tuned_tiles.items()},
+            "autotune_tiles_ms": {str(t): {k_: round(v, 4) for k_, v in r["ms"].items()}
+                                  for t, r in tuned_tiles.items()},
             "autotuned_crt_waves": tuned_waves["waves"],
             "autotune_crt_waves_ms": {str(w): round(t, 4) for w, t in tuned_waves["ms"].items()},
             "autotuned_parts": tuned_parts["parts"],

@@ -89,6 +89,8 @@ def library():
     lib.pm_get_batch_outer.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_set_batch_tiles_per_block.argtypes = [P, u32, u32]
     lib.pm_get_batch_tiles_per_block.argtypes = [P, u32, ctypes.POINTER(u32)]
+    lib.pm_set_batch_residues_per_thread.argtypes = [P, u32, u32]
+    lib.pm_get_batch_residues_per_thread.argtypes = [P, u32, ctypes.POINTER(u32)]
     lib.pm_set_batch_crt_waves.argtypes = [P, u32]
     lib.pm_set_batch_parts.argtypes = [P, u32]
     lib.pm_get_batch_parts.argtypes = [P, ctypes.POINTER(u32)]
@@ -128,7 +130,8 @@ EXPORTED = ["pm_plan_rns", "pm_plan_quartic", "pm_create", "pm_destroy", "pm_las
             "pm_multiply_submit", "pm_multiply_collect", "pm_set_io_chunks", "pm_set_io_wait_limit",
             "pm_inject_io_stall", "pm_place_near_device", "pm_multiply_device", "pm_multiply_device_noalloc",
             "pm_synchronize", "pm_set_backend", "pm_get_backend", "pm_set_outer", "pm_get_outer", "pm_set_batch_outer", "pm_get_batch_outer",
-            "pm_set_batch_tiles_per_block", "pm_get_batch_tiles_per_block", "pm_set_batch_crt_waves",
+            "pm_set_batch_tiles_per_block", "pm_get_batch_tiles_per_block", "pm_set_batch_residues_per_thread",
+            "pm_get_batch_residues_per_thread", "pm_set_batch_crt_waves",
             "pm_get_batch_crt_waves", "pm_set_batch_parts", "pm_get_batch_parts", "pm_set_conversion_slices", "pm_get_conversion_slices",
             "pm_stage_convert",
             "pm_stage_convert_scalar", "pm_stage_ntt", "pm_stage_crt", "pm_stage_crt_scalar",
@@ -349,10 +352,24 @@ class Context:
         self._check("pm_get_batch_outer", self._lib.pm_get_batch_outer(self._h, int(n), ctypes.byref(v)))
         return v.value
 
+    def set_batch_residues_per_thread(self, tile_n, residues):
+        """Batch schedule: residues each thread of a tile of size tile_n holds
+        (32, the default; or 16 for tiles of 2^6 .. 2^14); resets the size's
+        tiles per block to that choice's default."""
+        self._check("pm_set_batch_residues_per_thread",
+                    self._lib.pm_set_batch_residues_per_thread(self._h, int(tile_n), int(residues)))
+
+    def get_batch_residues_per_thread(self, tile_n):
+        v = ctypes.c_uint32()
+        self._check("pm_get_batch_residues_per_thread",
+                    self._lib.pm_get_batch_residues_per_thread(self._h, int(tile_n), ctypes.byref(v)))
+        return v.value
+
     def set_batch_tiles_per_block(self, tile_n, tiles):
         """Batch schedule: tiles of size tile_n per block (0 = the default:
-        one tile from 2^13 up, a 1024-thread block of smaller tiles) in
-        batches and served launches created afterwards."""
+        one tile from 2^13 up, a 1024-thread block of smaller tiles, at 32
+        residues per thread) in batches and served launches created
+        afterwards."""
         self._check("pm_set_batch_tiles_per_block",
                     self._lib.pm_set_batch_tiles_per_block(self._h, int(tile_n), int(tiles)))
 
@@ -402,13 +419,16 @@ class Context:
         """Apply a schedule written by `autotune_plain` / `autotune_batch` /
         `autotune_backend` (sizes this context cannot hold, and a quartic
         backend its modulus does not allow, are skipped); returns {"outer":
-        {n: u}, "batch_outer": {n: u}, "batch_tiles_per_block": {tile_n:
-        tiles}, "conversion_slices": {n: s}} and "backend",
-        "batch_crt_waves" and "batch_parts" when they were set, as applied."""
+        {n: u}, "batch_outer": {n: u}, "batch_residues_per_thread": {tile_n:
+        r}, "batch_tiles_per_block": {tile_n: tiles}, "conversion_slices": {n:
+        s}} and "backend", "batch_crt_waves" and "batch_parts" when they were
+        set, as applied (residues per thread before tiles per block, which
+        they reset)."""
         import json
         with open(path) as f:
             doc = json.load(f)
-        applied = {"outer": {}, "batch_outer": {}, "batch_tiles_per_block": {}, "conversion_slices": {}}
+        applied = {"outer": {}, "batch_outer": {}, "batch_residues_per_thread": {}, "batch_tiles_per_block": {},
+                   "conversion_slices": {}}
         if "backend" in doc:
             try:
                 self.set_backend(doc["backend"])
@@ -423,6 +443,7 @@ class Context:
             self.set_batch_parts(doc["batch_parts"])
             applied["batch_parts"] = int(doc["batch_parts"])
         for key, setter in (("outer", self.set_outer), ("batch_outer", self.set_batch_outer),
+                            ("batch_residues_per_thread", self.set_batch_residues_per_thread),
                             ("batch_tiles_per_block", self.set_batch_tiles_per_block),
                             ("conversion_slices", self.set_conversion_slices)):
             for n_str, outer in doc.get(key, {}).items():
@@ -674,16 +695,21 @@ def _time_batch(ctx, ns, inputs, reps, stream):
     return float(np.median(ms))
 
 
-def tiles_per_block_choices(tile_n):
+def tiles_per_block_choices(tile_n, residues=32):
     """Valid tiles per block of a tile size: powers of two with
-    2048 <= tiles x tile_n <= 32768 (whole waves of 32 residues per thread,
+    64 r <= tiles x tile_n <= 1024 r (whole waves of r residues per thread,
     at most 1024 threads)."""
     out, t = [], 1
-    while t * tile_n <= 32768:
-        if t * tile_n >= 2048:
+    while t * tile_n <= 1024 * residues:
+        if t * tile_n >= 64 * residues:
             out.append(t)
         t *= 2
     return out
+
+
+def residues_per_thread_choices(tile_n):
+    """Residues per thread a tile size can run: 32, and 16 for 2^6 .. 2^14."""
+    return [32, 16] if 64 <= tile_n <= 16384 else [32]
 
 
 def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None, coefficients=1 << 20):
@@ -694,8 +720,8 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None, coefficien
     products, at most 512 (a batch too small to fill the device would favour
     whatever spreads a few products widest):
 
-    1. tiles per block of each tile size 2^t the sizes can use, on products
-       of N = 2^t (no outer stages);
+    1. residues per thread (32 or 16) and tiles per block of each tile size
+       2^t the sizes can use, on products of N = 2^t (no outer stages);
     2. outer stages (0..2: the tile size, and radix-2 or radix-4 outer
        stages in the conversion and the CRT) for each N, with the tiles per
        block of step 1;
@@ -706,11 +732,12 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None, coefficien
 
     All are kept in the context (for batches and served launches created
     afterwards, on its current backend).  Returns {n: {"outer": u, "ms": {u:
-    median ms per launch}}, "tiles_per_block": {tile_n: {"tiles": t, "ms":
-    {t: ms}}}, "crt_waves": {"waves": w, "ms": {w: ms}}, "parts": {"parts":
+    median ms per launch}}, "tiles_per_block": {tile_n: {"residues": r,
+    "tiles": t, "ms": {"r/t": ms}}}, "crt_waves": {"waves": w, "ms": {w: ms}}, "parts": {"parts":
     p, "ms": {p: ms}}}; with `path` the choice is also written to the
     schedule file `load_schedule` reads ("batch_outer",
-    "batch_tiles_per_block", "batch_crt_waves", "batch_parts")."""
+    "batch_residues_per_thread", "batch_tiles_per_block", "batch_crt_waves",
+    "batch_parts")."""
     import json
     import torch
     from . import workload as wl
@@ -737,19 +764,22 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None, coefficien
                 tiles.setdefault(1 << (lg - outer), None)
     tile_result = {}
     for tile_n in sorted(tiles):
-        choices = tiles_per_block_choices(tile_n)
-        if tile_n > max_n or len(choices) < 2:
+        combos = [(r, t) for r in residues_per_thread_choices(tile_n) for t in tiles_per_block_choices(tile_n, r)]
+        if tile_n > max_n or len(combos) < 2:
             continue
         saved = ctx.get_batch_outer(tile_n)
         ctx.set_batch_outer(tile_n, 0)
         times = {}
-        for t in choices:
+        for r, t in combos:
+            ctx.set_batch_residues_per_thread(tile_n, r)
             ctx.set_batch_tiles_per_block(tile_n, t)
-            times[t] = trial(tile_n)
-        best = min(times, key=times.get)
-        ctx.set_batch_tiles_per_block(tile_n, best)
+            times[(r, t)] = trial(tile_n)
+        best_r, best_t = min(times, key=times.get)
+        ctx.set_batch_residues_per_thread(tile_n, best_r)
+        ctx.set_batch_tiles_per_block(tile_n, best_t)
         ctx.set_batch_outer(tile_n, saved)
-        tile_result[tile_n] = {"tiles": best, "ms": times}
+        tile_result[tile_n] = {"residues": best_r, "tiles": best_t,
+                               "ms": {f"{r}/{t}": ms for (r, t), ms in times.items()}}
     result = {}
     for n in sizes:
         logn = n.bit_length() - 1
@@ -778,6 +808,7 @@ def autotune_batch(ctx, ns, count=16, reps=5, path=None, stream=None, coefficien
     ctx.set_batch_parts(best_p)
     if path is not None:
         _save_schedule(path, "batch_outer", {n: r["outer"] for n, r in result.items()})
+        _save_schedule(path, "batch_residues_per_thread", {t: r["residues"] for t, r in tile_result.items()})
         _save_schedule(path, "batch_tiles_per_block", {t: r["tiles"] for t, r in tile_result.items()})
         with open(path) as f:
             doc = json.load(f)

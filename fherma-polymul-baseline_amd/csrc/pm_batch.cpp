@@ -80,7 +80,7 @@ static int upload_worklist(pm_context* ctx, pm_batch* b, const pmd::Job* d_jobs,
     const uint32_t k = b->tables.k;  // residue rows of the batch's backend
     std::vector<pmd::Item> items;
     std::vector<pmd::Unit> units;
-    pm_build_worklist(k, b->jobs.data() + first, count, true, items, units, ctx->batch_tiles);
+    pm_build_worklist(k, b->jobs.data() + first, count, true, items, units, ctx->batch_tiles, ctx->batch_rpt);
     if (!pm_phase_ranges(items, ranges)) return pm_fail(ctx, PM_EINVAL, "work list phases are not contiguous");
     if (units.empty()) units.push_back({0, 0});
     auto* d_items = static_cast<pmd::Item*>(b->alloc(items.size() * sizeof(pmd::Item)));
@@ -260,9 +260,14 @@ bool pm_phase_ranges(const std::vector<pmd::Item>& items, pmd::PhaseRanges& r) {
             case pmd::kItemConv: ok = add(i, r.conv_first, r.conv_count); break;
             case pmd::kItemOuterFwd: ok = add(i, r.fwd_first, r.fwd_count); break;
             case pmd::kItemTile: {
-                const uint32_t tlog = it.a0 & 0xFFu, units = it.a0 >> 8;
-                ok = tlog >= 5 && tlog <= 15 && units >= 1 && (units << tlog) <= 32768u &&
-                     add(i, r.tile_first[tlog], r.tile_count[tlog]);
+                const uint32_t tlog = it.a0 & 0xFu, units = it.a0 >> 8;
+                const uint32_t rpt16 = (it.a0 & pmd::kItemRpt16) ? 1u : 0u;
+                const bool first_of_class = r.tile_count[tlog] == 0;
+                // a block holds at most 1024 threads: 2^15 residues at 32 per thread, 2^14 at 16
+                ok = tlog >= 5 && tlog <= 15 && units >= 1 && (units << tlog) <= (rpt16 ? 16384u : 32768u) &&
+                     (!rpt16 || (tlog >= uint32_t(pmd::kRpt16MinLog) && tlog <= uint32_t(pmd::kRpt16MaxLog))) &&
+                     (first_of_class || r.tile_rpt16[tlog] == rpt16) && add(i, r.tile_first[tlog], r.tile_count[tlog]);
+                if (ok) r.tile_rpt16[tlog] = rpt16;
                 if (ok && units > r.tile_units[tlog]) r.tile_units[tlog] = units;
                 break;
             }
@@ -277,7 +282,7 @@ bool pm_phase_ranges(const std::vector<pmd::Item>& items, pmd::PhaseRanges& r) {
 
 void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool largest_first,
                        std::vector<pmd::Item>& items, std::vector<pmd::Unit>& units,
-                       const uint32_t* tiles_per_block) {
+                       const uint32_t* tiles_per_block, const uint32_t* residues_per_thread) {
     // Order: conversions, outer forward stages, fused tiles by size class,
     // outer inverse stages, CRT blocks.  Every dependency of an item points to
     // an earlier item, so claiming items in order cannot deadlock.
@@ -308,9 +313,10 @@ void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool la
                 if (jobs[j].logn - jobs[j].outer != tlog) continue;
                 for (uint32_t s = 0; s < (1u << jobs[j].outer); ++s) cls.push_back({j, (prime << 2) | s});
             }
+        const uint32_t rpt16 = residues_per_thread && residues_per_thread[tlog] == 16 ? pmd::kItemRpt16 : 0u;
         for (size_t first = 0; first < cls.size(); first += per_item) {
             const uint32_t n_units = uint32_t(std::min<size_t>(per_item, cls.size() - first));
-            items.push_back({pmd::kItemTile, 0, tlog | (n_units << 8), uint32_t(units.size())});
+            items.push_back({pmd::kItemTile, 0, tlog | rpt16 | (n_units << 8), uint32_t(units.size())});
             units.insert(units.end(), cls.begin() + first, cls.begin() + first + n_units);
         }
     }

@@ -14,10 +14,11 @@ struct pm_context;
 // the serving loop).
 bool pm_phase_ranges(const std::vector<pmd::Item>& items, pmd::PhaseRanges& ranges);
 // tiles_per_block[tlog]: tiles (units) per tile item of each size class,
-// NULL for the default (a 1024-thread block: 2^(15 - tlog) tiles)
+// NULL for the default (pm_default_batch_tiles); residues_per_thread[tlog]:
+// 16 or 32 (NULL, 0: 32)
 void pm_build_worklist(uint32_t k, const pmd::Job* jobs, uint32_t count, bool largest_first,
                        std::vector<pmd::Item>& items, std::vector<pmd::Unit>& units,
-                       const uint32_t* tiles_per_block = nullptr);
+                       const uint32_t* tiles_per_block = nullptr, const uint32_t* residues_per_thread = nullptr);
 
 struct pm_batch {
     pm_context* ctx = nullptr;

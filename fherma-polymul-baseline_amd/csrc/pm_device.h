@@ -109,7 +109,7 @@ enum ItemType : uint32_t { kItemConv = 0, kItemOuterFwd = 1, kItemTile = 2, kIte
 struct Item {
     uint32_t type;
     uint32_t job;   // product (tile items: unused)
-    uint32_t a0;    // conv: poly; outer fwd: poly*k+prime; tile: tlog | count<<8; outer inv: prime; crt: block
+    uint32_t a0;    // conv: poly; outer fwd: poly*k+prime; tile: tlog | kItemRpt16? | count<<8; outer inv: prime; crt: block
     uint32_t a1;    // conv: coefficient block; tile: first unit
 };
 
@@ -126,6 +126,13 @@ struct BatchArgs {
 };
 
 constexpr uint32_t kErrTimeout = 2;
+
+// Batch tiles at 16 residues per thread (a schedule choice beside the default
+// 32) are compiled for tile sizes 2^6 .. 2^14; a block holds at most 1024
+// threads, so 2^14 residues.  A tile item's a0 carries the choice in bit 4.
+constexpr int kRpt16MinLog = 6;
+constexpr int kRpt16MaxLog = 14;
+constexpr uint32_t kItemRpt16 = 1u << 4;
 
 // Persistent batch (one launch for the whole work list): one block of 512
 // threads per compute unit; tiles up to 2^14 residues (the block's 16384

@@ -46,11 +46,13 @@ struct PhaseRanges {
     uint32_t fwd_first = 0, fwd_count = 0;
     uint32_t tile_first[16] = {}, tile_count[16] = {};
     uint32_t tile_units[16] = {};  // most units (tiles) in one item of the class: tiles per block
+    uint32_t tile_rpt16[16] = {};  // 1: the class runs 16 residues per thread (else 32)
     uint32_t inv_first = 0, inv_count = 0;
     uint32_t crt_first = 0, crt_count = 0;
 };
-// phase kinds (tile phases carry the tile size class and log2 of the tiles
-// per block: kPhaseTile | tlog << 8 | log2(tiles) << 16)
+// phase kinds (tile phases carry the tile size class, log2 of the tiles
+// per block and 16 residues per thread: kPhaseTile | tlog << 8 |
+// log2(tiles) << 16 | rpt16 << 20)
 enum PhaseKind : uint32_t {
     kPhaseConv = 0, kPhaseOuterFwd = 1, kPhaseTile = 2, kPhaseOuterInv = 3, kPhaseCrt = 4, kPhaseConvScalar = 5,
     kPhaseCrtScalar = 6

@@ -119,21 +119,22 @@ __host__ __device__ constexpr uint32_t tile_block_threads(int tlog) {
 // and register).  Both forms are additive over disjoint bit sets, so a
 // thread's base address plus a register's constant offset still gives the
 // slot.  (For 2^11 and 2^12 no form with two such terms is conflict-free.)
-template <int TLOG>
+template <int TLOG, int EL>
 __host__ __device__ constexpr uint32_t lds_pad(uint32_t x) {
-    if constexpr (TLOG == 13) return x + ((x >> 5) << 1) + (x >> 9);
+    if constexpr (TLOG == 13 && EL == 5) return x + ((x >> 5) << 1) + (x >> 9);
     else return x + (x >> 5);
 }
 
-template <int TLOG>
+template <int TLOG, int EL = tile_elog(TLOG)>
 struct Geo {
-    static constexpr int ELOG = tile_elog(TLOG);
+    static_assert(EL >= 1 && EL <= 5 && EL <= TLOG, "residues per thread: 2^1 .. 2^5, at most the tile");
+    static constexpr int ELOG = EL;
     static constexpr int E = 1 << ELOG;
     static constexpr int TPTLOG = TLOG - ELOG;
     static constexpr int TPT = 1 << TPTLOG;
     static constexpr int NPH = (TLOG + ELOG - 1) / ELOG;
     static constexpr int T = 1 << TLOG;
-    static constexpr int STRIDE = int(lds_pad<TLOG>(uint32_t(T - 1))) + 1;  // LDS words of one tile
+    static constexpr int STRIDE = int(lds_pad<TLOG, EL>(uint32_t(T - 1))) + 1;  // LDS words of one tile
     static constexpr int width(int p) { return TLOG / NPH + (p < TLOG % NPH ? 1 : 0); }
     static constexpr int lo(int p) {
         int h = TLOG;
@@ -173,9 +174,13 @@ struct Geo {
     }
 };
 
-template <int TLOG, int P>
+// a thread's registers of one operand of a tile
+template <int TLOG, int EL>
+using TileRegs = uint32_t[Geo<TLOG, EL>::E];
+
+template <int TLOG, int EL, int P>
 __device__ __forceinline__ uint32_t thr_scatter(uint32_t tau) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     uint32_t x = 0;
 #pragma unroll
     for (int j = 0; j < G::TPTLOG; ++j) x |= ((tau >> j) & 1u) << G::thr_pos(P, j);
@@ -187,18 +192,18 @@ constexpr uint32_t kLdsHalf = 16384;  // words: 64 KB, the reach of an LDS immed
 // Butterflies between scheduling barriers, which bound the twiddles in flight
 // (VGPR budget of 1024-thread blocks): measured best at 16 for the 2^15 tile,
 // while the smaller tiles spill with more than 8.
-template <int TLOG>
-constexpr int twiddle_chunk() { return TLOG == 15 ? 16 : 8; }
-template <int TLOG>
-constexpr int inverse_twiddle_chunk() { return 2 * twiddle_chunk<TLOG>(); }
+template <int TLOG, int EL>
+constexpr int twiddle_chunk() { return TLOG == 15 && EL == 5 ? 16 : 8; }
+template <int TLOG, int EL>
+constexpr int inverse_twiddle_chunk() { return 2 * twiddle_chunk<TLOG, EL>(); }
 
 // An exchange keeps every element inside its wave when the wave bits of the
 // thread index (thread bits 6 and up) sit on the same index bits in both
 // phases; then no block barrier is needed: LDS instructions of one wave
 // execute in order, so its reads see its own writes.
-template <int TLOG>
+template <int TLOG, int EL>
 __host__ __device__ constexpr bool wave_local(int pfrom, int pto) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     for (int j = 6; j < G::TPTLOG; ++j)
         if (G::thr_pos(pfrom, j) != G::thr_pos(pto, j)) return false;
     return true;
@@ -206,17 +211,18 @@ __host__ __device__ constexpr bool wave_local(int pfrom, int pto) {
 
 // LEAD: a wave-local exchange came before this block-wide one, so other waves
 // may still be reading their part of the buffer.
-template <int TLOG, int PFROM, int PTO, bool LEAD>
-__device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* lds, uint32_t tau) {
-    using G = Geo<TLOG>;
-    constexpr bool kWave = wave_local<TLOG>(PFROM, PTO);
+template <int TLOG, int EL, int PFROM, int PTO, bool LEAD>
+__device__ __forceinline__ void exchange(TileRegs<TLOG, EL>& v, uint32_t* lds, uint32_t tau) {
+    using G = Geo<TLOG, EL>;
+    constexpr bool kWave = wave_local<TLOG, EL>(PFROM, PTO);
     // thread and register bits are disjoint, so lds_pad(x|c) = lds_pad(x) + lds_pad(c):
     // one base address per thread plus immediate offsets.  An LDS immediate
     // offset is below 64 KB, so slots above that use a second base 64 KB up,
     // formed in this exchange (opaque to the optimiser): left to itself the
     // compiler gives each such slot an address register of its own, shared
     // by every exchange of the same layout, and spills them across the tile.
-    const uint32_t wx = lds_pad<TLOG>(thr_scatter<TLOG, PFROM>(tau)), rx = lds_pad<TLOG>(thr_scatter<TLOG, PTO>(tau));
+    const uint32_t wx = lds_pad<TLOG, EL>(thr_scatter<TLOG, EL, PFROM>(tau)),
+                   rx = lds_pad<TLOG, EL>(thr_scatter<TLOG, EL, PTO>(tau));
     uint32_t wx_hi = wx + kLdsHalf, rx_hi = rx + kLdsHalf;
     if constexpr (G::STRIDE > kLdsHalf) asm volatile("" : "+v"(wx_hi), "+v"(rx_hi));
     const auto slot = [&](uint32_t x, uint32_t x_hi, uint32_t c) -> uint32_t& {
@@ -224,7 +230,7 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* 
     };
     if constexpr (LEAD && !kWave) __syncthreads();
 #pragma unroll
-    for (int r = 0; r < G::E; ++r) slot(wx, wx_hi, lds_pad<TLOG>(G::reg_scatter(PFROM, r))) = v[r];
+    for (int r = 0; r < G::E; ++r) slot(wx, wx_hi, lds_pad<TLOG, EL>(G::reg_scatter(PFROM, r))) = v[r];
     if constexpr (kWave) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -233,7 +239,7 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[Geo<TLOG>::E], uint32_t* 
         __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < G::E; ++r) v[r] = slot(rx, rx_hi, lds_pad<TLOG>(G::reg_scatter(PTO, r)));
+    for (int r = 0; r < G::E; ++r) v[r] = slot(rx, rx_hi, lds_pad<TLOG, EL>(G::reg_scatter(PTO, r)));
     if constexpr (kWave) {
         __builtin_amdgcn_wave_barrier();
     } else {
@@ -254,9 +260,9 @@ __device__ __forceinline__ Tw const_twiddle(const Tw* tw, uint32_t i) {
 // Group index (index bits above beta) of register r in phase P.  When every
 // thread bit of the phase lies below beta+1 the group -- hence the twiddle --
 // is the same for all threads, which lets the compiler use scalar loads.
-template <int TLOG, int P>
+template <int TLOG, int EL, int P>
 __device__ __forceinline__ uint32_t group_of(uint32_t tau_x, int r, int beta) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     bool thread_bits_above = false;
     for (int j = 0; j < G::TPTLOG; ++j)
         if (G::thr_pos(P, j) > beta) thread_bits_above = true;
@@ -268,11 +274,11 @@ __device__ __forceinline__ uint32_t group_of(uint32_t tau_x, int r, int beta) {
 // Forward CT stages of phase P on two polynomials that share twiddles.
 // Twiddle of local stage (m groups, group g) is table[bm*m + g] with
 // bm = 2^u + s for sub-block s of a transform with u outer stages.
-template <int TLOG, int P>
-__device__ __forceinline__ void fwd_phase2(uint32_t (&a)[Geo<TLOG>::E], uint32_t (&b)[Geo<TLOG>::E],
+template <int TLOG, int EL, int P>
+__device__ __forceinline__ void fwd_phase2(TileRegs<TLOG, EL>& a, TileRegs<TLOG, EL>& b,
                                            const Tw* __restrict__ tw, uint32_t bm, uint32_t tau_x,
                                            uint32_t p, uint32_t p2) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
 #pragma unroll
     for (int bit = G::width(P) - 1; bit >= 0; --bit) {
         const int beta = G::lo(P) + bit;
@@ -281,18 +287,18 @@ __device__ __forceinline__ void fwd_phase2(uint32_t (&a)[Geo<TLOG>::E], uint32_t
 #pragma unroll
         for (int r = 0; r < G::E; ++r) {
             if (r & (1 << bit)) continue;
-            const Tw w = const_twiddle(tw, bm * m + group_of<TLOG, P>(tau_x, r, beta));
+            const Tw w = const_twiddle(tw, bm * m + group_of<TLOG, EL, P>(tau_x, r, beta));
             ct_bfly(a[r], a[r | (1 << bit)], w, p, p2);
             ct_bfly(b[r], b[r | (1 << bit)], w, p, p2);
-            if (++done % twiddle_chunk<TLOG>() == 0) __builtin_amdgcn_sched_barrier(0);
+            if (++done % twiddle_chunk<TLOG, EL>() == 0) __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
 
-template <int TLOG, int P>
-__device__ __forceinline__ void inv_phase(uint32_t (&a)[Geo<TLOG>::E], const Tw* __restrict__ tw,
+template <int TLOG, int EL, int P>
+__device__ __forceinline__ void inv_phase(TileRegs<TLOG, EL>& a, const Tw* __restrict__ tw,
                                           uint32_t bm, uint32_t tau_x, uint32_t p, uint32_t p2) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     // the inverse table is indexed like the forward one: without this the
     // compiler keeps the forward phase's twiddle offsets live across the tile
     // (and spills them) instead of recomputing them here
@@ -308,50 +314,51 @@ __device__ __forceinline__ void inv_phase(uint32_t (&a)[Geo<TLOG>::E], const Tw*
 #pragma unroll
         for (int r = 0; r < G::E; ++r) {
             if (r & (1 << bit)) continue;
-            const Tw w = const_twiddle(tw, bm * m + group_of<TLOG, P>(tau_x, r, beta));
+            const Tw w = const_twiddle(tw, bm * m + group_of<TLOG, EL, P>(tau_x, r, beta));
             gs_bfly(a[r], a[r | (1 << bit)], w, p, p2);
-            if (++done % inverse_twiddle_chunk<TLOG>() == 0) __builtin_amdgcn_sched_barrier(0);
+            if (++done % inverse_twiddle_chunk<TLOG, EL>() == 0) __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
 
-template <int TLOG, int P>
-__device__ __forceinline__ void forward_from(uint32_t (&a)[Geo<TLOG>::E], uint32_t (&b)[Geo<TLOG>::E],
+template <int TLOG, int EL, int P>
+__device__ __forceinline__ void forward_from(TileRegs<TLOG, EL>& a, TileRegs<TLOG, EL>& b,
                                              uint32_t* lds, uint32_t tau, const Tw* tw, uint32_t bm,
                                              uint32_t p, uint32_t p2) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     if constexpr (P > 0) {
-        constexpr bool lead = P >= 2 && wave_local<TLOG>(P - 2, P - 1);
-        exchange<TLOG, P - 1, P, lead>(a, lds, tau);
-        exchange<TLOG, P - 1, P, false>(b, lds, tau);
+        constexpr bool lead = P >= 2 && wave_local<TLOG, EL>(P - 2, P - 1);
+        exchange<TLOG, EL, P - 1, P, lead>(a, lds, tau);
+        exchange<TLOG, EL, P - 1, P, false>(b, lds, tau);
     }
-    fwd_phase2<TLOG, P>(a, b, tw, bm, thr_scatter<TLOG, P>(tau), p, p2);
-    if constexpr (P + 1 < G::NPH) forward_from<TLOG, P + 1>(a, b, lds, tau, tw, bm, p, p2);
+    fwd_phase2<TLOG, EL, P>(a, b, tw, bm, thr_scatter<TLOG, EL, P>(tau), p, p2);
+    if constexpr (P + 1 < G::NPH) forward_from<TLOG, EL, P + 1>(a, b, lds, tau, tw, bm, p, p2);
 }
 
-template <int TLOG, int P>
-__device__ __forceinline__ void inverse_from(uint32_t (&a)[Geo<TLOG>::E], uint32_t* lds, uint32_t tau,
+template <int TLOG, int EL, int P>
+__device__ __forceinline__ void inverse_from(TileRegs<TLOG, EL>& a, uint32_t* lds, uint32_t tau,
                                              const Tw* tw, uint32_t bm, uint32_t p, uint32_t p2) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     if constexpr (P + 1 < G::NPH) {
         // the exchange before this one: the inverse step above, or the last
         // forward step when this is the first inverse exchange
-        constexpr bool lead = P + 2 < G::NPH ? wave_local<TLOG>(P + 2, P + 1) : wave_local<TLOG>(P, P + 1);
-        exchange<TLOG, P + 1, P, lead>(a, lds, tau);
+        constexpr bool lead =
+            P + 2 < G::NPH ? wave_local<TLOG, EL>(P + 2, P + 1) : wave_local<TLOG, EL>(P, P + 1);
+        exchange<TLOG, EL, P + 1, P, lead>(a, lds, tau);
     }
-    inv_phase<TLOG, P>(a, tw, bm, thr_scatter<TLOG, P>(tau), p, p2);
-    if constexpr (P > 0) inverse_from<TLOG, P - 1>(a, lds, tau, tw, bm, p, p2);
+    inv_phase<TLOG, EL, P>(a, tw, bm, thr_scatter<TLOG, EL, P>(tau), p, p2);
+    if constexpr (P > 0) inverse_from<TLOG, EL, P - 1>(a, lds, tau, tw, bm, p, p2);
 }
 
 // One tile: forward(a), forward(b), pointwise, inverse, scale to [0,p).
 // `valid` false means this thread's tile is padding: it joins every barrier
 // but touches no global memory.
-template <int TLOG>
+template <int TLOG, int EL = tile_elog(TLOG)>
 __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __restrict__ a_src,
                                           const uint32_t* __restrict__ b_src, uint32_t* __restrict__ dst,
                                           uint32_t prime, uint32_t bm, uint32_t logn_full,
                                           uint32_t* lds, uint32_t tau, bool valid) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     uint32_t pi = valid ? prime : 0u;
     if constexpr (G::TPTLOG >= 6) {
         // a tile that spans whole waves has one prime and sub-block per wave:
@@ -362,7 +369,7 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
     const uint32_t p = tb.primes[pi], p2 = 2u * p, neg_inv = tb.mont[pi];
     const size_t table = size_t(pi) << tb.max_logn;
     uint32_t a[G::E], b[G::E];
-    const uint32_t x0 = thr_scatter<TLOG, 0>(tau);
+    const uint32_t x0 = thr_scatter<TLOG, EL, 0>(tau);
     // Loads issued in the order the first forward stage pairs them (register r
     // with r + E/2, a then b), so that stage starts as the first loads land.
     // Unconditional (a padding tile points at a real tile of the same size and
@@ -394,10 +401,10 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
             b[r1] = b_src[x0 + G::reg_scatter(0, r1)];
         }
     }
-    forward_from<TLOG, 0>(a, b, lds, tau, tb.fwd + table, bm, p, p2);
+    forward_from<TLOG, EL, 0>(a, b, lds, tau, tb.fwd + table, bm, p, p2);
 #pragma unroll
     for (int r = 0; r < G::E; ++r) a[r] = mont_mul(csub(a[r], p2), csub(b[r], p2), p, neg_inv);
-    inverse_from<TLOG, G::NPH - 1>(a, lds, tau, tb.inv + table, bm, p, p2);
+    inverse_from<TLOG, EL, G::NPH - 1>(a, lds, tau, tb.inv + table, bm, p, p2);
     const Tw s = tb.scale[logn_full * tb.k + pi];
     const __amdgpu_buffer_rsrc_t rd =
         __builtin_amdgcn_make_buffer_rsrc(wave_uniform(dst), short(0), int(G::T * 4u), int(kRawBufferWord3));
@@ -1326,9 +1333,9 @@ __device__ __forceinline__ void outer_inverse_columns(const Tables& tb, uint32_t
 // ---------------------------------------------------------------------------
 // batch work-list items
 // ---------------------------------------------------------------------------
-template <int TLOG>
+template <int TLOG, int EL = tile_elog(TLOG)>
 __device__ __forceinline__ void tile_item(const Tables& tb, const BatchArgs& ba, const Item& it, uint32_t* lds) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     const uint32_t tile = threadIdx.x >> G::TPTLOG, tau = threadIdx.x & (G::TPT - 1);
     const uint32_t count = it.a0 >> 8;
     const bool valid = tile < count;
@@ -1336,8 +1343,8 @@ __device__ __forceinline__ void tile_item(const Tables& tb, const BatchArgs& ba,
     const Job job = ba.jobs[unit.job];
     const uint32_t prime = unit.prime_s >> 2, s = unit.prime_s & 3u;
     const size_t off = (size_t(prime) << job.logn) + (size_t(s) << TLOG);
-    conv_tile<TLOG>(tb, job.ra + off, job.rb + off, job.ra + off, prime, (1u << job.outer) + s, job.logn,
-                    lds + tile * G::STRIDE, tau, valid);
+    conv_tile<TLOG, EL>(tb, job.ra + off, job.rb + off, job.ra + off, prime, (1u << job.outer) + s, job.logn,
+                        lds + tile * G::STRIDE, tau, valid);
 }
 
 // Outer forward CT stages (bits logn-1 .. logn-u) for a and b: one thread per
@@ -1468,11 +1475,11 @@ __global__ void __launch_bounds__(kBlock) k_items_outer(Tables tb, BatchArgs ba,
     }
 }
 
-template <int TLOG>
+template <int TLOG, int EL>
 __global__ void __launch_bounds__(tile_block_threads(TLOG)) k_items_tile(Tables tb, BatchArgs ba, uint32_t first) {
     extern __shared__ uint32_t lds[];
     const Item it = ba.items[first + blockIdx.x];
-    tile_item<TLOG>(tb, ba, it, lds);
+    tile_item<TLOG, EL>(tb, ba, it, lds);
 }
 
 // One group of 64 coefficients of the CRT: residues t ([k][n]), output, its
@@ -1934,15 +1941,25 @@ hipError_t launch_ntt(const Tables& tb, const Job& job, uint32_t u, hipStream_t 
 // blocks of `tiles` tiles (whole waves; padding tiles, if any, store nothing):
 // fewer tiles per block leave room for several blocks per CU, so one block's
 // loads and barriers overlap another's arithmetic
-template <int TLOG>
+template <int TLOG, int EL>
 static hipError_t launch_items_tile(const Tables& tb, const BatchArgs& ba, uint32_t first, uint32_t count,
                                     uint32_t tiles, hipStream_t stream) {
-    using G = Geo<TLOG>;
+    using G = Geo<TLOG, EL>;
     while ((tiles << G::TPTLOG) < kWaveSize) tiles <<= 1;
     if ((tiles << G::TPTLOG) > tile_block_threads(TLOG)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_items_tile<TLOG>, dim3(count), dim3(tiles << G::TPTLOG),
+    hipLaunchKernelGGL((k_items_tile<TLOG, EL>), dim3(count), dim3(tiles << G::TPTLOG),
                        size_t(tiles) * G::STRIDE * sizeof(uint32_t), stream, tb, ba, first);
     return hipSuccess;
+}
+
+// a tile class at 32 residues per thread, or 16 (kind bit 20) where compiled
+template <int TLOG>
+static hipError_t launch_items_tile_rpt(const Tables& tb, const BatchArgs& ba, uint32_t first, uint32_t count,
+                                        uint32_t tiles, bool sixteen, hipStream_t stream) {
+    if constexpr (TLOG >= kRpt16MinLog && TLOG <= kRpt16MaxLog)
+        if (sixteen) return launch_items_tile<TLOG, 4>(tb, ba, first, count, tiles, stream);
+    if (sixteen) return hipErrorInvalidValue;
+    return launch_items_tile<TLOG, tile_elog(TLOG)>(tb, ba, first, count, tiles, stream);
 }
 
 // W-wave CRT blocks, as many as the LDS lets share a CU (two of 2 waves,
@@ -2005,19 +2022,20 @@ hipError_t launch_phase(const Tables& tb, const BatchArgs& ba, uint32_t kind, ui
             break;
         case kPhaseTile: {
             const uint32_t tiles = 1u << ((kind >> 16) & 0xFu);
+            const bool r16 = ((kind >> 20) & 1u) != 0;
             hipError_t e = hipSuccess;
             switch ((kind >> 8) & 0xFFu) {
-                case 5: e = launch_items_tile<5>(tb, ba, first, count, tiles, stream); break;
-                case 6: e = launch_items_tile<6>(tb, ba, first, count, tiles, stream); break;
-                case 7: e = launch_items_tile<7>(tb, ba, first, count, tiles, stream); break;
-                case 8: e = launch_items_tile<8>(tb, ba, first, count, tiles, stream); break;
-                case 9: e = launch_items_tile<9>(tb, ba, first, count, tiles, stream); break;
-                case 10: e = launch_items_tile<10>(tb, ba, first, count, tiles, stream); break;
-                case 11: e = launch_items_tile<11>(tb, ba, first, count, tiles, stream); break;
-                case 12: e = launch_items_tile<12>(tb, ba, first, count, tiles, stream); break;
-                case 13: e = launch_items_tile<13>(tb, ba, first, count, tiles, stream); break;
-                case 14: e = launch_items_tile<14>(tb, ba, first, count, tiles, stream); break;
-                case 15: e = launch_items_tile<15>(tb, ba, first, count, tiles, stream); break;
+                case 5: e = launch_items_tile_rpt<5>(tb, ba, first, count, tiles, r16, stream); break;
+                case 6: e = launch_items_tile_rpt<6>(tb, ba, first, count, tiles, r16, stream); break;
+                case 7: e = launch_items_tile_rpt<7>(tb, ba, first, count, tiles, r16, stream); break;
+                case 8: e = launch_items_tile_rpt<8>(tb, ba, first, count, tiles, r16, stream); break;
+                case 9: e = launch_items_tile_rpt<9>(tb, ba, first, count, tiles, r16, stream); break;
+                case 10: e = launch_items_tile_rpt<10>(tb, ba, first, count, tiles, r16, stream); break;
+                case 11: e = launch_items_tile_rpt<11>(tb, ba, first, count, tiles, r16, stream); break;
+                case 12: e = launch_items_tile_rpt<12>(tb, ba, first, count, tiles, r16, stream); break;
+                case 13: e = launch_items_tile_rpt<13>(tb, ba, first, count, tiles, r16, stream); break;
+                case 14: e = launch_items_tile_rpt<14>(tb, ba, first, count, tiles, r16, stream); break;
+                case 15: e = launch_items_tile_rpt<15>(tb, ba, first, count, tiles, r16, stream); break;
                 default: return hipErrorInvalidValue;
             }
             if (e != hipSuccess) return e;
@@ -2039,7 +2057,8 @@ std::vector<Phase> phase_list(const PhaseRanges& r, bool scalar_conversion, bool
         if (!r.tile_count[tlog]) continue;
         uint32_t lg = 0;  // tiles per block: the class's largest item, rounded up to a power of two
         while ((1u << lg) < r.tile_units[tlog]) ++lg;
-        out.push_back({kPhaseTile | (tlog << 8) | (lg << 16), r.tile_first[tlog], r.tile_count[tlog]});
+        out.push_back({kPhaseTile | (tlog << 8) | (lg << 16) | ((r.tile_rpt16[tlog] ? 1u : 0u) << 20),
+                       r.tile_first[tlog], r.tile_count[tlog]});
     }
     if (r.inv_count) out.push_back({kPhaseOuterInv, r.inv_first, r.inv_count});
     if (r.crt_count)
@@ -2156,9 +2175,15 @@ static hipError_t allow_lds() {
 }
 template <int TLOG>
 static hipError_t allow_items_lds() {
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_tile<TLOG>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(kTileLdsMaxWords * sizeof(uint32_t)));
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_tile<TLOG, tile_elog(TLOG)>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(kTileLdsMaxWords * sizeof(uint32_t)));
+    if constexpr (TLOG >= kRpt16MinLog && TLOG <= kRpt16MaxLog)
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_items_tile<TLOG, 4>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(kTileLdsMaxWords * sizeof(uint32_t)));
+    return e;
 }
 
 hipError_t configure_kernels() {

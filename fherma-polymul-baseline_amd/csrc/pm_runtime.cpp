@@ -88,9 +88,13 @@ uint32_t pm_default_batch_outer(uint32_t logn, bool quartic) {
 
 // one tile per block from 2^13 up (2^15: the whole 1024-thread block);
 // smaller tiles fill a 1024-thread block
-uint32_t pm_default_batch_tiles(uint32_t tlog) {
-    if (tlog >= 13 && tlog <= 15) return 1u;
-    return tlog >= 5 && tlog < 13 ? 1u << (15 - tlog) : 1u;
+uint32_t pm_default_batch_tiles(uint32_t tlog, uint32_t residues_per_thread) {
+    // 32 per thread: one tile per block from 2^13 up (256 threads or more),
+    // 1024-thread blocks below; 16 per thread: the same from 2^12 up
+    const uint32_t one_from = residues_per_thread == 16 ? 12u : 13u;
+    const uint32_t block_log = residues_per_thread == 16 ? 14u : 15u;  // residues of a 1024-thread block
+    if (tlog >= one_from && tlog <= 15) return 1u;
+    return tlog >= 5 && tlog < one_from ? 1u << (block_log - tlog) : 1u;
 }
 
 bool pm_outer_valid(uint32_t logn, uint32_t outer, uint32_t lo) {
@@ -282,7 +286,10 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
         ctx->batch_outer[PM_BACKEND_GENERAL][logn] = pm_default_batch_outer(logn, false);
         ctx->batch_outer[PM_BACKEND_QUARTIC][logn] = pm_default_batch_outer(logn, true);
     }
-    for (uint32_t tlog = 0; tlog < 16; ++tlog) ctx->batch_tiles[tlog] = pm_default_batch_tiles(tlog);
+    for (uint32_t tlog = 0; tlog < 16; ++tlog) {
+        ctx->batch_tiles[tlog] = pm_default_batch_tiles(tlog);
+        ctx->batch_rpt[tlog] = 32;
+    }
     // one product of size max_n on zero inputs: the runtime loads each kernel
     // at its first launch, which would otherwise land in the first timed call
     {
@@ -805,12 +812,34 @@ int pm_set_batch_tiles_per_block(pm_context* ctx, uint32_t tile_n, uint32_t tile
     if (!ctx) return PM_EINVAL;
     if (tile_n < 32 || tile_n > 32768 || (tile_n & (tile_n - 1)))
         return pm_fail(ctx, PM_EINVAL, "tile size must be a power of two in [32, 32768]");
-    const uint32_t tlog = uint32_t(__builtin_ctz(tile_n));
-    if (tiles == 0) tiles = pm_default_batch_tiles(tlog);
-    // a block holds 32 residues per thread, at most 1024 threads, whole waves
-    if ((tiles & (tiles - 1)) || uint64_t(tiles) * tile_n > 32768u || uint64_t(tiles) * tile_n < 2048u)
-        return pm_fail(ctx, PM_EINVAL, "tiles per block: a power of two with 2048 <= tiles x tile size <= 32768");
+    const uint32_t tlog = uint32_t(__builtin_ctz(tile_n)), rpt = ctx->batch_rpt[tlog];
+    if (tiles == 0) tiles = pm_default_batch_tiles(tlog, rpt);
+    // a block holds rpt residues per thread, at most 1024 threads, whole waves
+    if ((tiles & (tiles - 1)) || uint64_t(tiles) * tile_n > 1024u * rpt || uint64_t(tiles) * tile_n < 64u * rpt)
+        return pm_fail(ctx, PM_EINVAL, "tiles per block: a power of two with 64 x r <= tiles x tile size <= 1024 x r "
+                                       "(r residues per thread)");
     ctx->batch_tiles[tlog] = tiles;
+    return PM_OK;
+}
+
+int pm_set_batch_residues_per_thread(pm_context* ctx, uint32_t tile_n, uint32_t residues) {
+    if (!ctx) return PM_EINVAL;
+    if (tile_n < 32 || tile_n > 32768 || (tile_n & (tile_n - 1)))
+        return pm_fail(ctx, PM_EINVAL, "tile size must be a power of two in [32, 32768]");
+    const uint32_t tlog = uint32_t(__builtin_ctz(tile_n));
+    if (residues == 0) residues = 32;
+    if (residues != 32 && residues != 16)
+        return pm_fail(ctx, PM_EINVAL, "residues per thread: 16 or 32");
+    if (residues == 16 && (tlog < uint32_t(pmd::kRpt16MinLog) || tlog > uint32_t(pmd::kRpt16MaxLog)))
+        return pm_fail(ctx, PM_EUNSUPPORTED, "16 residues per thread: tiles of 2^6 .. 2^14");
+    ctx->batch_rpt[tlog] = residues;
+    ctx->batch_tiles[tlog] = pm_default_batch_tiles(tlog, residues);  // keep the pair valid
+    return PM_OK;
+}
+
+int pm_get_batch_residues_per_thread(pm_context* ctx, uint32_t tile_n, uint32_t* residues) {
+    if (!ctx || !residues || tile_n < 32 || tile_n > 32768 || (tile_n & (tile_n - 1))) return PM_EINVAL;
+    *residues = ctx->batch_rpt[__builtin_ctz(tile_n)];
     return PM_OK;
 }
 

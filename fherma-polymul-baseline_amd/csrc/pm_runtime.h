@@ -43,8 +43,10 @@ struct pm_context {
     // general backend runs its outer stages inside conversion and CRT
     uint32_t batch_outer[2][pm::kMaxLogN + 1] = {};
     // tiles per block of each tile size class 2^tlog in batches and served
-    // launches (blocks of tiles << (tlog - 5) threads)
+    // launches (blocks of tiles x 2^tlog / batch_rpt[tlog] threads), and the
+    // residues each thread holds (32, or 16 for 2^6 .. 2^14)
     uint32_t batch_tiles[16] = {};
+    uint32_t batch_rpt[16] = {};
     // waves per block of the batches' MFMA CRT (2, 4 or 8)
     uint32_t batch_crt_waves = pmd::kCrtMfmaWaves;
     // staggered parts of a batch's graph (1-4; 0: chosen per batch)
@@ -93,7 +95,7 @@ hipError_t pm_workspace_release(pm_context* ctx, hipStream_t stream);
 uint32_t pm_default_outer(uint32_t logn);
 uint32_t pm_default_batch_outer(uint32_t logn, bool quartic = false);
 // tiles per block that fill 1024 threads: 2^(15 - tlog)
-uint32_t pm_default_batch_tiles(uint32_t tlog);
+uint32_t pm_default_batch_tiles(uint32_t tlog, uint32_t residues_per_thread = 32);
 // whether `outer` passes leave a tile of size 2^lo..2^15 for products of 2^logn
 bool pm_outer_valid(uint32_t logn, uint32_t outer, uint32_t lo);
 uint32_t pm_io_chunks(const pm_context* ctx, uint32_t logn);

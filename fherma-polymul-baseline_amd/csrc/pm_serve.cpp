@@ -177,7 +177,7 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
         job.ra = reinterpret_cast<uint32_t*>(base + align_up(3 * n * kLimbs * 4));
         job.rb = job.ra + size_t(k) * n;
     }
-    pm_build_worklist(k, L.h_jobs, count, false, items, units, ctx->batch_tiles);
+    pm_build_worklist(k, L.h_jobs, count, false, items, units, ctx->batch_tiles, ctx->batch_rpt);
     if (items.size() > max_items || units.size() > max_units) {
         why = "work list exceeds the launch slot";
         return false;

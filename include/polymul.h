@@ -189,12 +189,24 @@ int pm_get_conversion_slices(pm_context* ctx, uint32_t n, uint32_t* slices);
  * reference's compile-time tile and primes-per-block knobs,
  * quartic/host_setup.h:11-12, quartic/solve.cu:325): tiles of size tile_n
  * (a power of two in [32, 32768]) that one block holds, in batches and
- * served launches created afterwards.  A block has tiles x tile_n / 32
- * threads (32 residues each): a power of two with 2048 <= tiles x tile_n <=
- * 32768; 0 restores the default, a 1024-thread block.  Smaller blocks let
- * several run on one compute unit.  Set by `autotune_batch`. */
+ * served launches created afterwards.  A block has tiles x tile_n / r
+ * threads (r residues each, pm_set_batch_residues_per_thread): a power of two
+ * with 64 r <= tiles x tile_n <= 1024 r; 0 restores the default (one tile
+ * per block from 2^13 up, 1024-thread blocks below, at r = 32).  Smaller
+ * blocks let several run on one compute unit.  Set by `autotune_batch`. */
 int pm_set_batch_tiles_per_block(pm_context* ctx, uint32_t tile_n, uint32_t tiles);
 int pm_get_batch_tiles_per_block(pm_context* ctx, uint32_t tile_n, uint32_t* tiles);
+/* Batch schedule (no counterpart in the reference; the role of its
+ * compile-time registers-per-thread of the fused tile, quartic/solve.cu:576 and
+ * quartic/host_setup.h:11-12): residues each thread of a tile of size tile_n
+ * holds in registers, 32 (the default; 0 selects it) or 16 for tiles of
+ * 2^6 .. 2^14 (twice the threads, half the registers, one more LDS exchange
+ * per transform).  Setting it resets the size's tiles per block to that
+ * choice's default (pm_set_batch_tiles_per_block then counts threads as tiles
+ * x tile_n / residues).  Batches and served launches created afterwards use
+ * it; the persistent mode keeps 32.  Set by `autotune_batch`. */
+int pm_set_batch_residues_per_thread(pm_context* ctx, uint32_t tile_n, uint32_t residues);
+int pm_get_batch_residues_per_thread(pm_context* ctx, uint32_t tile_n, uint32_t* residues);
 
 /* ---- stage entry points (device buffers, context stream) ------------------
  * The three kernels of the plain path, exposed so each can be checked alone.

@@ -150,16 +150,14 @@ def test_batch_rejects_small_n(pm, workload):
         pm.Batch(ctx, [16, 1024])
     with pytest.raises(pm.PolymulError):
         pm.Batch(ctx, [2048])
-    with pytest.raises(pm.PolymulError):
-        pm.Batch(ctx, [1024], mode="persistent")  # removed mode is refused, never launched
     ctx.close()
 
 
 def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
-    """Every batch schedule -- outer stages leaving a tile of 2^5..2^15, and
-    every tiles-per-block count of that tile size -- gives the golden
-    product; the batch autotuner keeps the fastest and its choice
-    round-trips through the schedule file."""
+    """Every batch schedule -- outer stages leaving a tile of 2^5..2^15, 32
+    or 16 residues per thread, and every tiles-per-block count of that tile
+    size -- gives the golden product; the batch autotuner keeps the fastest
+    and its choice round-trips through the schedule file."""
     for n, outers in ((4096, (0, 1, 2)), (32768, (0, 1, 2)), (65536, (1, 2))):
         case = [c for c in GOLDEN["cases"] if c["n"] == n and c["modulus"].startswith("seed:")][0]
         q = workload.modulus(int(case["modulus"].split(":")[1]))
@@ -167,19 +165,23 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
         a, b = workload.family(case["family"], n, q, case["seed"])
         for outer in outers:
             tile_n = n >> outer
-            for tiles in pm.tiles_per_block_choices(tile_n):
-                say("batch schedule", n, outer, tiles)
-                ctx.set_batch_outer(n, outer)
-                ctx.set_batch_tiles_per_block(tile_n, tiles)
-                assert ctx.get_batch_outer(n) == outer and ctx.get_batch_tiles_per_block(tile_n) == tiles
-                batch = pm.Batch(ctx, [n] * 3)
-                for i in range(3):
-                    batch.upload(i, a, b)
-                batch.run()
-                batch.wait()
-                for i in range(3):
-                    assert sha(batch.download(i)) == case["sha256"]["c"], (n, outer, tiles, i)
-                batch.close()
+            for residues in pm.residues_per_thread_choices(tile_n):
+                ctx.set_batch_residues_per_thread(tile_n, residues)
+                for tiles in pm.tiles_per_block_choices(tile_n, residues):
+                    say("batch schedule", n, outer, residues, tiles)
+                    ctx.set_batch_outer(n, outer)
+                    ctx.set_batch_tiles_per_block(tile_n, tiles)
+                    assert ctx.get_batch_outer(n) == outer and ctx.get_batch_tiles_per_block(tile_n) == tiles
+                    assert ctx.get_batch_residues_per_thread(tile_n) == residues
+                    batch = pm.Batch(ctx, [n] * 3)
+                    for i in range(3):
+                        batch.upload(i, a, b)
+                    batch.run()
+                    batch.wait()
+                    for i in range(3):
+                        assert sha(batch.download(i)) == case["sha256"]["c"], (n, outer, residues, tiles, i)
+                    batch.close()
+            ctx.set_batch_residues_per_thread(tile_n, 0)
             ctx.set_batch_tiles_per_block(tile_n, 0)
         # waves per block of the MFMA CRT, on the default tiles
         ctx.set_batch_outer(n, outers[-1])
@@ -223,9 +225,12 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
         assert r["outer"] == ctx.get_batch_outer(n) and r["outer"] in r["ms"]
     assert 8192 in tiles  # 2^13: the tile of 32768 with two outer stages
     for tile_n, r in tiles.items():
-        assert r["tiles"] == ctx.get_batch_tiles_per_block(tile_n) and r["tiles"] in r["ms"]
+        assert r["tiles"] == ctx.get_batch_tiles_per_block(tile_n) and f"{r['residues']}/{r['tiles']}" in r["ms"]
+        assert r["residues"] == ctx.get_batch_residues_per_thread(tile_n)
+    assert {"32/1", "16/1"} <= set(tiles[8192]["ms"])  # both residue counts tried on the 2^13 tile
     ctx.set_batch_outer(32768, 2 if res[32768]["outer"] != 2 else 0)
-    ctx.set_batch_tiles_per_block(8192, 4 if tiles[8192]["tiles"] != 4 else 1)
+    ctx.set_batch_residues_per_thread(8192, 16 if tiles[8192]["residues"] != 16 else 32)
+    ctx.set_batch_tiles_per_block(8192, 2)
     ctx.set_batch_crt_waves(2 if waves["waves"] != 2 else 4)
     ctx.set_batch_parts(4)
     applied = ctx.load_schedule(str(path))
@@ -233,6 +238,12 @@ def test_batch_schedules_agree_and_autotune(pm, workload, tmp_path):
     assert applied["batch_parts"] == parts["parts"] == ctx.get_batch_parts()
     assert applied["batch_outer"][32768] == res[32768]["outer"] == ctx.get_batch_outer(32768)
     assert applied["batch_tiles_per_block"][8192] == tiles[8192]["tiles"] == ctx.get_batch_tiles_per_block(8192)
+    assert (applied["batch_residues_per_thread"][8192] == tiles[8192]["residues"]
+            == ctx.get_batch_residues_per_thread(8192))
+    with pytest.raises(pm.PolymulError):
+        ctx.set_batch_residues_per_thread(32768, 16)  # a 2^15 tile at 16 per thread needs 2048 threads
+    with pytest.raises(pm.PolymulError):
+        ctx.set_batch_residues_per_thread(8192, 8)
     ctx.close()
 
 

@@ -43,7 +43,7 @@ def test_worklist_coverage(pm, ns):
             assert a0 * BLOCK < n
             seen_crt.add((j, a0))
         elif t == TILE:
-            tlog, count = a0 & 0xFF, a0 >> 8
+            tlog, count = a0 & 0xF, a0 >> 8
             assert 5 <= tlog <= 15 and 1 <= count <= (1 << (15 - tlog))
             for u in units[a1:a1 + count]:
                 job, ps = int(u[0]), int(u[1])
