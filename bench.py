@@ -330,7 +330,7 @@ def run_headline(args):
     return 0
 
 
-def clock_and_power(launch, seconds=1.0):
+def clock_and_power(launch, seconds=2.0):
     """Mean GFX clock (MHz, over the XCDs) and socket power (W) sampled with
     amdsmi every ~10 ms while `launch` is called back to back for about
     `seconds`; None where amdsmi is not available.  Under a sustained load the
@@ -365,7 +365,7 @@ def clock_and_power(launch, seconds=1.0):
     th.start()
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        for _ in range(10):
+        for _ in range(20):  # keep the queue full: the device never waits for the host here
             launch()
         torch.cuda.synchronize()
     stop[0] = True
