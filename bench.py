@@ -489,6 +489,7 @@ def extra_workloads(args):
                                   "conversion_gbs": sc["conversion_gbs"]},
             "bit_exact": lg["mfma_and_scalar_bit_identical"]},
         "serving": {"workload": sv["workload"], "io": sv["io"], "runs_per_setting": sv["runs_per_setting"],
+                    "pcie_bytes_per_request_mean": sv["pcie_bytes_per_request_mean"],
                     "rates": {rate: {name: {k_: v[k_] for k_ in ("products_per_s", "p50_ms", "p99_ms",
                                                                   "same_results")}
                                      for name, v in modes.items()}
@@ -744,7 +745,7 @@ def run_serve(args):
     gc.collect()
     gc.disable()
     warm = pm.Server(ctx, 64 << 30, max_batch=64, max_inflight=3)
-    gaps = rng.exponential(1.0 / args.rates[0], size=min(count, 120))
+    gaps = rng.exponential(1.0 / max(args.rates[0], 1.0), size=min(count, 120))
     t_next = time.perf_counter()
     ids = []
     for i, gap in enumerate(gaps):
@@ -758,8 +759,13 @@ def run_serve(args):
     rng = np.random.default_rng(args.seed + 1)  # the timed arrivals do not depend on the warm-up's length
     reference = None
     out["runs_per_setting"] = args.serve_runs
+    # page-locked bytes a request moves over PCIe: a and b in, c out
+    out["pcie_bytes_per_request_mean"] = round(float(np.mean([3 * n_i * 112 for n_i in ns])), 1)
     for rate in args.rates:
-        gaps = rng.exponential(1.0 / rate, size=count)
+        # rate 0: every request submitted at once (a burst): what the server
+        # sustains when the queue never empties
+        gaps = rng.exponential(1.0 / rate, size=count) if rate > 0 else np.zeros(count)
+        key = str(rate) if rate > 0 else "burst"
         for name, (max_batch, inflight) in (("continuous_batching", (64, 3)), ("one_at_a_time", (1, 1))):
             # the same request stream several times, latencies pooled: one
             # host or runtime stall of ~20 ms delays a burst of consecutive
@@ -788,14 +794,14 @@ def run_serve(args):
                 for n_i, (_, lat) in zip(ns, results):
                     lats.append(lat * 1e3)
                     by_n.setdefault(n_i, []).append(lat * 1e3)
-            out["rates"].setdefault(str(rate), {})[name] = {
+            out["rates"].setdefault(key, {})[name] = {
                 "products_per_s": round(count * args.serve_runs / wall, 1), "p50_ms": round(percentile(lats, 50), 4),
                 "p99_ms": round(percentile(lats, 99), 4),
                 "p50_p99_ms_by_n": {str(n_i): [round(percentile(v, 50), 3), round(percentile(v, 99), 3)]
                                     for n_i, v in sorted(by_n.items())},
                 "launches": launches, "max_concurrent_launches": max_conc, "peak_hbm_bytes": peak,
                 "same_results": exact}
-            print(f"[bench] serve rate {rate} {name}: {out['rates'][str(rate)][name]}", file=sys.stderr, flush=True)
+            print(f"[bench] serve rate {key} {name}: {out['rates'][key][name]}", file=sys.stderr, flush=True)
             gc.collect()  # between settings, outside the timed loops
     gc.enable()
     ctx.close()
@@ -818,7 +824,8 @@ def main():
                     help="batched launch: one launch per phase (int8 MFMA or scalar residue conversion)")
     ap.add_argument("--workload", default="headline", choices=["headline", "uneven", "large", "serve"])
     ap.add_argument("--count", type=int, default=0, help="products / requests for the extra workloads")
-    ap.add_argument("--rates", type=float, nargs="*", default=[2000.0, 8000.0])
+    ap.add_argument("--rates", type=float, nargs="*", default=[2000.0, 8000.0, 0.0],
+                    help="serving: offered requests/s (0: all requests at once)")
     ap.add_argument("--serve-runs", type=int, default=3, help="serving: runs of the request stream per setting")
     ap.add_argument("--no-extra", action="store_true",
                     help="headline only: skip the uneven, large-N and serving records")

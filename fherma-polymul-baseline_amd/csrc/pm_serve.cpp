@@ -153,12 +153,20 @@ struct pm_server {
         const size_t n = size_t(1) << logn;
         return align_up(3 * n * kLimbs * 4) + align_up(2 * size_t(tables.k) * n * 4);
     }
+    // Under a backlog the page-locked inputs of requests from 2^13
+    // coefficients (1.8 MB a polynomial) go in by DMA on the copy engines
+    // instead of the gather kernel: a kernel reading host memory and one
+    // writing it share about 63 GB/s of PCIe between them on MI355X, a
+    // copy-engine read beside the scatter kernel's writes about 85 GB/s.  A
+    // DMA call can now and then hold this thread in the runtime for some
+    // milliseconds, so a server that keeps up uses the kernel alone.
+    static constexpr uint32_t kDmaMinLogN = 13;
     void run();
-    bool start_launch(Launch& L, std::vector<Request*>& reqs, std::string& why);
+    bool start_launch(Launch& L, std::vector<Request*>& reqs, bool dma_inputs, std::string& why);
     void finish_launch(Launch& L, hipError_t status);
 };
 
-bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string& why) {
+bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, bool dma_inputs, std::string& why) {
     const uint32_t k = tables.k;
     std::vector<pmd::Item> items;
     std::vector<pmd::Unit> units;
@@ -206,10 +214,10 @@ bool pm_server::start_launch(Launch& L, std::vector<Request*>& reqs, std::string
         const size_t bytes = (size_t(1) << reqs[j]->logn) * kLimbs * 4;
         uint32_t* da = const_cast<uint32_t*>(L.h_jobs[j].a);
         uint32_t* db = const_cast<uint32_t*>(L.h_jobs[j].b);
-        if (reqs[j]->ab_pinned) {
+        if (reqs[j]->ab_pinned && !(dma_inputs && reqs[j]->logn >= kDmaMinLogN)) {
             span(L.h_in_segs, n_in, reqs[j]->a_dev, da, bytes);
             span(L.h_in_segs, n_in, reqs[j]->b_dev, db, bytes);
-        } else {
+        } else {  // pageable inputs, or page-locked ones by DMA under a backlog
             e = hipMemcpyAsync(da, reqs[j]->a, bytes, hipMemcpyHostToDevice, L.stream);
             if (e == hipSuccess) e = hipMemcpyAsync(db, reqs[j]->b, bytes, hipMemcpyHostToDevice, L.stream);
         }
@@ -279,6 +287,7 @@ void pm_server::run() {
     for (;;) {
         std::vector<Request*> batch;
         Launch* slot = nullptr;
+        bool backlog = false;  // requests wait beyond this launch: the server is behind
         {
             std::unique_lock<std::mutex> lock(mu);
             // admission: arrival order, while the head fits the budget
@@ -300,6 +309,7 @@ void pm_server::run() {
                     batch.push_back(admitted.front());
                     admitted.pop_front();
                 }
+                backlog = !admitted.empty() || !pending.empty();
             } else if (!any_busy) {
                 if (stop && pending.empty() && admitted.empty()) return;
                 cv_work.wait_for(lock, std::chrono::milliseconds(2));
@@ -308,7 +318,7 @@ void pm_server::run() {
         }
         if (!batch.empty()) {
             std::string why;
-            if (!start_launch(*slot, batch, why)) {
+            if (!start_launch(*slot, batch, backlog, why)) {
                 std::lock_guard<std::mutex> lock(mu);
                 error = why;
                 for (Request* r : batch) {
