@@ -141,6 +141,16 @@ def test_quartic_batch_matches_general(pm, workload):
         batch.wait()
         assert all((batch.download(i) == outs["general"][i]).all() for i in range(len(ns))), "quartic outer 2"
         batch.close()
+        # tiles at 16 residues per thread run the quartic channel rows too
+        for tile_n in (256, 1024, 4096, 8192, 16384):
+            ctx.set_batch_residues_per_thread(tile_n, 16)
+        batch = pm.Batch(ctx, ns)
+        for i, (a, b) in enumerate(ops):
+            batch.upload(i, a, b)
+        batch.run()
+        batch.wait()
+        assert all((batch.download(i) == outs["general"][i]).all() for i in range(len(ns))), "quartic 16 per thread"
+        batch.close()
         ctx.close()
 
 

@@ -48,6 +48,14 @@ def test_served_results_equal_alone(pm, orc, workload):
     with ThreadPoolExecutor(16) as pool:
         agree = list(pool.map(oracle_equal, range(len(reqs))))
     assert all(agree), [i for i, ok in enumerate(agree) if not ok]
+    # the same mix served with tiles at 16 residues per thread
+    for tile_n in (256, 1024, 2048, 4096, 8192, 16384):
+        ctx.set_batch_residues_per_thread(tile_n, 16)
+    server = pm.Server(ctx, 4 << 30, max_batch=8, max_inflight=2)
+    ids = [server.submit(a, b) for _, a, b in reqs]
+    again = [server.wait(r)[0] for r in ids]
+    server.close()
+    assert all((x == y[0]).all() for x, y in zip(again, results))
     ctx.close()
 
 
