@@ -21,9 +21,11 @@ def say(*a):
     print(*a, flush=True)
 
 
-@pytest.mark.skip(reason="a modified persistent kernel faulted the GPU on this uneven mix and the one run "
-                         "testing a fix faulted again; no cause was shown, the persistent mode was removed "
-                         "(the library refuses it) and the case stays off the GPU")
+@pytest.mark.skip(reason="an earlier persistent kernel (blocks claiming items and waiting on per-product "
+                         "counters, since removed) faulted the GPU on this uneven mix and the one run testing a "
+                         "fix faulted again; no cause was shown, so the case stays off the GPU.  "
+                         "PM_BATCH_PERSISTENT now names a grid-barrier kernel, covered by "
+                         "test_persistent_launch_matches_phased")
 def test_batch_uneven_sizes(pm, orc, workload):
     q = workload.modulus(1)
     ctx = pm.Context(q, 1 << 17)
@@ -78,8 +80,10 @@ def test_batch_uneven_sizes_phased(pm, orc, workload):
     ctx.close()
 
 
-PERSISTENT_OFF = ("the persistent kernel hung the GPU on this batch after a change to the tile code it shares; "
-                  "no cause was found, the mode was removed (the library refuses it) and the case stays off the GPU")
+PERSISTENT_OFF = ("an earlier persistent kernel (claims and per-product waits, since removed) hung the GPU on "
+                  "this batch after a change to the tile code it shared; no cause was found, so the case stays off "
+                  "the GPU.  PM_BATCH_PERSISTENT now names a grid-barrier kernel, covered by "
+                  "test_persistent_launch_matches_phased")
 
 
 @pytest.mark.parametrize("mode", [pytest.param("persistent", marks=pytest.mark.skip(reason=PERSISTENT_OFF)),
