@@ -49,6 +49,12 @@ __device__ __forceinline__ uint32_t shoup(uint32_t a, Tw w, uint32_t p) {
     const uint32_t quotient = __umulhi(a, w.shoup);
     return a * w.value - quotient * p;
 }
+// The same with np = 2^32 - p, which the caller keeps opaque to the optimiser
+// (an empty asm): a*w + q*np is one multiply and one multiply-add (the low
+// half of v_mad_u64_u32) instead of two multiplies and a subtract.
+__device__ __forceinline__ uint32_t shoup_np(uint32_t a, Tw w, uint32_t np) {
+    return a * w.value + __umulhi(a, w.shoup) * np;
+}
 // x < 2m -> x mod m   (x - m wraps above x when x < m)
 __device__ __forceinline__ uint32_t csub(uint32_t x, uint32_t m) {
     const uint32_t t = x - m;
@@ -86,10 +92,12 @@ __device__ __forceinline__ void gs_bfly(uint32_t& X, uint32_t& Y, Tw w, uint32_t
     Y = d * w.value + quotient * np;
 }
 // a*b*2^-32 mod p in [0,2p) for a, b in [0,2p) (Montgomery REDC).
+// (a*b + m*p) / 2^32 with m = lo * (-p^-1): the high half of m*p + lo is
+// umulhi(m, p) plus the carry of lo + (m*p mod 2^32), one v_mad_u64_u32.
 __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t neg_inv) {
     const uint32_t lo = a * b, hi = __umulhi(a, b);
     const uint32_t m = lo * neg_inv;
-    return hi + __umulhi(m, p) + (lo != 0u ? 1u : 0u);
+    return hi + uint32_t((uint64_t(m) * p + lo) >> 32);
 }
 
 // ---------------------------------------------------------------------------
@@ -412,10 +420,12 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
     // stages follow, in the CRT or a pass of their own) stores its scaled
     // values in [0,2p), as those Gentleman-Sande stages take them; they end in
     // [0,p).  A whole transform stores canonical residues for the CRT.
+    uint32_t np = 0u - p;
+    asm volatile("" : "+v"(np));
     const auto store = [&](auto canonical) {
 #pragma unroll
         for (int r = 0; r < G::E; ++r) {
-            uint32_t t = shoup(a[r], s, p);  // [0,2p)
+            uint32_t t = shoup_np(a[r], s, np);  // [0,2p)
             if constexpr (decltype(canonical)::value) t = csub(t, p);
             if (!valid) continue;
             if constexpr (kWaveTile)
@@ -524,15 +534,26 @@ __device__ __forceinline__ void conv_load_host(const uint32_t* __restrict__ coef
 // valid multiplied (upper) operand of the forward NTT's first butterflies as
 // it is, and one conditional subtraction of 2p makes it a lower operand
 // (below 4p).  The corrections and offsets are folded into two constants
-// (ConvLds::fold): 8 VALU instructions per residue.
+// (ConvLds::fold): 7 VALU instructions per residue.
 __device__ __forceinline__ uint32_t conv_residue(const v4i& w0, const v4i& w1, const v4i& b0, const v4i& b1,
                                                  uint2 fold, uint32_t p, Tw c16) {
     v4i acc = v4i{0, 0, 0, 0};
     acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w0, b0, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(w1, b1, acc, 0, 0, 0);
+    // u = T0' + 256 T1' alone is one shift-add; the accumulators are read by
+    // compiled code only, which waits for the MFMA results (an inline asm
+    // reading them gets no wait states: it read unfinished results)
     const uint32_t u = uint32_t(acc[0]) + (uint32_t(acc[1]) << 8);
     const uint32_t w = uint32_t(acc[2]) + (uint32_t(acc[3]) << 8) + fold.y;
-    return u + fold.x + shoup(w, c16, p);
+    // Shoup product by 2^16 as w 2^16 + floor(w sh16 / 2^32) (2^32 - p): one
+    // multiply-add (the low half of v_mad_u64_u32), 2^32 - p kept opaque as in
+    // gs_bfly
+    uint32_t np = 0u - p;
+    asm("" : "+v"(np));
+    const uint32_t x = (w << 16) + __umulhi(w, c16.shoup) * np;
+    uint32_t v;
+    asm("v_add3_u32 %0, %1, %2, %3" : "=v"(v) : "v"(u), "v"(fold.x), "v"(x));
+    return v;
 }
 
 // Residues of the group for quads [quad_begin, quad_end) of primes; n is the
