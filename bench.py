@@ -206,7 +206,10 @@ def run_headline(args):
     achieved = phase_gbs[dominant]
     traffic = valu_instructions = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_batched_n{n}_b{B}.json")
-    dominant_kernel = (f"pmd::k_items_tile<{dominant.split('^')[1]}>" if dominant.startswith("tile")
+    # the tile kernel's name carries its size and log2 of its residues per thread
+    dominant_kernel = (f"pmd::k_items_tile<{dominant.split('^')[1]}, "
+                       f"{5 if ctx.get_batch_residues_per_thread(1 << int(dominant.split('^')[1])) == 32 else 4}>"
+                       if dominant.startswith("tile")
                        else {"conversion": "pmd::k_items_conv", "crt": "pmd::k_items_crt_mfma"}.get(dominant))
     if os.path.exists(tfile):
         # HBM bytes per launch of the dominant phase from the FETCH_SIZE /
