@@ -48,7 +48,8 @@ struct Tables {
     const uint32_t* mont;         // [k] -p^-1 mod 2^32
     const Tw* fwd;                // [k][max_n] psi^brv(i), value stored negated mod 2^32
     const Tw* inv;                // [k][max_n] psi^-brv(i)
-    const Tw* scale;              // [max_logn+1][k]
+    const Tw* scale;              // [max_logn+1][k] the tile's scale (N >= 4: 4 times the post-inverse scale)
+    const Tw* zs;                 // [k][2^(max_logn-1)] zeta_G * scale of block G at 2^(logn-2) + G
     const uint4* conv_w;          // [quads][2][64] int8x16 weight operands
     const int32_t* conv_corr;     // [quads*4][4]
     const uint32_t* conv_sh16;    // [k] floor(2^48/p)
@@ -62,7 +63,7 @@ struct Tables {
     uint32_t mu0, mu1;            // floor(2^908/q)
     uint32_t k, quads, max_logn, cus;  // cus: compute units of the device
     // Quartic backend (q = 2^868 - qdelta): the rows above (primes, mont,
-    // fwd, inv, scale, k) are then the 4 qk channels, row 4i + j channel j of
+    // fwd, inv, scale, zs, k) are then the 4 qk channels, row 4i + j channel j of
     // prime i, and these hold the per-prime constants of the conversion and
     // the CRT; quartic = 0 and null pointers for the general RNS.
     uint32_t quartic, qk, qdelta, pad_;

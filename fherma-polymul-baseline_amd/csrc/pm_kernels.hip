@@ -91,6 +91,13 @@ __device__ __forceinline__ void gs_bfly(uint32_t& X, uint32_t& Y, Tw w, uint32_t
     const uint32_t quotient = __umulhi(d, w.shoup);
     Y = d * w.value + quotient * np;
 }
+// T*2^-32 mod p for a 64-bit T below 8p^2 (Montgomery REDC): m = T*(-p^-1)
+// mod 2^32, (T + m*p) / 2^32 with T + m*p < 8p^2 + 2^32 p < 2^64 (p < 2^30),
+// one multiply and one v_mad_u64_u32; the result is below 8p^2/2^32 + p < 3p.
+__device__ __forceinline__ uint32_t redc64(uint64_t T, uint32_t p, uint32_t neg_inv) {
+    const uint32_t m = uint32_t(T) * neg_inv;
+    return uint32_t((uint64_t(m) * p + T) >> 32);
+}
 // a*b*2^-32 mod p in [0,2p) for a, b in [0,2p) (Montgomery REDC).
 // (a*b + m*p) / 2^32 with m = lo * (-p^-1): the high half of m*p + lo is
 // umulhi(m, p) plus the carry of lo + (m*p mod 2^32), one v_mad_u64_u32.
@@ -279,6 +286,14 @@ __device__ __forceinline__ uint32_t group_of(uint32_t tau_x, int r, int beta) {
     return thread_bits_above ? ((tau_x >> (beta + 1)) + reg_part) : reg_part;
 }
 
+// The tile's transforms are incomplete: the forward stages stop at span 4,
+// where each block of 4 positions holds its operand modulo X^4 - zeta, and
+// the inverse starts there (block_product between them).  Stages on index
+// bits below kBlockLog are not run; a tile below 4 residues runs them all.
+constexpr int kBlockLog = 2;
+template <int TLOG>
+__host__ __device__ constexpr int first_stage_bit() { return TLOG >= kBlockLog ? kBlockLog : 0; }
+
 // Forward CT stages of phase P on two polynomials that share twiddles.
 // Twiddle of local stage (m groups, group g) is table[bm*m + g] with
 // bm = 2^u + s for sub-block s of a transform with u outer stages.
@@ -290,6 +305,7 @@ __device__ __forceinline__ void fwd_phase2(TileRegs<TLOG, EL>& a, TileRegs<TLOG,
 #pragma unroll
     for (int bit = G::width(P) - 1; bit >= 0; --bit) {
         const int beta = G::lo(P) + bit;
+        if (beta < first_stage_bit<TLOG>()) continue;
         const uint32_t m = 1u << (TLOG - 1 - beta);
         int done = 0;
 #pragma unroll
@@ -317,6 +333,7 @@ __device__ __forceinline__ void inv_phase(TileRegs<TLOG, EL>& a, const Tw* __res
 #pragma unroll
     for (int bit = 0; bit < G::width(P); ++bit) {
         const int beta = G::lo(P) + bit;
+        if (beta < first_stage_bit<TLOG>()) continue;
         const uint32_t m = 1u << (TLOG - 1 - beta);
         int done = 0;
 #pragma unroll
@@ -358,8 +375,49 @@ __device__ __forceinline__ void inverse_from(TileRegs<TLOG, EL>& a, uint32_t* ld
     if constexpr (P > 0) inverse_from<TLOG, EL, P - 1>(a, lds, tau, tw, bm, p, p2);
 }
 
-// One tile: forward(a), forward(b), pointwise, inverse, scale to [0,p).
-// `valid` false means this thread's tile is padding: it joins every barrier
+// Product of the blocks of the incomplete transforms, in the last forward
+// phase's registers: r..r+3 (r = 0 mod 4) are positions x..x+3 of one block,
+// a and b modulo X^4 - zeta.  With A_i = a_i reduced to [0,p), B_j = b_j S
+// and Z_j = b_j zeta S (Shoup, [0,2p); S the scale, zrow[x/4] = zeta S),
+// coefficient k of the product is REDC of
+//   sum_{i<=k} A_i B_{k-i} + sum_{i>k} A_i Z_{k-i+4}  (below 4 p 2p = 8p^2),
+// in [0,3p), and after one conditional subtraction in [0,2p), the inverse
+// stages' input range.  16 v_mad_u64_u32 per block; the scale of the result
+// is folded in, so the store multiplies by nothing.
+template <int TLOG, int EL>
+__device__ __forceinline__ void block_product(TileRegs<TLOG, EL>& a, const TileRegs<TLOG, EL>& b, const Tw* zrow,
+                                              uint32_t x_thr, Tw S, uint32_t p, uint32_t p2, uint32_t neg_inv) {
+    using G = Geo<TLOG, EL>;
+    constexpr int P = G::NPH - 1;
+    static_assert(G::reg_pos(P, 0) == 0 && G::reg_pos(P, 1) == 1,
+                  "the last phase holds index bits 0 and 1 in register bits 0 and 1");
+    uint32_t np = 0u - p;
+    asm volatile("" : "+v"(np));
+#pragma unroll
+    for (int r = 0; r < G::E; r += 4) {
+        const Tw z = const_twiddle(zrow, (x_thr + G::reg_scatter(P, r)) >> 2);
+        uint32_t A[4], B[4], Z[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            A[i] = csub(csub(a[r + i], p2), p);
+            B[i] = shoup_np(b[r + i], S, np);
+            Z[i] = i ? shoup_np(b[r + i], z, np) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint64_t T = uint64_t(A[0]) * B[k];
+#pragma unroll
+            for (int i = 1; i < 4; ++i) T += uint64_t(A[i]) * (i <= k ? B[k - i] : Z[k - i + 4]);
+            a[r + k] = csub(redc64(T, p, neg_inv), p);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// One tile: forward(a) and forward(b) down to blocks of 4, the scaled block
+// product, the inverse, the store ([0,p) for a whole transform).  (A tile
+// below 4 residues runs complete transforms, a Montgomery pointwise product
+// and the scale at the store.)  `valid` false means this thread's tile is padding: it joins every barrier
 // but touches no global memory.
 template <int TLOG, int EL = tile_elog(TLOG)>
 __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __restrict__ a_src,
@@ -410,10 +468,18 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
         }
     }
     forward_from<TLOG, EL, 0>(a, b, lds, tau, tb.fwd + table, bm, p, p2);
-#pragma unroll
-    for (int r = 0; r < G::E; ++r) a[r] = mont_mul(csub(a[r], p2), csub(b[r], p2), p, neg_inv);
-    inverse_from<TLOG, EL, G::NPH - 1>(a, lds, tau, tb.inv + table, bm, p, p2);
     const Tw s = tb.scale[logn_full * tb.k + pi];
+    constexpr bool kBlocks = TLOG >= kBlockLog;
+    if constexpr (kBlocks) {
+        // block G of the whole transform (zeta table entry 2^(logn-2) + G) is
+        // block x/4 of sub-block s = bm - 2^u: bm * 2^(TLOG-2) + x/4
+        const Tw* zrow = tb.zs + (size_t(pi) << (tb.max_logn - 1)) + (size_t(bm) << (TLOG - kBlockLog));
+        block_product<TLOG, EL>(a, b, zrow, thr_scatter<TLOG, EL, G::NPH - 1>(tau), s, p, p2, neg_inv);
+    } else {
+#pragma unroll
+        for (int r = 0; r < G::E; ++r) a[r] = mont_mul(csub(a[r], p2), csub(b[r], p2), p, neg_inv);
+    }
+    inverse_from<TLOG, EL, G::NPH - 1>(a, lds, tau, tb.inv + table, bm, p, p2);
     const __amdgpu_buffer_rsrc_t rd =
         __builtin_amdgcn_make_buffer_rsrc(wave_uniform(dst), short(0), int(G::T * 4u), int(kRawBufferWord3));
     // A tile that is a sub-block of a larger transform (bm > 1: outer inverse
@@ -425,7 +491,11 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
     const auto store = [&](auto canonical) {
 #pragma unroll
         for (int r = 0; r < G::E; ++r) {
-            uint32_t t = shoup_np(a[r], s, np);  // [0,2p)
+            uint32_t t;  // [0,2p)
+            if constexpr (kBlocks)
+                t = a[r];  // scaled in the block product
+            else
+                t = shoup_np(a[r], s, np);
             if constexpr (decltype(canonical)::value) t = csub(t, p);
             if (!valid) continue;
             if constexpr (kWaveTile)

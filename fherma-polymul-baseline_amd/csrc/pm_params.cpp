@@ -530,4 +530,23 @@ std::string build_quartic_params(uint32_t max_logn, const uint32_t* q_words, Qua
     return std::string();
 }
 
+void block_product_constants(uint32_t p, uint32_t max_logn, const Twiddle* forward, const Twiddle* scale,
+                             size_t scale_stride, Twiddle* scale4, Twiddle* zeta) {
+    zeta[0] = make_twiddle(0, p);
+    for (uint32_t logn = 0; logn <= max_logn; ++logn) {
+        const uint32_t s = scale[size_t(logn) * scale_stride].value;
+        const uint32_t S = logn >= 2 ? mulmod(4, s, p) : s;
+        scale4[logn] = make_twiddle(S, p);
+        if (logn < 2) continue;
+        for (uint32_t g = 0; g < (1u << (logn - 2)); ++g) {
+            uint32_t z = p - 1;  // N = 4: X^4 + 1
+            if (logn >= 3) {
+                const uint32_t w = forward[(1u << (logn - 3)) + g / 2].value;
+                z = (g & 1u) && w ? p - w : w;
+            }
+            zeta[(1u << (logn - 2)) + g] = make_twiddle(mulmod(z, S, p), p);
+        }
+    }
+}
+
 }  // namespace pm

@@ -109,6 +109,18 @@ struct QuarticParams {
 // Empty on success, else why q (28 words) is not a supported special modulus.
 std::string build_quartic_params(uint32_t max_logn, const uint32_t* q, QuarticParams& out);
 
+// Constants of the tile's block product for one prime p.  For N = 2^logn >= 4
+// the tile's forward transform stops at blocks of 4 positions, block G
+// holding its operand modulo X^4 - zeta_G, and its inverse runs logn - 2
+// stages, so the product carries 4 times the post-inverse scale:
+//   scale4[logn] = 4 scale[logn * scale_stride] mod p  (logn >= 2; as is below),
+//   zeta[2^(logn-2) + G] = zeta_G scale4[logn] mod p   (G < N/4),
+// zeta_G = +w (G even) or -w (G odd) with w the forward twiddle (psi^brv) of
+// index 2^(logn-3) + G/2, and zeta = -1 for N = 4.  zeta has 2^(max_logn-1)
+// entries (entry 0 unused).
+void block_product_constants(uint32_t p, uint32_t max_logn, const Twiddle* forward, const Twiddle* scale,
+                             size_t scale_stride, Twiddle* scale4, Twiddle* zeta);
+
 // Shared helpers (also used by tests through the C-ABI).
 uint32_t pow_mod(uint32_t a, uint64_t e, uint32_t p);
 bool is_prime(uint32_t p);

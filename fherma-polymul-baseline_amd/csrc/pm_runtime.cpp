@@ -175,7 +175,19 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     // forward twiddles carry -w mod 2^32: the butterfly forms q*p - Y*w as q*p + Y*(-w)
     for (size_t i = 0; i < fwd.size(); ++i) fwd[i] = {0u - s.forward[i].value, s.forward[i].shoup};
     for (size_t i = 0; i < inv.size(); ++i) inv[i] = {s.inverse[i].value, s.inverse[i].shoup};
-    for (size_t i = 0; i < scale.size(); ++i) scale[i] = {s.scale[i].value, s.scale[i].shoup};
+    // the tile's scale and block-product constants, per prime
+    const uint32_t zrow = 1u << (ctx->max_logn - 1);
+    std::vector<pmd::Tw> zs(size_t(s.k) * zrow);
+    {
+        std::vector<pm::Twiddle> s4(ctx->max_logn + 1), z(zrow);
+        for (uint32_t i = 0; i < s.k; ++i) {
+            pm::block_product_constants(s.primes[i], ctx->max_logn, s.forward.data() + (size_t(i) << ctx->max_logn),
+                                        s.scale.data() + i, s.k, s4.data(), z.data());
+            for (uint32_t logn = 0; logn <= ctx->max_logn; ++logn)
+                scale[size_t(logn) * s.k + i] = {s4[logn].value, s4[logn].shoup};
+            for (uint32_t x = 0; x < zrow; ++x) zs[size_t(i) * zrow + x] = {z[x].value, z[x].shoup};
+        }
+    }
 
     pmd::Tables& tb = ctx->tables;
     tb.primes = upload(ctx, s.primes, &status);
@@ -183,6 +195,7 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     tb.fwd = upload(ctx, fwd, &status);
     tb.inv = upload(ctx, inv, &status);
     tb.scale = upload(ctx, scale, &status);
+    tb.zs = upload(ctx, zs, &status);
     tb.conv_w = reinterpret_cast<const uint4*>(upload(ctx, conv_w, &status));
     tb.conv_corr = upload(ctx, conv_corr, &status);
     tb.conv_sh16 = upload(ctx, s.conv_shoup16, &status);
@@ -695,7 +708,10 @@ static int build_quartic(pm_context* ctx) {
     const pm::QuarticParams& s = ctx->qparams;
     const uint32_t k = s.k, rows = 4 * k, max_n = 1u << ctx->max_logn;
     std::vector<uint32_t> rp(rows), rm(rows);
-    std::vector<pmd::Tw> fwd(size_t(rows) * max_n), inv(size_t(rows) * max_n), scale(size_t(ctx->max_logn + 1) * rows);
+    const uint32_t zrow = 1u << (ctx->max_logn - 1);
+    std::vector<pmd::Tw> fwd(size_t(rows) * max_n), inv(size_t(rows) * max_n), scale(size_t(ctx->max_logn + 1) * rows),
+        zs(size_t(rows) * zrow);
+    std::vector<pm::Twiddle> s4(ctx->max_logn + 1), z(zrow);
     for (uint32_t row = 0; row < rows; ++row) {
         const uint32_t i = row / 4;
         rp[row] = s.primes[i];
@@ -705,10 +721,13 @@ static int build_quartic(pm_context* ctx) {
             fwd[size_t(row) * max_n + x] = {0u - f.value, f.shoup};  // the butterfly's negated forward twiddle
             inv[size_t(row) * max_n + x] = {g.value, g.shoup};
         }
-        for (uint32_t logn = 0; logn <= ctx->max_logn; ++logn) {
-            const pm::Twiddle sc = s.scale[size_t(logn) * k + i];
-            scale[size_t(logn) * rows + row] = {sc.value, sc.shoup};
-        }
+        // the tile's scale and block-product constants (the prime's, for each of its channels)
+        if (row % 4 == 0)
+            pm::block_product_constants(s.primes[i], ctx->max_logn, s.forward.data() + size_t(i) * max_n,
+                                        s.scale.data() + i, k, s4.data(), z.data());
+        for (uint32_t logn = 0; logn <= ctx->max_logn; ++logn)
+            scale[size_t(logn) * rows + row] = {s4[logn].value, s4[logn].shoup};
+        for (uint32_t x = 0; x < zrow; ++x) zs[size_t(row) * zrow + x] = {z[x].value, z[x].shoup};
     }
     auto tw = [](const std::vector<pm::Twiddle>& v) {
         std::vector<pmd::Tw> out(v.size());
@@ -722,6 +741,7 @@ static int build_quartic(pm_context* ctx) {
     tb.fwd = upload(ctx, fwd, &status);
     tb.inv = upload(ctx, inv, &status);
     tb.scale = upload(ctx, scale, &status);
+    tb.zs = upload(ctx, zs, &status);
     tb.k = rows;
     tb.quads = 0;
     tb.conv_w = nullptr;
