@@ -60,7 +60,7 @@ struct Tables {
     const uint4* crt_a;           // [8 row tiles][4 k steps][64 lanes] int8x16 MFMA operands
     const int4* crt_corr;         // [32]: row corrections of rows 4r..4r+3
     const uint32_t* q;            // [28]
-    uint32_t mu0, mu1;            // floor(2^908/q)
+    uint32_t mu0, mu1;            // floor(2^906/q)
     uint32_t k, quads, max_logn, cus;  // cus: compute units of the device
     // Quartic backend (q = 2^868 - qdelta): the rows above (primes, mont,
     // fwd, inv, scale, zs, k) are then the 4 qk channels, row 4i + j channel j of

@@ -868,11 +868,15 @@ __device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], 
 #pragma unroll
     for (int w = 0; w < 28; ++w) x[w] = __builtin_addc(x[w], nk[w], cin, &cin);
     x[28] += cin;
-    // Barrett: X < 2^904, Qh = floor(floor(X/2^864) * floor(2^908/q) / 2^44) < 2^37
-    const uint64_t xh = uint64_t(x[27]) | (uint64_t(x[28]) << 32);
+    // Barrett: X < 2^905, Qh = floor(floor(X/2^863) * floor(2^906/q) / 2^43) < 2^42.
+    // X/q - Qh < (X mod 2^863)/q + floor(X/2^863)/2^43 + 1 < 1/2 + 1/2 + 1 for
+    // q > 2^864, so the result is in [0, 2q).  (The truncation at 2^864 with
+    // floor(2^908/q) leaves up to 2^864/q + 1/16 + 1, which passes 2 for q
+    // just above 2^864.)
+    const uint64_t xh = (uint64_t(x[26]) >> 31) | (uint64_t(x[27]) << 1) | (uint64_t(x[28]) << 33);
     const uint64_t mu = uint64_t(tb.mu0) | (uint64_t(tb.mu1) << 32);
     const uint64_t plo = xh * mu, phi = __umul64hi(xh, mu);
-    const uint64_t qh = (phi << 20) | (plo >> 44);
+    const uint64_t qh = (phi << 21) | (plo >> 43);
     const uint32_t q0 = uint32_t(qh), q1 = uint32_t(qh >> 32);
     // X -= q0*q and X -= q1*q*2^32: two multiply chains (each word product
     // plus the previous carry fits 64 bits) with their own borrow chains; the

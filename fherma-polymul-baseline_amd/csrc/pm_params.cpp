@@ -176,9 +176,9 @@ std::string build_params(uint32_t max_logn, const uint32_t* q_words, uint32_t li
     trim(q);
     if (q.empty() || (q[0] & 1u) == 0) return "q must be odd";
     if (bits(q) > kWidth) return "q must be below 2^868";
-    // The CRT's Barrett step estimates the quotient from floor(X / 2^864) and
-    // floor(2^908 / q); its error is below 1 + X/2^908 + 2^864/q, so a single
-    // conditional subtraction finishes it only when q > 2^864.
+    // The CRT's Barrett step estimates the quotient of X < 2^905 from
+    // floor(X / 2^863) and floor(2^906 / q); its error is below
+    // 1 + X/2^906 + 2^863/q, under 2 (one conditional subtraction) when q > 2^864.
     if (bits(q) < kMinModulusBits) return "q must have 865 to 868 bits (q > 2^864)";
     s = Params{};
     s.max_logn = max_logn;
@@ -301,7 +301,7 @@ std::string build_params(uint32_t max_logn, const uint32_t* q_words, uint32_t li
         for (size_t w = 0; w < v.size() && w < kLimbs; ++w) s.crt_neg_kp[size_t(K) * kLimbs + w] = v[w];
     }
     Big mu;
-    divmod(pow2(906 + 2), q, &mu, nullptr);  // floor(2^908 / q)
+    divmod(pow2(906), q, &mu, nullptr);  // floor(2^906 / q)
     mu.resize(2, 0);
     s.barrett_mu = {mu[0], mu[1]};
 

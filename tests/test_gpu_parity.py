@@ -173,6 +173,44 @@ def test_crt_mfma_matches_scalar(pm, workload, torch_cuda, n):
         assert (to_host(c_mfma) == to_host(c_scalar)).all(), name
 
 
+def test_crt_reduction_edge_vs_integers(pm, torch_cuda):
+    """Residue vectors (tests/golden/crt_barrett_cases.json) whose CRT sum
+    for an 865-bit q just above 2^864 sits where a quotient estimate from the
+    bits above 2^864 leaves X - Qh q in [2q, 3q): both CRTs must give the
+    words of (sum_i t_i M_i - K P) mod q, K the kernel's nearest-integer
+    quotient, computed here with Python integers; zeros and random residues
+    fill the other coefficients."""
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "crt_barrett_cases.json")))
+    say("crt reduction edge")
+    q_words = np.asarray(fx["q_words"], dtype=np.uint32)
+    q = sum(int(w) << (32 * i) for i, w in enumerate(fx["q_words"]))
+    ctx = context(pm, q_words, fx["max_n"])
+    primes = [p for p, _ in ctx.primes()]
+    assert primes == fx["primes"]
+    k, n = len(primes), 64
+    product = 1
+    for p in primes:
+        product *= p
+    rng = np.random.default_rng(5)
+    t = np.zeros((k, n), dtype=np.uint64)
+    t[:, 32:] = rng.integers(0, 1 << 62, size=(k, n - 32), dtype=np.uint64) % np.asarray(primes, dtype=np.uint64)[:, None]
+    for j, case in zip((0, 9, 17, 31), fx["residues"]):
+        t[:, j] = case
+    want = np.zeros((n, 28), dtype=np.uint32)
+    for j in range(n):
+        tj = [int(v) for v in t[:, j]]
+        est = sum(ti * ((1 << 42) // p) for ti, p in zip(tj, primes))
+        K = (est + (1 << 41)) >> 42
+        c = (sum(ti * (product // p) for ti, p in zip(tj, primes)) - K * product) % q
+        want[j] = [(c >> (32 * w)) & 0xFFFFFFFF for w in range(28)]
+    t_dev = to_dev(torch_cuda, np.ascontiguousarray(t.astype(np.uint32)))
+    for stage in (ctx.stage_crt, ctx.stage_crt_scalar):
+        c_dev = torch_cuda.zeros(n * 28, dtype=torch_cuda.int32, device="cuda")
+        stage(n, t_dev, c_dev)
+        ctx.synchronize()
+        assert (to_host(c_dev).reshape(n, 28) == want).all(), stage.__name__
+
+
 @pytest.mark.parametrize("n", [32, 64, 1024, 4096, 8192, 16384, 32768, 65536, 131072])
 def test_product_vs_golden(pm, workload, n):
     for case in GOLDEN["cases"]:
