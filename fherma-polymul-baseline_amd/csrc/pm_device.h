@@ -56,7 +56,7 @@ struct Tables {
     const Tw* limb_pow;           // [k][28] 2^(32j) mod p (scalar conversion)
     const uint32_t* crt_m30;      // [k][29]
     const uint32_t* crt_round;    // [k] floor(2^42/p)
-    const uint32_t* crt_negkp;    // [k+1][28]
+    const uint32_t* crt_negkp;    // [2k+1][28]
     const uint4* crt_a;           // [8 row tiles][4 k steps][64 lanes] int8x16 MFMA operands
     const int4* crt_corr;         // [32]: row corrections of rows 4r..4r+3
     const uint32_t* q;            // [28]

@@ -854,15 +854,16 @@ __device__ __forceinline__ void crt_one(const Tables& tb, const uint32_t* __rest
 }
 
 // From the 29 words of A = sum_i t_i (M_i mod q) and the rounding estimate
-// est = sum_i t_i floor(2^42/p_i): K = nearest integer of sum_i t_i/p_i,
-// X = A + (-K*P mod q) < 2^904, then Barrett reduction modulo q and the
-// 28-word store (role of rns/solve.cu:336-420).
+// est = sum_i t_i floor(2^42/p_i) of residues t_i < 2 p_i (canonical or one
+// p above): K = nearest integer of sum_i t_i/p_i (at most 2k), X = A +
+// (-K*P mod q) < 2^905, then Barrett reduction modulo q and the 28-word store
+// (role of rns/solve.cu:336-420).
 __device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], uint64_t est,
                                            uint32_t* __restrict__ out) {
     const uint32_t k = tb.k;
     // nearest integer quotient of the CRT sum by P; add (-K*P) mod q
     uint32_t K = uint32_t((est + (uint64_t(1) << 41)) >> 42);
-    K = K <= k ? K : k;  // canonical t_i give K <= k; clamp keeps the table read in bounds
+    K = K <= 2u * k ? K : 2u * k;  // t_i < 2 p_i give K <= 2k; the clamp keeps the table read in bounds
     const uint32_t* nk = tb.crt_negkp + size_t(K) * kLimbs;
     uint32_t cin = 0;
 #pragma unroll
@@ -973,7 +974,7 @@ __host__ __device__ constexpr uint32_t crt_ot_slot(uint32_t i) { return 2u * i +
 // the lane's 16 primes its four values (positions x, x + n/4, x + n/2,
 // x + 3n/4 of the tile's scaled output, in [0,2p)) go through the radix-2 or
 // radix-4 Gentleman-Sande stages of the separate outer pass, in the same
-// order, and back to [0,p) (the CRT needs canonical residues).  ot holds per
+// order, and stay in [0,2p) (the CRT takes residues below 2p).  ot holds per
 // prime {p, w1, w1', w2} {w2', w3, w3', 0}: the inverse twiddles 1..3.
 __device__ __forceinline__ void crt_outer_inverse(uint32_t (&b)[4][4][4], const uint4* ot, uint32_t k, uint32_t g,
                                                   uint32_t outer) {
@@ -992,10 +993,10 @@ __device__ __forceinline__ void crt_outer_inverse(uint32_t (&b)[4][4][4], const 
             }
             gs_bfly(x0, x2, Tw{c0.y, c0.z}, p, p2);
             gs_bfly(x1, x3, Tw{c0.y, c0.z}, p, p2);
-            b[s][0][j] = csub(x0, p);
-            b[s][1][j] = csub(x1, p);
-            b[s][2][j] = csub(x2, p);
-            b[s][3][j] = csub(x3, p);
+            b[s][0][j] = x0;
+            b[s][1][j] = x1;
+            b[s][2][j] = x2;
+            b[s][3][j] = x3;
             // one prime's constants at a time: left to itself the compiler
             // loads all 16 primes' ahead and spills
             __builtin_amdgcn_sched_barrier(0);
@@ -1032,8 +1033,8 @@ __device__ __forceinline__ void crt_mfma_group(const Tables& tb, uint32_t (&bop)
         if (word < kCrtWords) {
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
-                // |D| <= 64 primes * 128 * (3*255 + 63) < 2^22.7 (top byte of t < p < 2^30 is
-                // below 64), so d0 + 256 d1 and d2 + 256 d3 fit in int32 and the word
+                // |D| <= 64 primes * 128 * (3*255 + 127) < 2^22.8 (top byte of t < 2p < 2^31 is
+                // below 128), so d0 + 256 d1 and d2 + 256 d3 fit in int32 and the word
                 // partial u + 2^16 w stays below 2^47 in magnitude
                 const int32_t u = acc[ct][0] + acc[ct][1] * 256;
                 const int32_t w = acc[ct][2] + acc[ct][3] * 256;

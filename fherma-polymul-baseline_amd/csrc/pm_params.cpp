@@ -292,8 +292,8 @@ std::string build_params(uint32_t max_logn, const uint32_t* q_words, uint32_t li
     }
     Big product_mod_q;
     divmod(product, q, nullptr, &product_mod_q);
-    s.crt_neg_kp.assign(size_t(k + 1) * kLimbs, 0);
-    for (uint32_t K = 0; K <= k; ++K) {
+    s.crt_neg_kp.assign(size_t(2 * k + 1) * kLimbs, 0);
+    for (uint32_t K = 0; K <= 2 * k; ++K) {  // residues below 2p_i: K <= 2k
         // (-K * P) mod q = (q - (K*P mod q)) mod q
         Big v;
         divmod(mul_small(product_mod_q, K), q, nullptr, &v);

@@ -53,7 +53,7 @@ struct Params {
     // CRT
     std::vector<uint32_t> crt_bases30;           // k * 29: M_i mod q in 30-bit limbs
     std::vector<uint32_t> crt_round;             // k: floor(2^42 / p_i)
-    std::vector<uint32_t> crt_neg_kp;            // (k+1) * 28: (-K * P) mod q
+    std::vector<uint32_t> crt_neg_kp;            // (2k+1) * 28: (-K * P) mod q
     std::vector<uint32_t> barrett_mu;            // 2 words: floor(2^906 / q)
     // CRT on int8 MFMA: the 128 x 256 digit matrix A[j][4i+u] (row j = byte
     // position of the sum; balanced base-256 digit j-u of M_i mod q for
