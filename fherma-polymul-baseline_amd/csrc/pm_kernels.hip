@@ -272,6 +272,20 @@ __device__ __forceinline__ Tw const_twiddle(const Tw* tw, uint32_t i) {
     return Tw{e[0], e[1]};
 }
 
+// A work-list entry (item, product) at a wave-uniform index, read through the
+// constant address space so that it lands in scalar registers: the lists are
+// read-only while any kernel runs.
+template <class T>
+__device__ __forceinline__ T const_entry(const T* list, uint32_t i) {
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    const ConstWords w = (ConstWords)(list + i);
+    T out;
+    uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (uint32_t j = 0; j < sizeof(T) / 4; ++j) o[j] = w[j];
+    return out;
+}
+
 // Group index (index bits above beta) of register r in phase P.  When every
 // thread bit of the phase lies below beta+1 the group -- hence the twiddle --
 // is the same for all threads, which lets the compiler use scalar loads.
@@ -696,6 +710,8 @@ __device__ __forceinline__ void conv_slot_compute(const Tables& tb, const v4u (&
                                                   uint32_t* __restrict__ res_any, const ConvLds& L, uint32_t lane,
                                                   uint32_t quad_begin = 0, uint32_t quad_end = 0xFFFFFFFFu) {
     const global_store_u32 res = (global_store_u32)res_any;
+    const __amdgpu_buffer_rsrc_t rres =
+        __builtin_amdgcn_make_buffer_rsrc(wave_uniform(res_any), short(0), int(tb.k * n * 4u), int(kRawBufferWord3));
     const uint32_t col = lane & 15u, kg = lane >> 4;
     const v4u m = v4u{0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u};
     v4i a0[4], a1[4], b0[4], b1[4];
@@ -754,13 +770,17 @@ __device__ __forceinline__ void conv_slot_compute(const Tables& tb, const v4u (&
         // rows (lane / 16) of A: primes 4q..4q+3 at columns x..x+15; of B the
         // same primes at x+16..x+31.  After the swap A holds rows {A0, B0, A2,
         // B2} (primes 4q, 4q, 4q+2, 4q+2) and B rows {A1, B1, A3, B3}.
+        // Buffer stores: the lane's byte offset of (pa, column tile 0) in a
+        // VGPR, column tile ct and prime pb = pa + 1 as scalar offsets
+        // (k * n * 4 <= 64 * 2^17 * 4 bytes: 32-bit offsets).
         const uint32_t pa = quad * 4u + (kg & 2u), pb = pa + 1u;
+        const uint32_t voff = ((pa * n + cA + 16u * (kg & 1u) + col) << 2);
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
             const auto sw = __builtin_amdgcn_permlane16_swap(ra[ct], rb[ct], false, false);
-            const uint32_t c = cA + 16u * (kg & 1u) + cstride * ct + col;
-            if (pa < tb.k) res[size_t(pa) * n + c] = sw[0];
-            if (pb < tb.k) res[size_t(pb) * n + c] = sw[1];
+            const uint32_t soff = __builtin_amdgcn_readfirstlane(cstride * uint32_t(ct) * 4u);
+            if (pa < tb.k) __builtin_amdgcn_raw_buffer_store_b32(sw[0], rres, voff, soff, 0);
+            if (pb < tb.k) __builtin_amdgcn_raw_buffer_store_b32(sw[1], rres, voff, soff + 4u * n, 0);
         }
     }
 }
@@ -1519,24 +1539,31 @@ __global__ void __launch_bounds__(kCrtPerBlock) k_crt(Tables tb, Job job) {
 __device__ __forceinline__ void conv_items_loop(const Tables& tb, const BatchArgs& ba, uint32_t first,
                                                 uint32_t count, const ConvLds& L) {
     constexpr uint32_t per_item = kConvItemCoefs / kConvSlotCoefs;
-    const uint32_t waves = blockDim.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t total = count * per_item, stride = gridDim.x * waves;
+    // the slot index and all that follows from it (item, product, pointers)
+    // in scalar registers: blockDim.x comes from the dispatch packet through a
+    // vector load, and a stride left in a VGPR makes the loop, the pointers
+    // and the buffer descriptors per-lane (waterfall loops around every load)
+    const uint32_t waves = __builtin_amdgcn_readfirstlane(blockDim.x >> 6), lane = threadIdx.x & 63u;
+    const uint32_t total = count * per_item, stride = __builtin_amdgcn_readfirstlane(gridDim.x * waves);
     for (uint32_t g = __builtin_amdgcn_readfirstlane(blockIdx.x * waves + (threadIdx.x >> 6)); g < total;
          g += stride) {
-        const Item it = ba.items[first + g / per_item];
-        const Job job = ba.jobs[it.job];
-        const uint32_t n = 1u << job.logn, slot = it.a1 * per_item + g % per_item;
+        const Item it = const_entry(ba.items, first + g / per_item);
+        const Job job = const_entry(ba.jobs, it.job);
+        const uint32_t n = __builtin_amdgcn_readfirstlane(1u << job.logn);
+        const uint32_t slot = __builtin_amdgcn_readfirstlane(it.a1) * per_item + g % per_item;
         if (slot * kConvSlotCoefs >= n) continue;  // the item's tail past a small n
-        const uint32_t* coef = it.a0 ? job.b : job.a;
-        uint32_t* res = it.a0 ? job.rb : job.ra;
-        const bool fused = job.fused != 0;
+        const bool second = __builtin_amdgcn_readfirstlane(it.a0) != 0;
+        const uint32_t* coef = wave_uniform(second ? job.b : job.a);
+        uint32_t* res = wave_uniform(second ? job.rb : job.ra);
+        const bool fused = __builtin_amdgcn_readfirstlane(job.fused) != 0;
         // fused: slot s covers columns 32s .. 32s + 31 of each quarter
         const uint32_t cA = fused ? 32u * slot : kConvSlotCoefs * slot, cstride = fused ? n / 4u : 16u;
         const uint32_t cB = fused ? cA + 16u : cA + 64u;
         v4u a0[4], a1[4], b0[4], b1[4];
         conv_load(coef, n, cA, lane, a0, a1, cstride);
         conv_load(coef, n, cB, lane, b0, b1, cstride);
-        conv_slot_compute(tb, a0, a1, b0, b1, n, cA, cstride, fused, job.outer, res, L, lane);
+        conv_slot_compute(tb, a0, a1, b0, b1, n, cA, cstride, fused, __builtin_amdgcn_readfirstlane(job.outer), res, L,
+                          lane);
     }
 }
 
