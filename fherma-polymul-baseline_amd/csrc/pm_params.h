@@ -19,7 +19,7 @@ namespace pm {
 
 constexpr uint32_t kLimbs = 28;          // u32 limbs per ABI coefficient (fherma.h:35)
 constexpr uint32_t kWidth = 868;         // W
-constexpr uint32_t kMinModulusBits = 865; // q > 2^864: the CRT's Barrett quotient is then off by at most 1
+constexpr uint32_t kMinModulusBits = 865; // q > 2^864: the CRT's Barrett finish then needs one conditional subtraction
 constexpr uint32_t kMaxLogN = 17;        // largest supported N = 2^17
 constexpr uint32_t kMaxPrimes = 64;
 constexpr uint32_t kCrtLimbs = 29;       // 30-bit limbs of M_i mod q (870 bits)

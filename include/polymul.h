@@ -221,7 +221,8 @@ int pm_stage_convert(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uin
 int pm_stage_convert_scalar(pm_context* ctx, uint32_t n, const uint32_t* d_a, const uint32_t* d_b, uint32_t* d_ra,
                             uint32_t* d_rb, void* hip_stream);
 /* forward NTTs, pointwise, inverse, scale: ra <- N^-1 (M_i mod p_i)^-1 * (a*b mod p_i) --
- * rns/solve.cu:225-333; outer = number of radix-2 stages run as separate passes */
+ * rns/solve.cu:225-333 (here incomplete transforms down to blocks of 4, multiplied
+ * modulo X^4 - zeta); outer = number of radix-2 stages run as separate passes */
 int pm_stage_ntt(pm_context* ctx, uint32_t n, uint32_t outer, uint32_t* d_ra, uint32_t* d_rb, void* hip_stream);
 /* CRT + mod q from scaled residues -- rns/solve.cu:336-420 (int8 MFMA sums) */
 int pm_stage_crt(pm_context* ctx, uint32_t n, const uint32_t* d_t, uint32_t* d_c, void* hip_stream);
