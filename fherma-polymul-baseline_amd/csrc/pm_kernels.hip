@@ -826,6 +826,7 @@ __device__ __forceinline__ void convert_scalar_one(const Tables& tb, const uint3
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], uint64_t est,
                                            uint32_t* __restrict__ out);
+__device__ __forceinline__ void crt_reduce(const Tables& tb, uint32_t (&x)[29], uint64_t est);
 
 __device__ __forceinline__ void crt_one(const Tables& tb, const uint32_t* __restrict__ t_res, uint32_t n,
                                         uint32_t c, uint32_t* __restrict__ out) {
@@ -873,13 +874,21 @@ __device__ __forceinline__ void crt_one(const Tables& tb, const uint32_t* __rest
     crt_finish(tb, x, est, out + size_t(c) * kLimbs);
 }
 
+// The 28-word result of crt_reduce stored for one coefficient.
+__device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], uint64_t est,
+                                           uint32_t* __restrict__ out) {
+    crt_reduce(tb, x, est);
+    uint4* dst = reinterpret_cast<uint4*>(out);
+#pragma unroll
+    for (int w = 0; w < 28; w += 4) dst[w / 4] = make_uint4(x[w], x[w + 1], x[w + 2], x[w + 3]);
+}
+
 // From the 29 words of A = sum_i t_i (M_i mod q) and the rounding estimate
 // est = sum_i t_i floor(2^42/p_i) of residues t_i < 2 p_i (canonical or one
 // p above): K = nearest integer of sum_i t_i/p_i (at most 2k), X = A +
-// (-K*P mod q) < 2^905, then Barrett reduction modulo q and the 28-word store
+// (-K*P mod q) < 2^905, then Barrett reduction modulo q into x[0..27]
 // (role of rns/solve.cu:336-420).
-__device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], uint64_t est,
-                                           uint32_t* __restrict__ out) {
+__device__ __forceinline__ void crt_reduce(const Tables& tb, uint32_t (&x)[29], uint64_t est) {
     const uint32_t k = tb.k;
     // nearest integer quotient of the CRT sum by P; add (-K*P) mod q
     uint32_t K = uint32_t((est + (uint64_t(1) << 41)) >> 42);
@@ -918,11 +927,8 @@ __device__ __forceinline__ void crt_finish(const Tables& tb, uint32_t (&x)[29], 
 #pragma unroll
     for (int w = 0; w < 28; ++w) y[w] = __builtin_subc(x[w], qw[w], bq, &bq);
     const bool ge = x[28] >= bq;
-    uint4* dst = reinterpret_cast<uint4*>(out);
 #pragma unroll
-    for (int w = 0; w < 28; w += 4)
-        dst[w / 4] = ge ? make_uint4(y[w], y[w + 1], y[w + 2], y[w + 3])
-                        : make_uint4(x[w], x[w + 1], x[w + 2], x[w + 3]);
+    for (int w = 0; w < 28; ++w) x[w] = ge ? y[w] : x[w];
 }
 
 // CRT on int8 MFMA, one wave per group of 64 coefficients [c0, c0+64).  The
@@ -1082,7 +1088,25 @@ __device__ __forceinline__ void crt_mfma_group(const Tables& tb, uint32_t (&bop)
     const int64_t e0 = int64_t(uint64_t(lo[29 * kCrtLdsStride + lane])) + (int64_t(hi[29 * kCrtLdsStride + lane]) << 32);
     const int64_t e1 = int64_t(uint64_t(lo[30 * kCrtLdsStride + lane])) + (int64_t(hi[30 * kCrtLdsStride + lane]) << 32);
     const uint64_t est = uint64_t(e0 + (e1 << 32));
-    if (c < n) crt_finish(tb, x, est, out + size_t(c) * kLimbs);
+    crt_reduce(tb, x, est);
+    // The results leave through the same LDS rows: lane L's 112 bytes at
+    // 112 L, so the group's column tiles (lanes 16 ct .. 16 ct + 15, 16
+    // consecutive coefficients) are 1792 contiguous bytes each, and store i
+    // moves 16-byte chunk 64 i + lane: every store instruction writes whole
+    // runs of lines instead of 16 bytes at a 112-byte stride.
+    uint4* stage = reinterpret_cast<uint4*>(wave_lds);
+#pragma unroll
+    for (int w = 0; w < 28; w += 4) stage[lane * 7u + uint32_t(w / 4)] = make_uint4(x[w], x[w + 1], x[w + 2], x[w + 3]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (uint32_t i = 0; i < 7; ++i) {
+        const uint32_t idx = 64u * i + lane, ct = idx / 112u, off = idx % 112u;
+        const uint32_t cbase = c0 + ct * cstride;
+        const uint4 v = stage[idx];
+        if (cbase + off / 7u < n) reinterpret_cast<uint4*>(out + size_t(cbase) * kLimbs)[off] = v;
+    }
     // the next group of this wave rewrites the same LDS rows
     __builtin_amdgcn_wave_barrier();
 }
