@@ -224,13 +224,29 @@ def test_product_vs_golden(pm, workload, n):
         assert sha(c) == case["sha256"]["c"], (n, case["modulus"], case["family"])
 
 
-@pytest.mark.parametrize("n", [2, 4, 8, 16, 128, 2048])
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 128, 256, 512, 2048])
 def test_product_small_sizes_vs_schoolbook(pm, orc, workload, n):
     say("small", n)
     q = workload.modulus(21)
     ctx = context(pm, q, 1 << 15)
     for family in ("dense-random-1", "all-q-minus-one", "wrap-sign"):
         a, b = workload.family(family, n, q, 9)
+        want = orc.schoolbook(q, a, b) if n <= 256 else orc.rns_multiply(q, a, b, orc.primes_reference())
+        assert (ctx.multiply(a, b) == want).all(), (n, family)
+
+
+@pytest.mark.parametrize("n", [64, 4096])
+def test_product_865_bit_modulus_vs_oracle(pm, orc, workload, n):
+    """An 865-bit q just above 2^864 (the smallest the library accepts)
+    through the whole product, against the oracle's RNS port (schoolbook for
+    N = 64) on the families with the largest sums."""
+    say("865-bit q", n)
+    import random
+    rng = random.Random(865)
+    q = workload.from_int((1 << 864) + rng.getrandbits(850) | 1)
+    ctx = context(pm, q, 1 << 15)
+    for family in ("dense-random-1", "all-q-minus-one", "limb-boundaries"):
+        a, b = workload.family(family, n, q, 5)
         want = orc.schoolbook(q, a, b) if n <= 256 else orc.rns_multiply(q, a, b, orc.primes_reference())
         assert (ctx.multiply(a, b) == want).all(), (n, family)
 
