@@ -48,6 +48,7 @@ struct Tables {
     const uint32_t* mont;         // [k] -p^-1 mod 2^32
     const Tw* fwd;                // [k][max_n] psi^brv(i), value stored negated mod 2^32
     const Tw* inv;                // [k][max_n] psi^-brv(i)
+    const Tw* invc;               // [k][max_n] p - psi^-brv(i): the inverse's complemented outputs
     const Tw* scale;              // [max_logn+1][k] the tile's scale (N >= 4: 4 times the post-inverse scale)
     const Tw* zs;                 // [k][2^(max_logn-1)] zeta_G * scale of block G at 2^(logn-2) + G
     const uint4* conv_w;          // [quads][2][64] int8x16 weight operands
@@ -63,7 +64,7 @@ struct Tables {
     uint32_t mu0, mu1;            // floor(2^906/q)
     uint32_t k, quads, max_logn, cus;  // cus: compute units of the device
     // Quartic backend (q = 2^868 - qdelta): the rows above (primes, mont,
-    // fwd, inv, scale, zs, k) are then the 4 qk channels, row 4i + j channel j of
+    // fwd, inv, invc, scale, zs, k) are then the 4 qk channels, row 4i + j channel j of
     // prime i, and these hold the per-prime constants of the conversion and
     // the CRT; quartic = 0 and null pointers for the general RNS.
     uint32_t quartic, qk, qdelta, pad_;

@@ -91,6 +91,42 @@ __device__ __forceinline__ void gs_bfly(uint32_t& X, uint32_t& Y, Tw w, uint32_t
     const uint32_t quotient = __umulhi(d, w.shoup);
     Y = d * w.value + quotient * np;
 }
+// Gentleman-Sande on complemented operands.  In the tile's inverse a value
+// whose position is the multiplied (Y) operand of the next stage can be held
+// complemented, Yc = 2p - Y in [0,2p] (congruent to -Y), so that the stage
+// forms X - Y + 2p = X + Yc with one add and X + Y = X - Yc (+2p) with one
+// subtract and a min (7 instructions instead of 8).  The outputs are normal
+// (X', Y' in [0,2p)) when the next stage takes them as they are, or
+// complemented when they are the next stage's Y operands (both outputs of a
+// butterfly share that bit of their position).  wc is the complement twiddle
+// (p - w, its Shoup quotient): Shoup(d, p - w) is congruent to -d*w.
+// Yc in [0,2p], X in [0,2p) -> X', Y' in [0,2p)
+__device__ __forceinline__ void gs_bfly_cn(uint32_t& X, uint32_t& Y, Tw w, uint32_t p, uint32_t p2) {
+    const uint32_t e = X - Y;  // X - Yc in [-2p, 2p), congruent to X + Y
+    const uint32_t d = X + Y;  // X - Y + 2p
+    uint32_t np = 0u - p;
+    asm("" : "+v"(np));
+    X = min(e, e + p2);
+    Y = d * w.value + __umulhi(d, w.shoup) * np;
+}
+// Yc in [0,2p], X in [0,2p) -> Xc', Yc' in [0,2p]
+__device__ __forceinline__ void gs_bfly_cc(uint32_t& X, uint32_t& Y, Tw wc, uint32_t p, uint32_t p2) {
+    const uint32_t f = Y - X;  // Yc - X in (-2p, 2p], congruent to -(X + Y)
+    const uint32_t d = X + Y;
+    uint32_t np = 0u - p;
+    asm("" : "+v"(np));
+    X = min(f, f + p2);
+    Y = d * wc.value + __umulhi(d, wc.shoup) * np;
+}
+// X, Y in [0,2p) -> Xc', Yc' in [0,2p]
+__device__ __forceinline__ void gs_bfly_nc(uint32_t& X, uint32_t& Y, Tw wc, uint32_t p, uint32_t p2) {
+    const uint32_t u = 2u * p2 - (X + Y);  // 4p - (X + Y) in (0, 4p]
+    const uint32_t d = X - Y + p2;
+    uint32_t np = 0u - p;
+    asm("" : "+v"(np));
+    X = csub(u, p2);
+    Y = d * wc.value + __umulhi(d, wc.shoup) * np;
+}
 // T*2^-32 mod p for a 64-bit T below 8p^2 (Montgomery REDC): m = T*(-p^-1)
 // mod 2^32, (T + m*p) / 2^32 with T + m*p < 8p^2 + 2^32 p < 2^64 (p < 2^30),
 // one multiply and one v_mad_u64_u32; the result is below 8p^2/2^32 + p < 3p.
@@ -179,6 +215,18 @@ struct Geo {
             if (need == 0) return pos;
             --need;
         }
+        return -1;
+    }
+    // the register bit holding index bit pos in phase p, or -1 (a thread bit)
+    static constexpr int reg_bit_of(int p, int pos) {
+        for (int j = 0; j < ELOG; ++j)
+            if (reg_pos(p, j) == pos) return j;
+        return -1;
+    }
+    // the phase whose stages run on index bit pos
+    static constexpr int phase_of(int pos) {
+        for (int p = 0; p < NPH; ++p)
+            if (in_range(p, pos)) return p;
         return -1;
     }
     static constexpr uint32_t reg_scatter(int p, int r) {
@@ -333,9 +381,68 @@ __device__ __forceinline__ void fwd_phase2(TileRegs<TLOG, EL>& a, TileRegs<TLOG,
     }
 }
 
+// Complemented operands in the inverse (gs_bfly_cn): stage beta leaves the
+// positions with bit beta+1 set complemented when that bit is a register bit
+// of its phase, so the butterfly knows its mode at compile time (both outputs
+// share the bit); the last stage and a stage whose next bit is a thread bit
+// leave normal values.  The block product complements the positions with bit
+// 2 set for the first stage the same way.
+template <int TLOG, int EL>
+__host__ __device__ constexpr bool inv_comp_after(int beta) {
+    using G = Geo<TLOG, EL>;
+    if (TLOG <= kBlockLog || beta + 1 >= TLOG) return false;
+    return G::reg_bit_of(G::phase_of(beta), beta + 1) >= 0;
+}
+template <int TLOG, int EL>
+__host__ __device__ constexpr bool inv_comp_before(int beta) {
+    using G = Geo<TLOG, EL>;
+    if (TLOG <= kBlockLog) return false;
+    if (beta == kBlockLog) return G::reg_bit_of(G::NPH - 1, kBlockLog) >= 0;
+    return inv_comp_after<TLOG, EL>(beta - 1);
+}
+
+// The stages of inverse phase P from register bit BIT on, one template level
+// per stage, so that each stage's operand modes are compile-time constants.
+template <int TLOG, int EL, int P, int BIT>
+__device__ __forceinline__ void inv_stages(TileRegs<TLOG, EL>& a, const Tw* __restrict__ tw,
+                                           const Tw* __restrict__ twc, uint32_t bm, uint32_t tau_x, uint32_t p,
+                                           uint32_t p2) {
+    using G = Geo<TLOG, EL>;
+    if constexpr (BIT < G::width(P)) {
+        constexpr int beta = G::lo(P) + BIT;
+        if constexpr (beta >= first_stage_bit<TLOG>()) {
+            constexpr bool cin = inv_comp_before<TLOG, EL>(beta);
+            constexpr int jn = inv_comp_after<TLOG, EL>(beta) ? G::reg_bit_of(P, beta + 1) : -1;
+            const uint32_t m = 1u << (TLOG - 1 - beta);
+            int done = 0;
+#pragma unroll
+            for (int r = 0; r < G::E; ++r) {
+                if (r & (1 << BIT)) continue;
+                const uint32_t idx = bm * m + group_of<TLOG, EL, P>(tau_x, r, beta);
+                if (jn >= 0 && ((r >> jn) & 1)) {
+                    const Tw wc = const_twiddle(twc, idx);
+                    if constexpr (cin)
+                        gs_bfly_cc(a[r], a[r | (1 << BIT)], wc, p, p2);
+                    else
+                        gs_bfly_nc(a[r], a[r | (1 << BIT)], wc, p, p2);
+                } else {
+                    const Tw w = const_twiddle(tw, idx);
+                    if constexpr (cin)
+                        gs_bfly_cn(a[r], a[r | (1 << BIT)], w, p, p2);
+                    else
+                        gs_bfly(a[r], a[r | (1 << BIT)], w, p, p2);
+                }
+                if (++done % inverse_twiddle_chunk<TLOG, EL>() == 0) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        inv_stages<TLOG, EL, P, BIT + 1>(a, tw, twc, bm, tau_x, p, p2);
+    }
+}
+
 template <int TLOG, int EL, int P>
 __device__ __forceinline__ void inv_phase(TileRegs<TLOG, EL>& a, const Tw* __restrict__ tw,
-                                          uint32_t bm, uint32_t tau_x, uint32_t p, uint32_t p2) {
+                                          const Tw* __restrict__ twc, uint32_t bm, uint32_t tau_x, uint32_t p,
+                                          uint32_t p2) {
     using G = Geo<TLOG, EL>;
     // the inverse table is indexed like the forward one: without this the
     // compiler keeps the forward phase's twiddle offsets live across the tile
@@ -344,20 +451,7 @@ __device__ __forceinline__ void inv_phase(TileRegs<TLOG, EL>& a, const Tw* __res
         asm volatile("" : "+v"(tau_x));  // bm is scalar here (conv_tile)
     else
         asm volatile("" : "+v"(tau_x), "+v"(bm));
-#pragma unroll
-    for (int bit = 0; bit < G::width(P); ++bit) {
-        const int beta = G::lo(P) + bit;
-        if (beta < first_stage_bit<TLOG>()) continue;
-        const uint32_t m = 1u << (TLOG - 1 - beta);
-        int done = 0;
-#pragma unroll
-        for (int r = 0; r < G::E; ++r) {
-            if (r & (1 << bit)) continue;
-            const Tw w = const_twiddle(tw, bm * m + group_of<TLOG, EL, P>(tau_x, r, beta));
-            gs_bfly(a[r], a[r | (1 << bit)], w, p, p2);
-            if (++done % inverse_twiddle_chunk<TLOG, EL>() == 0) __builtin_amdgcn_sched_barrier(0);
-        }
-    }
+    inv_stages<TLOG, EL, P, 0>(a, tw, twc, bm, tau_x, p, p2);
 }
 
 template <int TLOG, int EL, int P>
@@ -376,7 +470,7 @@ __device__ __forceinline__ void forward_from(TileRegs<TLOG, EL>& a, TileRegs<TLO
 
 template <int TLOG, int EL, int P>
 __device__ __forceinline__ void inverse_from(TileRegs<TLOG, EL>& a, uint32_t* lds, uint32_t tau,
-                                             const Tw* tw, uint32_t bm, uint32_t p, uint32_t p2) {
+                                             const Tw* tw, const Tw* twc, uint32_t bm, uint32_t p, uint32_t p2) {
     using G = Geo<TLOG, EL>;
     if constexpr (P + 1 < G::NPH) {
         // the exchange before this one: the inverse step above, or the last
@@ -385,8 +479,8 @@ __device__ __forceinline__ void inverse_from(TileRegs<TLOG, EL>& a, uint32_t* ld
             P + 2 < G::NPH ? wave_local<TLOG, EL>(P + 2, P + 1) : wave_local<TLOG, EL>(P, P + 1);
         exchange<TLOG, EL, P + 1, P, lead>(a, lds, tau);
     }
-    inv_phase<TLOG, EL, P>(a, tw, bm, thr_scatter<TLOG, EL, P>(tau), p, p2);
-    if constexpr (P > 0) inverse_from<TLOG, EL, P - 1>(a, lds, tau, tw, bm, p, p2);
+    inv_phase<TLOG, EL, P>(a, tw, twc, bm, thr_scatter<TLOG, EL, P>(tau), p, p2);
+    if constexpr (P > 0) inverse_from<TLOG, EL, P - 1>(a, lds, tau, tw, twc, bm, p, p2);
 }
 
 // Product of the blocks of the incomplete transforms, in the last forward
@@ -417,12 +511,17 @@ __device__ __forceinline__ void block_product(TileRegs<TLOG, EL>& a, const TileR
             B[i] = shoup_np(b[r + i], S, np);
             Z[i] = i ? shoup_np(b[r + i], z, np) : 0u;
         }
+        // positions with bit 2 set are the first inverse stage's Y operands:
+        // complemented, csub(3p - t, p) in [0,2p] for t in [0,3p)
+        constexpr int j2 = G::reg_bit_of(P, kBlockLog);
+        const bool comp = inv_comp_before<TLOG, EL>(kBlockLog) && j2 >= 0 && ((r >> j2) & 1);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             uint64_t T = uint64_t(A[0]) * B[k];
 #pragma unroll
             for (int i = 1; i < 4; ++i) T += uint64_t(A[i]) * (i <= k ? B[k - i] : Z[k - i + 4]);
-            a[r + k] = csub(redc64(T, p, neg_inv), p);
+            const uint32_t t = redc64(T, p, neg_inv);
+            a[r + k] = comp ? csub(3u * p - t, p) : csub(t, p);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -493,7 +592,7 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
 #pragma unroll
         for (int r = 0; r < G::E; ++r) a[r] = mont_mul(csub(a[r], p2), csub(b[r], p2), p, neg_inv);
     }
-    inverse_from<TLOG, EL, G::NPH - 1>(a, lds, tau, tb.inv + table, bm, p, p2);
+    inverse_from<TLOG, EL, G::NPH - 1>(a, lds, tau, tb.inv + table, tb.invc + table, bm, p, p2);
     const __amdgpu_buffer_rsrc_t rd =
         __builtin_amdgcn_make_buffer_rsrc(wave_uniform(dst), short(0), int(G::T * 4u), int(kRawBufferWord3));
     // A tile that is a sub-block of a larger transform (bm > 1: outer inverse

@@ -530,6 +530,8 @@ std::string build_quartic_params(uint32_t max_logn, const uint32_t* q_words, Qua
     return std::string();
 }
 
+Twiddle complement_twiddle(Twiddle w, uint32_t p) { return make_twiddle(w.value ? p - w.value : 0u, p); }
+
 void block_product_constants(uint32_t p, uint32_t max_logn, const Twiddle* forward, const Twiddle* scale,
                              size_t scale_stride, Twiddle* scale4, Twiddle* zeta) {
     zeta[0] = make_twiddle(0, p);

@@ -109,6 +109,10 @@ struct QuarticParams {
 // Empty on success, else why q (28 words) is not a supported special modulus.
 std::string build_quartic_params(uint32_t max_logn, const uint32_t* q, QuarticParams& out);
 
+// (p - w, floor((p - w) 2^32 / p)): the twiddle whose Shoup product is
+// congruent to -w times the operand (w in [1, p)).
+Twiddle complement_twiddle(Twiddle w, uint32_t p);
+
 // Constants of the tile's block product for one prime p.  For N = 2^logn >= 4
 // the tile's forward transform stops at blocks of 4 positions, block G
 // holding its operand modulo X^4 - zeta_G, and its inverse runs logn - 2

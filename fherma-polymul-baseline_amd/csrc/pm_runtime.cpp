@@ -175,6 +175,12 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     // forward twiddles carry -w mod 2^32: the butterfly forms q*p - Y*w as q*p + Y*(-w)
     for (size_t i = 0; i < fwd.size(); ++i) fwd[i] = {0u - s.forward[i].value, s.forward[i].shoup};
     for (size_t i = 0; i < inv.size(); ++i) inv[i] = {s.inverse[i].value, s.inverse[i].shoup};
+    // complement twiddles p - w of the inverse (its complemented outputs)
+    std::vector<pmd::Tw> invc(s.inverse.size());
+    for (size_t i = 0; i < invc.size(); ++i) {
+        const pm::Twiddle c = pm::complement_twiddle(s.inverse[i], s.primes[i >> ctx->max_logn]);
+        invc[i] = {c.value, c.shoup};
+    }
     // the tile's scale and block-product constants, per prime
     const uint32_t zrow = 1u << (ctx->max_logn - 1);
     std::vector<pmd::Tw> zs(size_t(s.k) * zrow);
@@ -194,6 +200,7 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     tb.mont = upload(ctx, s.mont_neg_inv, &status);
     tb.fwd = upload(ctx, fwd, &status);
     tb.inv = upload(ctx, inv, &status);
+    tb.invc = upload(ctx, invc, &status);
     tb.scale = upload(ctx, scale, &status);
     tb.zs = upload(ctx, zs, &status);
     tb.conv_w = reinterpret_cast<const uint4*>(upload(ctx, conv_w, &status));
@@ -709,8 +716,8 @@ static int build_quartic(pm_context* ctx) {
     const uint32_t k = s.k, rows = 4 * k, max_n = 1u << ctx->max_logn;
     std::vector<uint32_t> rp(rows), rm(rows);
     const uint32_t zrow = 1u << (ctx->max_logn - 1);
-    std::vector<pmd::Tw> fwd(size_t(rows) * max_n), inv(size_t(rows) * max_n), scale(size_t(ctx->max_logn + 1) * rows),
-        zs(size_t(rows) * zrow);
+    std::vector<pmd::Tw> fwd(size_t(rows) * max_n), inv(size_t(rows) * max_n), invc(size_t(rows) * max_n),
+        scale(size_t(ctx->max_logn + 1) * rows), zs(size_t(rows) * zrow);
     std::vector<pm::Twiddle> s4(ctx->max_logn + 1), z(zrow);
     for (uint32_t row = 0; row < rows; ++row) {
         const uint32_t i = row / 4;
@@ -720,6 +727,8 @@ static int build_quartic(pm_context* ctx) {
             const pm::Twiddle f = s.forward[size_t(i) * max_n + x], g = s.inverse[size_t(i) * max_n + x];
             fwd[size_t(row) * max_n + x] = {0u - f.value, f.shoup};  // the butterfly's negated forward twiddle
             inv[size_t(row) * max_n + x] = {g.value, g.shoup};
+            const pm::Twiddle c = pm::complement_twiddle(g, s.primes[i]);
+            invc[size_t(row) * max_n + x] = {c.value, c.shoup};
         }
         // the tile's scale and block-product constants (the prime's, for each of its channels)
         if (row % 4 == 0)
@@ -740,6 +749,7 @@ static int build_quartic(pm_context* ctx) {
     tb.mont = upload(ctx, rm, &status);
     tb.fwd = upload(ctx, fwd, &status);
     tb.inv = upload(ctx, inv, &status);
+    tb.invc = upload(ctx, invc, &status);
     tb.scale = upload(ctx, scale, &status);
     tb.zs = upload(ctx, zs, &status);
     tb.k = rows;

@@ -17,7 +17,10 @@ span 4 upward undo the forward ones.  This checks:
     wrong scale fails;
   * the bounds the kernel relies on hold for every candidate prime: the
     64-bit REDC sum T + m p stays below 2^64, its result below 3p, and every
-    stage's outputs inside the ranges the next stage assumes.
+    stage's outputs inside the ranges the next stage assumes;
+  * the inverse's complemented operands (gs_bfly_cn / _cc / _nc: a value that
+    is the next stage's Y operand held as 2p - Y) give the same product for
+    any choice of the stages that complement, extremes included.
 """
 import numpy as np
 import pytest
@@ -90,6 +93,36 @@ def gs_bfly(X, Y, w, p):
     return csub(s, 2 * p), shoup_np(d, w, p)
 
 
+def gs_bfly_cn(X, Yc, w, p):
+    """Complemented Y in (Yc = 2p - Y in [0,2p]), normal outputs."""
+    assert X < 2 * p and Yc <= 2 * p
+    e = (X - Yc) & M32
+    d = X + Yc
+    Xo = min(e, (e + 2 * p) & M32)
+    assert Xo < 2 * p
+    return Xo, shoup_np(d, w, p)
+
+
+def gs_bfly_cc(X, Yc, w, p):
+    """Complemented Y in, complemented outputs (twiddle p - w)."""
+    assert X < 2 * p and Yc <= 2 * p
+    f = (Yc - X) & M32
+    d = X + Yc
+    Xo = min(f, (f + 2 * p) & M32)
+    assert Xo <= 2 * p
+    return Xo, shoup_np(d, (p - w) % p, p)
+
+
+def gs_bfly_nc(X, Y, w, p):
+    """Normal inputs, complemented outputs (twiddle p - w)."""
+    assert X < 2 * p and Y < 2 * p
+    u = 4 * p - (X + Y)
+    d = (X - Y + 2 * p) & M32
+    Xo = csub(u, 2 * p)
+    assert Xo <= 2 * p
+    return Xo, shoup_np(d, (p - w) % p, p)
+
+
 def redc64(T, p):
     neg_inv = (-pow(p, -1, 1 << 32)) & M32
     m = (T * neg_inv) & M32
@@ -107,7 +140,10 @@ def zeta_of(g, logn, fwd, p):
     return (p - w) % p if g & 1 else w
 
 
-def tile_product(a, b, p):
+def tile_product(a, b, p, complemented=()):
+    """complemented: the inverse stages beta that leave the positions with bit
+    beta + 1 set complemented (2p - v) for stage beta + 1 (kBlockLog - 1 = 1
+    stands for the block product complementing the positions with bit 2 set)."""
     n = len(a)
     logn = n.bit_length() - 1
     psi = psi_for(p, n)
@@ -132,15 +168,22 @@ def tile_product(a, b, p):
         for k in range(4):
             T = sum(A[i] * (B[k - i] if i <= k else Z[k - i + 4]) for i in range(4))
             assert T < 8 * p * p
-            a[4 * g + k] = csub(redc64(T, p), p)
+            t = redc64(T, p)
+            comp = 1 in complemented and logn > 2 and (4 * g + k) >> 2 & 1
+            a[4 * g + k] = csub(3 * p - t, p) if comp else csub(t, p)
     for beta in range(2, logn):  # inverse stages from span 4 upward
         m = 1 << (logn - 1 - beta)
+        cin = (beta - 1) in complemented
         for x in range(n):
             if x >> beta & 1:
                 continue
             w = inv[m + (x >> (beta + 1))]
             y = x | 1 << beta
-            a[x], a[y] = gs_bfly(a[x], a[y], w, p)
+            cout = beta in complemented and beta + 1 < logn and x >> (beta + 1) & 1
+            bf = {(False, False): gs_bfly, (True, False): gs_bfly_cn,
+                  (True, True): gs_bfly_cc, (False, True): gs_bfly_nc}[(cin, bool(cout))]
+            a[x], a[y] = bf(a[x], a[y], w, p)
+    assert all(v < 2 * p for v in a)  # the last stage leaves normal values
     return [csub(v, p) for v in a]
 
 
@@ -171,6 +214,29 @@ def test_block_product_equals_negacyclic_product(n, which):
         got = tile_product(a, b, p)
         want = negacyclic([v % p for v in a], [v % p for v in b], p)
         assert got == want, (n, p, case)
+
+
+@pytest.mark.parametrize("n", [8, 64, 256])
+@pytest.mark.parametrize("which", [0, -1])
+def test_complemented_inverse_stages_give_the_same_product(n, which):
+    """The kernel complements, per stage, the next stage's Y operands when that
+    bit is held in registers; any such choice of stages (none, all, every
+    other, random) gives the same exact product."""
+    p = PRIMES[which]
+    logn = n.bit_length() - 1
+    rng = np.random.default_rng(n + which)
+    a = [int(v) for v in rng.integers(0, 4 * p, size=n)]
+    b = [int(v) for v in rng.integers(0, 4 * p, size=n)]
+    want = negacyclic([v % p for v in a], [v % p for v in b], p)
+    stages = list(range(1, logn - 1))
+    choices = [(), tuple(stages), tuple(stages[::2]), tuple(stages[1::2]),
+               tuple(s for s in stages if rng.integers(0, 2))]
+    for chosen in choices:
+        assert tile_product(a, b, p, chosen) == want, chosen
+    # extremes: every residue at the top of its range
+    top = [4 * p - 1] * n
+    want_top = negacyclic([v % p for v in top], [v % p for v in top], p)
+    assert tile_product(top, top, p, tuple(stages)) == want_top
 
 
 def test_block_product_bounds_hold_for_every_candidate_prime():
