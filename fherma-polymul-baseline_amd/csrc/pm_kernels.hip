@@ -4,12 +4,17 @@
 //    coefficient bytes against balanced base-256 digits of 256^j mod p_i
 //    (role of rns/tensor_prepare.h:20-71 and rns/solve.cu:138-157);
 //  * fused per-prime convolution tile: Harvey lazy odd-root Cooley-Tukey
-//    forward NTT of a and b, Montgomery pointwise product, Gentleman-Sande
-//    inverse, scale -- spectra stay in registers/LDS (role of
-//    quartic/solve.cu:641-770 and rns/solve.cu:225-333);
-//  * outer radix-2 stages for N > 2^15 (role of quartic/solve.cu:771-817);
-//  * exact CRT with 30-bit limb accumulation, nearest-integer quotient and a
-//    Barrett reduction modulo any odd q (role of rns/solve.cu:336-420).
+//    forward NTTs of a and b stopped at blocks of 4 residues, the scaled
+//    block product modulo X^4 - zeta, the Gentleman-Sande inverse (on
+//    complemented operands where the stage's mode is known at compile time)
+//    -- spectra stay in registers/LDS (role of quartic/solve.cu:641-770 and
+//    rns/solve.cu:225-333);
+//  * outer radix-2/radix-4 stages of N > 2^13 tiles, fused into the
+//    conversion and the CRT or as passes of their own (role of
+//    quartic/solve.cu:771-817);
+//  * exact CRT on int8 MFMA (or 30-bit limb accumulation), nearest-integer
+//    quotient and a Barrett reduction modulo any q > 2^864 (role of
+//    rns/solve.cu:336-420).
 // All arithmetic is exact 32/64-bit integer arithmetic.
 #include <hip/hip_runtime.h>
 
