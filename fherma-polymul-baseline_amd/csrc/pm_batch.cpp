@@ -44,15 +44,16 @@ constexpr double kPartCut[kMaxParts + 1][kMaxParts + 1] = {
 
 // Parts when the context leaves the choice open: 3 where some product runs
 // 2^15 tiles (a 1024-thread block fills a CU, and only blocks of another
-// part's conversion or CRT fill the CUs its tail leaves), else 1 (smaller
-// tiles share a CU four to a CU already).  Measured in A/B runs of one
-// build: 32 x N = 2^17 (2^15 tiles) 3 parts 11.97-12.05k against
-// 11.50-11.59k products/s in one; 128 x N = 2^15 (2^13 tiles) one part
-// 53.8-54.2k against 53.2k; 256 uneven products of 2^12..2^16 alike.
+// part's conversion or CRT fill the CUs its tail leaves), else 2 (the second
+// part's conversion runs beside the first part's tiles, its tiles beside the
+// first part's CRT).  Measured in steady state (50 warm launches, 200 timed):
+// 128 x N = 2^15 (2^13 tiles) 2 parts 62.6k against 61.8-62.0k products/s
+// for 1, 3 alike, 4 slower; 32 x N = 2^17 (2^15 tiles) 3 parts 11.97-12.05k
+// against 11.50-11.59k in one part.
 uint32_t auto_parts(const std::vector<pmd::Job>& jobs) {
     for (const pmd::Job& j : jobs)
         if (j.logn - j.outer == 15) return 3;
-    return 1;
+    return 2;
 }
 
 }  // namespace

@@ -255,7 +255,8 @@ def test_batch_in_staggered_parts(pm, workload):
     """A batch split into 1-4 staggered parts on as many graph branches:
     every product equals the same product on the plain path, on two replays
     in a row (the parts' work lists and branches); the per-batch default
-    splits where a product runs 2^15 tiles, and a part keeps 4 products."""
+    splits in 3 where a product runs 2^15 tiles and in 2 otherwise, and a
+    part keeps 4 products."""
     q = workload.modulus(6)
     ctx = pm.Context(q, 1 << 17)
     ns = [1024, 4096, 32768, 256, 8192, 65536, 2048, 16384] * 3
@@ -265,7 +266,7 @@ def test_batch_in_staggered_parts(pm, workload):
         say("batch in parts", len(ns), parts)
         ctx.set_batch_parts(parts)
         batch = pm.Batch(ctx, ns)
-        assert batch.info()["parts"] == (parts or 1)  # no 2^15 tile here: 2^16 runs two outer stages
+        assert batch.info()["parts"] == (parts or 2)  # no 2^15 tile here (2^16 runs two outer stages): 2 by default
         for i, (a, b) in enumerate(inputs):
             batch.upload(i, a, b)
         for replay in range(2):
