@@ -178,7 +178,7 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     // complement twiddles p - w of the inverse (its complemented outputs)
     std::vector<pmd::Tw> invc(s.inverse.size());
     for (size_t i = 0; i < invc.size(); ++i) {
-        const pm::Twiddle c = pm::complement_twiddle(s.inverse[i], s.primes[i >> ctx->max_logn]);
+        const pm::Twiddle c = pm::complement_twiddle(s.inverse[i], s.primes[i >> s.max_logn]);
         invc[i] = {c.value, c.shoup};
     }
     // the tile's scale and block-product constants, per prime
@@ -187,7 +187,7 @@ int pm_create(uint32_t max_n, uint32_t w_bits, uint32_t limbs, const uint32_t* q
     {
         std::vector<pm::Twiddle> s4(ctx->max_logn + 1), z(zrow);
         for (uint32_t i = 0; i < s.k; ++i) {
-            pm::block_product_constants(s.primes[i], ctx->max_logn, s.forward.data() + (size_t(i) << ctx->max_logn),
+            pm::block_product_constants(s.primes[i], ctx->max_logn, s.forward.data() + (size_t(i) << s.max_logn),
                                         s.scale.data() + i, s.k, s4.data(), z.data());
             for (uint32_t logn = 0; logn <= ctx->max_logn; ++logn)
                 scale[size_t(logn) * s.k + i] = {s4[logn].value, s4[logn].shoup};
