@@ -602,8 +602,9 @@ __device__ __forceinline__ void conv_tile(const Tables& tb, const uint32_t* __re
         __builtin_amdgcn_make_buffer_rsrc(wave_uniform(dst), short(0), int(G::T * 4u), int(kRawBufferWord3));
     // A tile that is a sub-block of a larger transform (bm > 1: outer inverse
     // stages follow, in the CRT or a pass of their own) stores its scaled
-    // values in [0,2p), as those Gentleman-Sande stages take them; they end in
-    // [0,p).  A whole transform stores canonical residues for the CRT.
+    // values in [0,2p), as those Gentleman-Sande stages take them (the CRT's
+    // leave [0,2p), the pass's [0,p)).  A whole transform stores canonical
+    // residues.
     uint32_t np = 0u - p;
     asm volatile("" : "+v"(np));
     const auto store = [&](auto canonical) {
