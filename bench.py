@@ -248,7 +248,7 @@ def run_headline(args):
         done += 1
     cpu_elapsed = time.perf_counter() - t0
     cpu_rate = done / cpu_elapsed
-    # and on the box's host cores: independent products on one thread per core
+    # and on the host's cores: independent products on one thread per core
     # (the oracle's C code runs outside the GIL), each checked the same way
     from concurrent.futures import ThreadPoolExecutor
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
@@ -840,7 +840,7 @@ def main():
                     help="also run a known-size streaming kernel (for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
               "products are independent, so more GPUs would each run their own batch", file=sys.stderr)
         return 2
     if args.workload == "headline":

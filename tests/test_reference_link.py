@@ -2,8 +2,8 @@
 (main.cpp:137-209, which calls fherma_init / fherma_run / fherma_free,
 fherma.h:46-48) compiles unchanged, against the reference's fherma.h and
 against include/fherma.h, and links against libfherma_polymul.so +
-libpolymul.so with every fherma symbol resolved.  Build-container only: the
-reference tree is not present on the GPU box, and nothing here runs it."""
+libpolymul.so with every fherma symbol resolved.  Skipped where the reference
+tree is absent; nothing here runs the reference's program."""
 import os
 import subprocess
 
