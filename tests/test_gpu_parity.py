@@ -251,6 +251,32 @@ def test_product_865_bit_modulus_vs_oracle(pm, orc, workload, n):
         assert (ctx.multiply(a, b) == want).all(), (n, family)
 
 
+def test_random_moduli_sweep(pm, orc, workload):
+    """Twelve random odd moduli of 865 to 868 bits, each with its own
+    context, at sizes 2 to 4096 and a different validator family each, plus
+    the all-(q-1) inputs whose sums are largest: against schoolbook up to
+    N = 256 and the oracle's RNS port with the reference's 31-bit primes above
+    (a prime set unrelated to the library's 30-bit one)."""
+    import random
+    rng = random.Random(20261204)
+    sizes = [2, 8, 32, 64, 256, 1024, 4096]
+    for i in range(12):
+        bits = 865 + i % 4
+        q = workload.from_int((1 << (bits - 1)) | rng.getrandbits(bits - 1) | 1)
+        n = sizes[i % len(sizes)]
+        first = workload.FAMILIES[i % len(workload.FAMILIES)]
+        families = (first, "all-q-minus-one" if first != "all-q-minus-one" else "limb-boundaries")
+        say("random q", i, bits, n, families[0])
+        ctx = pm.Context(q, 4096)
+        try:
+            for family in families:
+                a, b = workload.family(family, n, q, 100 + i)
+                want = orc.schoolbook(q, a, b) if n <= 256 else orc.rns_multiply(q, a, b, orc.primes_reference())
+                assert (ctx.multiply(a, b) == want).all(), (i, bits, n, family)
+        finally:
+            ctx.close()
+
+
 def test_outer_schedules_agree(pm, workload, torch_cuda):
     say("outer schedules")
     case = [c for c in GOLDEN["cases"] if c["n"] == 32768 and c["family"] == "dense-random-1"][0]
