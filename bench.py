@@ -263,6 +263,18 @@ def run_headline(args):
         par_exact = all(pool.map(one, range(sample)))
     par_rate = sample / (time.perf_counter() - t0)
     exact = exact and par_exact
+    # and one product at a time split over the same threads (per-prime units,
+    # then blocks of coefficients for the CRT): its latency, median and p99
+    # over 20 products
+    split_ms = []
+    t0 = time.perf_counter()
+    while len(split_ms) < 20 and (time.perf_counter() - t0) < max(args.cpu_seconds, 1.0):
+        i = len(split_ms) % len(batch_c)
+        a, b = inputs[i]
+        t1 = time.perf_counter()
+        c = orc.rns_multiply_threads(q, a, b, primes, threads)
+        split_ms.append((time.perf_counter() - t1) * 1e3)
+        exact = exact and bool(np.array_equal(c, batch_c[i]))
 
     line = {
         "metric": METRIC,
@@ -304,10 +316,14 @@ def run_headline(args):
         "cpu_baseline": {
             "value": round(par_rate, 4), "unit": "products/s", "cores": threads, "kind": "port",
             "sample": f"{sample} products of the same workload (N={n}), oracle RNS port, one product per thread on "
-                      f"{threads} threads; single thread: {done} products, each timed",
+                      f"{threads} threads; single thread: {done} products, each timed; split: {len(split_ms)} "
+                      f"products, each over {threads} threads by prime, then by coefficient block",
             "single_thread_value": round(cpu_rate, 4),
             "single_thread_product_p50_ms": round(percentile(cpu_ms, 50), 2) if cpu_ms else None,
             "single_thread_product_p99_ms": round(percentile(cpu_ms, 99), 2) if cpu_ms else None,
+            "split_product_p50_ms": round(percentile(split_ms, 50), 2) if split_ms else None,
+            "split_product_p99_ms": round(percentile(split_ms, 99), 2) if split_ms else None,
+            "split_product_reps": len(split_ms),
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
         },
         "bit_exact": bool(exact and gpu_consistent and done > 0 and persistent["equal_to_phased"]),

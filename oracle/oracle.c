@@ -2,7 +2,8 @@
  * oracle.c -- CPU restatement of the reference's algorithm for the exact
  * 868-bit negacyclic product.  TEST INFRASTRUCTURE ONLY: it is the checker
  * that tests/, smoke() and bench.py's cpu_baseline leg compare against; the
- * product path never calls it.  Plain C99, no dependencies.
+ * product path never calls it.  Plain C99 (+ POSIX threads), no other
+ * dependencies.
  *
  * Reference anchors (paths relative to the reference tree):
  *   prime rule, tables, CRT bases ........ rns/host_setup.h:61-126
@@ -17,6 +18,7 @@
  */
 #include "oracle.h"
 
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -327,27 +329,29 @@ static uint32_t brv(uint32_t x, uint32_t bits) {
 /* ------------------------------------------------------------------------ */
 /* stage functions                                                          */
 /* ------------------------------------------------------------------------ */
+/* one prime's row: x_j mod p for every coefficient (rns/solve.cu:138-157) */
+static void residues_row(uint32_t n, const uint32_t* x, uint32_t p, uint32_t* row) {
+    tw_t powers[L];
+    uint32_t base = (uint32_t)((1ull << 32) % p), power = 1;
+    for (int limb = 0; limb < L; ++limb) {
+        powers[limb] = twiddle(power, p);
+        power = mulmod(power, base, p);
+    }
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t* w = x + (size_t)j * L;
+        uint32_t even = 0, odd = 0;
+        for (int limb = 0; limb < L; ++limb) {
+            uint32_t term = shoup(w[limb], powers[limb], p);
+            if (limb & 1) odd = addmod(odd, term, p);
+            else even = addmod(even, term, p);
+        }
+        row[j] = addmod(even, odd, p);
+    }
+}
+
 void orc_residues(uint32_t n, const uint32_t* x, uint32_t k, const uint32_t* primes,
                   uint32_t* out) {
-    for (uint32_t i = 0; i < k; ++i) {
-        const uint32_t p = primes[i];
-        tw_t powers[L];
-        uint32_t base = (uint32_t)((1ull << 32) % p), power = 1;
-        for (int limb = 0; limb < L; ++limb) {
-            powers[limb] = twiddle(power, p);
-            power = mulmod(power, base, p);
-        }
-        for (uint32_t j = 0; j < n; ++j) {
-            const uint32_t* w = x + (size_t)j * L;
-            uint32_t even = 0, odd = 0;
-            for (int limb = 0; limb < L; ++limb) {
-                uint32_t term = shoup(w[limb], powers[limb], p);
-                if (limb & 1) odd = addmod(odd, term, p);
-                else even = addmod(even, term, p);
-            }
-            out[(size_t)i * n + j] = addmod(even, odd, p);
-        }
-    }
+    for (uint32_t i = 0; i < k; ++i) residues_row(n, x, primes[i], out + (size_t)i * n);
 }
 
 /* tw[i] = psi^brv(i) for i in [0,n) */
@@ -509,63 +513,138 @@ static void cyclic_dit(uint32_t* x, uint32_t n, uint32_t p, const tw_t* powers /
     }
 }
 
-int orc_rns_multiply(uint32_t n, const uint32_t q[L], uint32_t k, const uint32_t* primes,
-                     const uint32_t* a, const uint32_t* b, uint32_t* c) {
+/* The product as two kinds of independent units: per prime, conversion of
+ * a and b, twist, forward NTTs, pointwise product, inverse, scale (row i of
+ * ra); then per block of coefficients, the CRT.  One thread runs them in
+ * order; orc_rns_multiply_threads spreads each kind over threads. */
+typedef struct {
+    uint32_t n, logn, k, threads, index;
+    const uint32_t* q;
+    const uint32_t* primes;
+    const uint32_t* a;
+    const uint32_t* b;
+    uint32_t* c;
+    const crt_t* s;
+    uint32_t* ra;
+    uint32_t* rb;
+    int failed;
+} rns_job_t;
+
+static int rns_prime_unit(const rns_job_t* job, uint32_t i, uint32_t* tmp, tw_t* fwd, tw_t* inv, tw_t* scale) {
+    const uint32_t n = job->n, logn = job->logn, p = job->primes[i];
+    const uint32_t psi = orc_find_psi(p, n);
+    const uint32_t ipsi = orc_pow_mod(psi, p - 2, p);
+    const uint32_t inv_n = orc_pow_mod(n, p - 2, p);
+    const uint64_t recip = UINT64_MAX / p;
+    uint32_t f = 1, bk = 1, sc = mulmod(inv_n, job->s->basis_inverse[i], p);
+    uint32_t* xa = job->ra + (size_t)i * n;
+    uint32_t* xb = job->rb + (size_t)i * n;
+    residues_row(n, job->a, p, xa);
+    residues_row(n, job->b, p, xb);
+    for (uint32_t e = 0; e < n; ++e) {
+        fwd[e] = twiddle(f, p);
+        inv[e] = twiddle(bk, p);
+        scale[e] = twiddle(sc, p);
+        f = mulmod(f, psi, p);
+        bk = mulmod(bk, ipsi, p);
+        sc = mulmod(sc, ipsi, p);
+    }
+    /* twist by psi^j and store bit-reversed (rns/solve.cu:156) */
+    for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = shoup(xa[j], fwd[j], p);
+    memcpy(xa, tmp, sizeof(uint32_t) * n);
+    for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = shoup(xb[j], fwd[j], p);
+    memcpy(xb, tmp, sizeof(uint32_t) * n);
+    cyclic_dit(xa, n, p, fwd);
+    cyclic_dit(xb, n, p, fwd);
+    /* pointwise Barrett product scattered bit-reversed (rns/solve.cu:327-333) */
+    for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = barrett(xa[j], xb[j], p, recip);
+    cyclic_dit(tmp, n, p, inv);
+    /* scale = N^-1 (M_i mod p_i)^-1 psi^-j (rns/host_setup.h:111-118) */
+    for (uint32_t j = 0; j < n; ++j) xa[j] = shoup(tmp[j], scale[j], p);
+    return 0;
+}
+
+/* primes index, index + threads, ... */
+static void* rns_prime_units(void* arg) {
+    rns_job_t* job = (rns_job_t*)arg;
+    const uint32_t n = job->n;
+    uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    tw_t* tw = (tw_t*)malloc(sizeof(tw_t) * 3 * (size_t)n);
+    if (!tmp || !tw) {
+        job->failed = 1;
+    } else {
+        for (uint32_t i = job->index; i < job->k; i += job->threads)
+            rns_prime_unit(job, i, tmp, tw, tw + n, tw + 2 * (size_t)n);
+    }
+    free(tmp);
+    free(tw);
+    return NULL;
+}
+
+/* block `index` of `threads` equal coefficient ranges */
+static void* rns_crt_units(void* arg) {
+    rns_job_t* job = (rns_job_t*)arg;
+    const uint32_t n = job->n, per = (n + job->threads - 1) / job->threads;
+    const uint32_t j0 = job->index * per, j1 = j0 + per < n ? j0 + per : n;
+    uint32_t t[ORC_MAX_PRIMES];
+    for (uint32_t j = j0; j < j1; ++j) {
+        for (uint32_t i = 0; i < job->k; ++i) t[i] = job->ra[(size_t)i * n + j];
+        crt_one(job->s, t, job->q, job->c + (size_t)j * L);
+    }
+    return NULL;
+}
+
+/* runs fn over `threads` copies of base (index 0..threads-1): thread 0 on
+ * the caller, the rest on their own threads; 0 if every unit ran */
+static int run_units(void* (*fn)(void*), const rns_job_t* base, uint32_t threads) {
+    rns_job_t jobs[ORC_MAX_THREADS];
+    pthread_t ids[ORC_MAX_THREADS];
+    int started[ORC_MAX_THREADS] = {0};
+    int failed = 0;
+    for (uint32_t t = 0; t < threads; ++t) {
+        jobs[t] = *base;
+        jobs[t].index = t;
+        jobs[t].failed = 0;
+    }
+    for (uint32_t t = 1; t < threads; ++t) started[t] = pthread_create(&ids[t], NULL, fn, &jobs[t]) == 0;
+    fn(&jobs[0]);
+    for (uint32_t t = 1; t < threads; ++t) {
+        if (started[t]) pthread_join(ids[t], NULL);
+        else fn(&jobs[t]); /* no thread: run the unit here */
+    }
+    for (uint32_t t = 0; t < threads; ++t) failed |= jobs[t].failed;
+    return failed ? -3 : 0;
+}
+
+int orc_rns_multiply_threads(uint32_t n, const uint32_t q[L], uint32_t k, const uint32_t* primes,
+                             const uint32_t* a, const uint32_t* b, uint32_t* c, uint32_t threads) {
     uint32_t logn = 0;
     while ((1u << logn) < n) ++logn;
-    if (!n || (1u << logn) != n || k > ORC_MAX_PRIMES) return -1;
+    if (!n || (1u << logn) != n || k > ORC_MAX_PRIMES || threads < 1 || threads > ORC_MAX_THREADS) return -1;
     if (!orc_crt_bound_ok(n, q, k, primes)) return -2;
     crt_t* s = (crt_t*)malloc(sizeof(crt_t));
     uint32_t* ra = (uint32_t*)malloc(sizeof(uint32_t) * n * k);
     uint32_t* rb = (uint32_t*)malloc(sizeof(uint32_t) * n * k);
-    uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * n);
-    tw_t* fwd = (tw_t*)malloc(sizeof(tw_t) * n);
-    tw_t* inv = (tw_t*)malloc(sizeof(tw_t) * n);
-    tw_t* scale = (tw_t*)malloc(sizeof(tw_t) * n);
-    if (!s || !ra || !rb || !tmp || !fwd || !inv || !scale) {
-        free(s); free(ra); free(rb); free(tmp); free(fwd); free(inv); free(scale);
-        return -3;
-    }
-    crt_setup(q, k, primes, s);
-    orc_residues(n, a, k, primes, ra);
-    orc_residues(n, b, k, primes, rb);
-    for (uint32_t i = 0; i < k; ++i) {
-        const uint32_t p = primes[i];
-        const uint32_t psi = orc_find_psi(p, n);
-        const uint32_t ipsi = orc_pow_mod(psi, p - 2, p);
-        const uint32_t inv_n = orc_pow_mod(n, p - 2, p);
-        const uint64_t recip = UINT64_MAX / p;
-        uint32_t f = 1, bk = 1, sc = mulmod(inv_n, s->basis_inverse[i], p);
-        for (uint32_t e = 0; e < n; ++e) {
-            fwd[e] = twiddle(f, p);
-            inv[e] = twiddle(bk, p);
-            scale[e] = twiddle(sc, p);
-            f = mulmod(f, psi, p);
-            bk = mulmod(bk, ipsi, p);
-            sc = mulmod(sc, ipsi, p);
+    int rc = -3;
+    if (s && ra && rb) {
+        crt_setup(q, k, primes, s);
+        rns_job_t job = {n, logn, k, 0, 0, q, primes, a, b, c, s, ra, rb, 0};
+        const uint32_t prime_threads = threads < k ? threads : k;
+        const uint32_t crt_threads = threads < n ? threads : n;
+        job.threads = prime_threads;
+        rc = run_units(rns_prime_units, &job, prime_threads);
+        if (rc == 0) {
+            job.threads = crt_threads;
+            rc = run_units(rns_crt_units, &job, crt_threads);
         }
-        uint32_t* xa = ra + (size_t)i * n;
-        uint32_t* xb = rb + (size_t)i * n;
-        /* twist by psi^j and store bit-reversed (rns/solve.cu:156) */
-        for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = shoup(xa[j], fwd[j], p);
-        memcpy(xa, tmp, sizeof(uint32_t) * n);
-        for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = shoup(xb[j], fwd[j], p);
-        memcpy(xb, tmp, sizeof(uint32_t) * n);
-        cyclic_dit(xa, n, p, fwd);
-        cyclic_dit(xb, n, p, fwd);
-        /* pointwise Barrett product scattered bit-reversed (rns/solve.cu:327-333) */
-        for (uint32_t j = 0; j < n; ++j) tmp[brv(j, logn)] = barrett(xa[j], xb[j], p, recip);
-        cyclic_dit(tmp, n, p, inv);
-        /* scale = N^-1 (M_i mod p_i)^-1 psi^-j (rns/host_setup.h:111-118) */
-        for (uint32_t j = 0; j < n; ++j) xa[j] = shoup(tmp[j], scale[j], p);
     }
-    uint32_t t[ORC_MAX_PRIMES];
-    for (uint32_t j = 0; j < n; ++j) {
-        for (uint32_t i = 0; i < k; ++i) t[i] = ra[(size_t)i * n + j];
-        crt_one(s, t, q, c + (size_t)j * L);
-    }
-    free(s); free(ra); free(rb); free(tmp); free(fwd); free(inv); free(scale);
-    return 0;
+    free(s); free(ra); free(rb);
+    return rc;
+}
+
+int orc_rns_multiply(uint32_t n, const uint32_t q[L], uint32_t k, const uint32_t* primes,
+                     const uint32_t* a, const uint32_t* b, uint32_t* c) {
+    return orc_rns_multiply_threads(n, q, k, primes, a, b, c, 1);
 }
 
 /* ------------------------------------------------------------------------ */

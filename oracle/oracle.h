@@ -96,6 +96,14 @@ int orc_crt_constants(const uint32_t q[ORC_LIMBS], uint32_t k, const uint32_t* p
 int orc_rns_multiply(uint32_t n, const uint32_t q[ORC_LIMBS], uint32_t k,
                      const uint32_t* primes, const uint32_t* a,
                      const uint32_t* b, uint32_t* c);
+/* The same product with its units spread over `threads` threads (1 to
+ * ORC_MAX_THREADS): per prime (conversion, NTTs, pointwise, inverse, scale),
+ * then per block of coefficients (CRT).  Word-for-word the result of
+ * orc_rns_multiply. */
+#define ORC_MAX_THREADS 256
+int orc_rns_multiply_threads(uint32_t n, const uint32_t q[ORC_LIMBS], uint32_t k,
+                             const uint32_t* primes, const uint32_t* a,
+                             const uint32_t* b, uint32_t* c, uint32_t threads);
 
 /* special modulus q = 2^868 - delta, 0 < delta < 2^28: the quartic RNS
  * product (quartic/host_setup.h:41-110, quartic/solve.cu:289-316,906-1040).

@@ -44,7 +44,7 @@ def sha(path):
 def main():
     with tempfile.TemporaryDirectory() as tmp:
         exe = Path(tmp) / "gen_golden"
-        subprocess.run(["gcc", "-O2", "-std=c99", f"-I{ROOT/'oracle'}", f"-I{GMP}/include",
+        subprocess.run(["gcc", "-O2", "-std=c99", "-D_POSIX_C_SOURCE=200809L", "-pthread", f"-I{ROOT/'oracle'}", f"-I{GMP}/include",
                         str(HERE / "gen_golden.c"), str(ROOT / "oracle" / "oracle.c"),
                         f"-L{GMP}/lib", "-lgmp", f"-Wl,-rpath,{GMP}/lib", "-o", str(exe)],
                        check=True)

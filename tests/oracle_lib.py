@@ -38,6 +38,7 @@ class Oracle:
         lib.orc_crt.argtypes = [u32, U32, u32, U32, U32, U32]
         lib.orc_crt_constants.argtypes = [U32, u32, U32, U32, U32, U32]
         lib.orc_rns_multiply.argtypes = [u32, U32, u32, U32, U32, U32, U32]
+        lib.orc_rns_multiply_threads.argtypes = [u32, U32, u32, U32, U32, U32, U32, u32]
         self.lib = lib
 
     @staticmethod
@@ -138,6 +139,16 @@ class Oracle:
         c = np.zeros(n * LIMBS, np.uint32)
         rc = self.lib.orc_rns_multiply(n, self._u32(q), len(primes), self._u32(primes),
                                        self._u32(a).ravel(), self._u32(b).ravel(), c)
+        assert rc == 0, rc
+        return c.reshape(n, LIMBS)
+
+    def rns_multiply_threads(self, q, a, b, primes, threads):
+        """rns_multiply with its per-prime and per-coefficient-block units on
+        `threads` threads (word-for-word the same result)."""
+        n = a.shape[0]
+        c = np.zeros(n * LIMBS, np.uint32)
+        rc = self.lib.orc_rns_multiply_threads(n, self._u32(q), len(primes), self._u32(primes),
+                                               self._u32(a).ravel(), self._u32(b).ravel(), c, threads)
         assert rc == 0, rc
         return c.reshape(n, LIMBS)
 

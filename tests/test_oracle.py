@@ -120,6 +120,26 @@ def test_golden_rns_oracle(orc, workload, n):
         assert sha(c) == case["sha256"]["c"], (n, case["family"])
 
 
+@pytest.mark.parametrize("n,threads", [(2, 5), (64, 3), (4096, 8), (32768, 16)])
+def test_threaded_rns_oracle(orc, workload, n, threads):
+    """The oracle product split over threads (per-prime units, then CRT
+    blocks; more threads than coefficients at N = 2) equals the one-thread
+    product word for word, equals schoolbook at N = 64 and the GMP fixture at
+    N = 2^15."""
+    q = workload.modulus(3)
+    a, b = workload.family("dense-random-1", n, q, 7)
+    primes = orc.reference_primes_for(n, q)
+    got = orc.rns_multiply_threads(q, a, b, primes, threads)
+    assert (got == orc.rns_multiply(q, a, b, primes)).all()
+    if n <= 64:
+        assert (got == orc.schoolbook(q, a, b)).all()
+    if n == 32768:
+        case = [c for c in GOLDEN["cases"] if c["n"] == n and c["family"] == "dense-random-1"][0]
+        q = modulus_of(case["modulus"], workload)
+        a, b = workload.family(case["family"], n, q, case["seed"])
+        assert sha(orc.rns_multiply_threads(q, a, b, orc.reference_primes_for(n, q), threads)) == case["sha256"]["c"]
+
+
 def test_general_primes(orc, workload):
     q = workload.modulus(1)
     primes = orc.primes_general(1 << 17, q)
