@@ -13,6 +13,8 @@
  *                            the reference's published validator families.
  *   orc_rns_multiply      -- equals orc_schoolbook on small N, equals the
  *                            GMP Kronecker SHA-256 fixtures at N up to 2^17.
+ *   orc_rns_multiply_threads -- equals orc_rns_multiply word for word, and
+ *                            orc_schoolbook and a GMP fixture directly.
  *   orc_primes_reference  -- reproduces results/rns-arithmetic.json and the
  *                            prime list of rns/host_setup.h (oracle/_ref).
  *   orc_ntt_forward/inverse -- direct evaluation at odd powers of psi for
