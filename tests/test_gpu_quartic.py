@@ -154,6 +154,36 @@ def test_quartic_batch_matches_general(pm, workload):
         ctx.close()
 
 
+def test_quartic_random_deltas(pm, orc, workload):
+    """Deltas other than the reference's (1, 3, five random ones below
+    2^28): each a plan of its own primes; quartic products against schoolbook
+    at N = 4 and 64 and against the oracle's quartic restatement and the
+    general backend at N = 4096."""
+    import random
+    rng = random.Random(2868)
+    deltas = [1, 3] + [rng.randrange(1, 1 << 28) | 1 for _ in range(5)]
+    for j, delta in enumerate(deltas):
+        q = workload.special_modulus(delta)
+        ctx = pm.Context(q, 4096)
+        try:
+            ctx.set_backend("quartic")
+            primes = pm.plan_quartic(q, 4096)[1]
+            for i, n in enumerate((4, 64, 4096)):
+                family = ("dense-random-1", "all-q-minus-one", "limb-boundaries", "wrap-sign")[(j + i) % 4]
+                say("quartic delta", delta, n, family)
+                a, b = workload.family(family, n, q, 60 + j)
+                got = ctx.multiply(a, b)
+                if n <= 64:
+                    assert (got == orc.schoolbook(q, a, b)).all(), (delta, n, family)
+                else:
+                    assert (got == orc.quartic_multiply(q, a, b, primes)).all(), (delta, n, family)
+                    ctx.set_backend("general")
+                    assert (ctx.multiply(a, b) == got).all(), (delta, n, family, "general")
+                    ctx.set_backend("quartic")
+        finally:
+            ctx.close()
+
+
 def test_quartic_serving(pm, workload):
     q = workload.special_modulus(DELTA)
     ctx = pm.Context(q, 1 << 14)
