@@ -98,10 +98,13 @@ def test_verify_flags_a_wrong_output(tmp_path, pm):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,delta,backend", [(32, None, None), (1024, None, None), (4096, 11993087, None),
-                                             (32, 11993087, "quartic"), (4096, 11993087, "quartic")])
+                                             (32768, None, None), (32, 11993087, "quartic"),
+                                             (4096, 11993087, "quartic"), (32768, 11993087, "quartic")])
 def test_solution_passes_validation(tmp_path, n, delta, backend):
     """The drop-in `solution` over the validator's cases: every case exact,
-    also with the quartic backend chosen through PM_BACKEND."""
+    also with the quartic backend chosen through PM_BACKEND; at N = 2^15 the
+    validator's modulus rule gives the reference's published q = 2^868 -
+    11993087."""
     print("validate", n, delta, backend, flush=True)
     assert vc.main(["--n", str(n), "--dir", str(tmp_path)] + (["--delta", str(delta)] if delta else [])
                    + (["--backend", backend] if backend else [])) == 0
