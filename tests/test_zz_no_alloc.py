@@ -48,3 +48,32 @@ def test_noalloc_matches_device_product(workload):
     assert st == pm.PM_EUNSUPPORTED
     assert b"static scratch" in lib.pm_last_error(ctx.handle)
     ctx.close()
+
+
+def test_noalloc_quartic_backend(workload):
+    """The same check with the special-modulus backend selected (64 channel
+    rows of residues per operand: 2 x 64 x 2^15 words fill the scratch
+    exactly), two calls in a row, the second smaller."""
+    pm = load_package()
+    lib = pm.library()
+    q = workload.special_modulus(11993087)
+    ctx = pm.Context(q, 1 << 15)
+    ctx.set_backend("quartic")
+    stream = torch.cuda.Stream()
+    for n, seed in ((32768, 3), (2048, 4)):
+        say("noalloc quartic", n)
+        a, b = workload.family("all-q-minus-one" if n == 2048 else "dense-random-2", n, q, seed)
+        da = torch.from_numpy(a.view(np.int32)).cuda()
+        db = torch.from_numpy(b.view(np.int32)).cuda()
+        c_ref = torch.zeros_like(da)
+        c_new = torch.full_like(da, -1)
+        torch.cuda.synchronize()
+        st = lib.pm_multiply_device(ctx.handle, n, da.data_ptr(), db.data_ptr(), c_ref.data_ptr(),
+                                    stream.cuda_stream)
+        assert st == pm.PM_OK
+        st = lib.pm_multiply_device_noalloc(ctx.handle, n, da.data_ptr(), db.data_ptr(), c_new.data_ptr(),
+                                            stream.cuda_stream)
+        assert st == pm.PM_OK, lib.pm_last_error(ctx.handle)
+        stream.synchronize()
+        assert np.array_equal(c_new.cpu().numpy().view(np.uint32), c_ref.cpu().numpy().view(np.uint32)), n
+    ctx.close()
