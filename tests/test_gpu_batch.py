@@ -80,6 +80,34 @@ def test_batch_uneven_sizes_phased(pm, orc, workload):
     ctx.close()
 
 
+def test_batch_random_moduli(pm, orc, workload):
+    """Batches under random odd moduli of 865 and 867 bits (the CRT's Barrett
+    finish at the low end of its range), sizes 2^5..2^15 with the validator's
+    families mixed in, against the oracle's RNS port with the reference's own
+    primes (schoolbook at N = 32), on both batch modes."""
+    import random
+    rng = random.Random(1865)
+    ns = [32768, 32, 4096, 256, 16384, 1024, 8192]
+    for bits in (865, 867):
+        q = workload.from_int((1 << (bits - 1)) | rng.getrandbits(bits - 1) | 1)
+        ctx = pm.Context(q, 1 << 15)
+        ops = [workload.family(workload.FAMILIES[(i + bits) % len(workload.FAMILIES)], n, q, 500 + i)
+               for i, n in enumerate(ns)]
+        want = [orc.schoolbook(q, a, b) if n <= 32 else orc.rns_multiply(q, a, b, orc.reference_primes_for(n, q))
+                for n, (a, b) in zip(ns, ops)]
+        for mode in ("phased", "persistent-grid"):
+            say("batch random q", bits, mode)
+            batch = pm.Batch(ctx, ns, mode="persistent" if mode == "persistent-grid" else mode)
+            for i, (a, b) in enumerate(ops):
+                batch.upload(i, a, b)
+            batch.run()
+            batch.wait()
+            for i, n in enumerate(ns):
+                assert (batch.download(i) == want[i]).all(), (bits, mode, n)
+            batch.close()
+        ctx.close()
+
+
 PERSISTENT_OFF = ("an earlier persistent kernel (claims and per-product waits, since removed) hung the GPU on "
                   "this batch after a change to the tile code it shared; no cause was found, so the case stays off "
                   "the GPU.  PM_BATCH_PERSISTENT now names a grid-barrier kernel, covered by "
